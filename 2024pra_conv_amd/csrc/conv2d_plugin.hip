@@ -1,0 +1,337 @@
+// Host side of the plugin: geometry validation, tile-configuration choice, launch-slot
+// filling and the additive C-ABI entry points.
+//
+// Reference counterpart: getParamsize / getkernelInfo (src/conv2d.cpp:9-14, 254-517), which
+// pick one of six hard-coded kernel quadruples by (n, c) alone and never check the other
+// dimensions. Here every dimension is validated, the Winograd geometry is derived once and
+// appended to the kernel argument, and the GEMM tile configuration comes from the tuned
+// table (see tuner.hip) or, failing that, from the shape.
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "conv2d.h"
+#include "mykernelParamType.h"
+#include "wino_common.h"
+#include "wino_gemm.h"
+#include "plugin_internal.h"
+
+namespace wino {
+
+int derive_geometry(const problem_t* pb, Geometry* g) {
+    if (!pb || !g) return CONV2D_ERR_NULL;
+    memset(g, 0, sizeof(*g));
+    if (pb->n == 0 || pb->c == 0 || pb->h == 0 || pb->w == 0 || pb->k == 0 || pb->r == 0 ||
+        pb->s == 0 || pb->u == 0 || pb->v == 0)
+        return CONV2D_ERR_INVALID;
+    const long long hp = (long long)pb->h + 2LL * pb->p, wp = (long long)pb->w + 2LL * pb->q;
+    if (hp < pb->r || wp < pb->s) return CONV2D_ERR_INVALID;
+    const long long Oh = (hp - pb->r) / pb->u + 1;
+    const long long Ow = (wp - pb->s) / pb->v + 1;
+    const unsigned lim = 32767u;
+    if (pb->n > lim || pb->c > lim || pb->h > lim || pb->w > lim || pb->k > lim || pb->r > lim ||
+        pb->s > lim || pb->u > lim || pb->v > lim || pb->p > lim || pb->q > lim || Oh > lim ||
+        Ow > lim)
+        return CONV2D_ERR_OVERFLOW;
+    g->Oh = (int)Oh;
+    g->Ow = (int)Ow;
+    g->winograd_ok = pb->r == 3 && pb->s == 3 && pb->u == 1 && pb->v == 1 && pb->p <= 2 &&
+                     pb->q <= 2;
+    g->tiles_h = (g->Oh + 1) / 2;
+    g->tiles_w = (g->Ow + 1) / 2;
+    g->T = (long long)g->tiles_h * g->tiles_w;
+    g->NT = g->T * pb->n;
+    if (g->NT > (1LL << 30)) return CONV2D_ERR_OVERFLOW;
+    g->G16 = (g->NT + 15) / 16;
+    g->CB = ((int)pb->c + 31) / 32;
+    g->KB = ((int)pb->k + 15) / 16;
+    g->v_bytes = (size_t)16 * g->CB * (size_t)g->G16 * kFragHalves * 2;
+    g->u_bytes = (size_t)16 * g->CB * (size_t)g->KB * kFragHalves * 2;
+    // input-transform band: largest of 8/4/2/1 tile rows whose LDS staging fits in 64 KiB,
+    // preferring 4 (keeps several workgroups per CU)
+    g->band = 0;
+    const int cands[4] = {4, 2, 1, 8};
+    for (int i = 0; i < 4 && !g->band; ++i) {
+        const int b = cands[i];
+        if (b > 1 && b > g->tiles_h) continue;
+        if (input_lds_bytes(b, g->tiles_w, (int)pb->w, (int)pb->q) <= 65536) g->band = b;
+    }
+    return CONV2D_OK;
+}
+
+int require_winograd(const problem_t* pb, Geometry* g) {
+    int rc = derive_geometry(pb, g);
+    if (rc) return rc;
+    if (!g->winograd_ok) return CONV2D_ERR_UNSUPPORTED;
+    if (!g->band) return CONV2D_ERR_UNSUPPORTED;  // image row too wide for LDS staging
+    return CONV2D_OK;
+}
+
+int heuristic_config(const problem_t* pb, const Geometry& g) {
+    (void)g;
+    const int kp = ((int)pb->k + 15) / 16 * 16;
+    if (kp >= 128) return 0;
+    if (kp >= 64) return 1;
+    if (kp >= 32) return 4;
+    return 6;
+}
+
+int select_config(const problem_t* pb, const Geometry& g) {
+    int cfg = tuned_config_lookup(pb);
+    if (cfg >= 0 && cfg < kNumGemmConfigs) return cfg;
+    return heuristic_config(pb, g);
+}
+
+void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {
+    memset(a, 0, sizeof(*a));
+    a->pin = pb->in;
+    a->pweight = pb->weight;
+    a->pout = pb->out;
+    a->transform_pin = pb->transform_pin;
+    a->transform_filter = pb->transform_filter;
+    a->mid_result = pb->mid_result;
+    a->n = (short)pb->n;
+    a->c = (short)pb->c;
+    a->h = (short)pb->h;
+    a->w = (short)pb->w;
+    a->k = (short)pb->k;
+    a->r = (short)pb->r;
+    a->s = (short)pb->s;
+    a->u = (short)pb->u;
+    a->v = (short)pb->v;
+    a->p = (short)pb->p;
+    a->q = (short)pb->q;
+    a->Oh = (short)g.Oh;
+    a->Ow = (short)g.Ow;
+    a->tiles_h = g.tiles_h;
+    a->tiles_w = g.tiles_w;
+    a->T = (int)g.T;
+    a->NT = (int)g.NT;
+    a->G16 = (int)g.G16;
+    a->CB = g.CB;
+    a->KB = g.KB;
+    a->band = g.band;
+    a->cfg = cfg;
+    a->flags = ((pb->w % 8) == 0 && (((uintptr_t)pb->in) & 15) == 0) ? 1 : 0;
+}
+
+Launch slot_filter(const Geometry& g) {
+    Launch L;
+    L.kernel = (const void*)&wino_filter_transform;
+    const long long threads = (long long)g.KB * g.CB * 64;
+    L.grid = dim3((unsigned)((threads + 255) / 256), 1, 1);
+    L.block = dim3(256, 1, 1);
+    L.lds = 0;
+    return L;
+}
+
+Launch slot_input(const problem_t* pb, const Geometry& g) {
+    Launch L;
+    L.kernel = (const void*)&wino_input_transform;
+    L.grid = dim3((unsigned)((g.tiles_h + g.band - 1) / g.band), (unsigned)g.CB, pb->n);
+    L.block = dim3(256, 1, 1);
+    L.lds = (unsigned)input_lds_bytes(g.band, g.tiles_w, (int)pb->w, (int)pb->q);
+    return L;
+}
+
+Launch slot_gemm(const Geometry& g, int cfg) {
+    const GemmConfig& c = kGemmConfigs[cfg];
+    Launch L;
+    L.kernel = c.kernel;
+    const int vf = c.bt / 16, uf = c.bk / 16;
+    L.grid = dim3((unsigned)((g.G16 + vf - 1) / vf), (unsigned)((g.KB + uf - 1) / uf), 1);
+    L.block = dim3((unsigned)c.threads, 1, 1);
+    L.lds = 0;
+    return L;
+}
+
+Launch slot_noop() {
+    Launch L;
+    L.kernel = (const void*)&wino_noop;
+    L.grid = dim3(1, 1, 1);
+    L.block = dim3(64, 1, 1);
+    L.lds = 0;
+    return L;
+}
+
+int launch(const Launch& L, mykernelParamType* a, hipStream_t stream) {
+    void* args[1] = {(void*)a};
+    hipError_t e = hipExtLaunchKernel(L.kernel, L.grid, L.block, args, L.lds, stream, nullptr,
+                                      nullptr, 0);
+    return e == hipSuccess ? CONV2D_OK : CONV2D_ERR_HIP;
+}
+
+}  // namespace wino
+
+using namespace wino;
+
+static void put_slot(const Launch& L, unsigned* bx, unsigned* by, unsigned* bz, unsigned* tx,
+                     unsigned* ty, unsigned* tz, unsigned* lds, void** ptr) {
+    *bx = L.grid.x;
+    *by = L.grid.y;
+    *bz = L.grid.z;
+    *tx = L.block.x;
+    *ty = L.block.y;
+    *tz = L.block.z;
+    *lds = L.lds;
+    *ptr = (void*)L.kernel;
+}
+
+extern "C" {
+
+int getParamsize(problem_t* problem, int* paramSize) {
+    if (!paramSize) return CONV2D_ERR_NULL;
+    (void)problem;
+    *paramSize = (int)sizeof(mykernelParamType);
+    return CONV2D_OK;
+}
+
+int getkernelInfo(problem_t* problem, kernelInfo_t* ki, void* param) {
+    if (!problem || !ki || !param) return CONV2D_ERR_NULL;
+    if (!problem->in || !problem->weight || !problem->out || !problem->transform_pin ||
+        !problem->transform_filter)
+        return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = require_winograd(problem, &g);
+    if (rc) return rc;
+    const int cfg = select_config(problem, g);
+    fill_param(problem, g, cfg, (mykernelParamType*)param);
+    put_slot(slot_filter(g), &ki->blockx1, &ki->blocky1, &ki->blockz1, &ki->threadx1,
+             &ki->thready1, &ki->threadz1, &ki->dynmicLdsSize1, &ki->kernelPtr1);
+    put_slot(slot_input(problem, g), &ki->blockx2, &ki->blocky2, &ki->blockz2, &ki->threadx2,
+             &ki->thready2, &ki->threadz2, &ki->dynmicLdsSize2, &ki->kernelPtr2);
+    put_slot(slot_gemm(g, cfg), &ki->blockx3, &ki->blocky3, &ki->blockz3, &ki->threadx3,
+             &ki->thready3, &ki->threadz3, &ki->dynmicLdsSize3, &ki->kernelPtr3);
+    put_slot(slot_noop(), &ki->blockx4, &ki->blocky4, &ki->blockz4, &ki->threadx4,
+             &ki->thready4, &ki->threadz4, &ki->dynmicLdsSize4, &ki->kernelPtr4);
+    return CONV2D_OK;
+}
+
+int conv2d_workspace_size(const problem_t* problem, size_t* pin_bytes, size_t* filter_bytes,
+                          size_t* mid_bytes) {
+    if (!problem) return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = require_winograd(problem, &g);
+    if (rc) return rc;
+    if (pin_bytes) *pin_bytes = g.v_bytes;
+    if (filter_bytes) *filter_bytes = g.u_bytes;
+    if (mid_bytes) *mid_bytes = 0;
+    return CONV2D_OK;
+}
+
+int conv2d_launch(const kernelInfo_t* ki, void* param, void* stream) {
+    if (!ki || !param) return CONV2D_ERR_NULL;
+    if (!ki->kernelPtr1 || !ki->kernelPtr2 || !ki->kernelPtr3 || !ki->kernelPtr4)
+        return CONV2D_ERR_NULL;
+    hipStream_t st = (hipStream_t)stream;
+    void* args[1] = {param};
+    const void* ptrs[4] = {ki->kernelPtr1, ki->kernelPtr2, ki->kernelPtr3, ki->kernelPtr4};
+    const dim3 grids[4] = {dim3(ki->blockx1, ki->blocky1, ki->blockz1),
+                           dim3(ki->blockx2, ki->blocky2, ki->blockz2),
+                           dim3(ki->blockx3, ki->blocky3, ki->blockz3),
+                           dim3(ki->blockx4, ki->blocky4, ki->blockz4)};
+    const dim3 blocks[4] = {dim3(ki->threadx1, ki->thready1, ki->threadz1),
+                            dim3(ki->threadx2, ki->thready2, ki->threadz2),
+                            dim3(ki->threadx3, ki->thready3, ki->threadz3),
+                            dim3(ki->threadx4, ki->thready4, ki->threadz4)};
+    const unsigned lds[4] = {ki->dynmicLdsSize1, ki->dynmicLdsSize2, ki->dynmicLdsSize3,
+                             ki->dynmicLdsSize4};
+    for (int i = 0; i < 4; ++i) {
+        hipError_t e = hipExtLaunchKernel(ptrs[i], grids[i], blocks[i], args, lds[i], st,
+                                          nullptr, nullptr, 0);
+        if (e != hipSuccess) return CONV2D_ERR_HIP;
+    }
+    return CONV2D_OK;
+}
+
+static int check_ptrs(const problem_t* pb, bool need_in, bool need_w, bool need_out,
+                      bool need_v, bool need_u) {
+    if (!pb) return CONV2D_ERR_NULL;
+    if ((need_in && !pb->in) || (need_w && !pb->weight) || (need_out && !pb->out) ||
+        (need_v && !pb->transform_pin) || (need_u && !pb->transform_filter))
+        return CONV2D_ERR_NULL;
+    return CONV2D_OK;
+}
+
+int conv2d_prepare_weights(const problem_t* pb, void* stream) {
+    int rc = check_ptrs(pb, false, true, false, false, true);
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = require_winograd(pb, &g))) return rc;
+    mykernelParamType a;
+    fill_param(pb, g, 0, &a);
+    return launch(slot_filter(g), &a, (hipStream_t)stream);
+}
+
+int conv2d_input_transform(const problem_t* pb, void* stream) {
+    int rc = check_ptrs(pb, true, false, false, true, false);
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = require_winograd(pb, &g))) return rc;
+    mykernelParamType a;
+    fill_param(pb, g, 0, &a);
+    return launch(slot_input(pb, g), &a, (hipStream_t)stream);
+}
+
+int conv2d_gemm(const problem_t* pb, int cfg, void* stream) {
+    int rc = check_ptrs(pb, false, false, true, true, true);
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = require_winograd(pb, &g))) return rc;
+    if (cfg < 0) cfg = select_config(pb, g);
+    if (cfg >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
+    mykernelParamType a;
+    fill_param(pb, g, cfg, &a);
+    return launch(slot_gemm(g, cfg), &a, (hipStream_t)stream);
+}
+
+int conv2d_forward(const problem_t* pb, int cfg, void* stream) {
+    int rc = check_ptrs(pb, true, false, true, true, true);
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = require_winograd(pb, &g))) return rc;
+    if (cfg < 0) cfg = select_config(pb, g);
+    if (cfg >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
+    mykernelParamType a;
+    fill_param(pb, g, cfg, &a);
+    if ((rc = launch(slot_input(pb, g), &a, (hipStream_t)stream))) return rc;
+    return launch(slot_gemm(g, cfg), &a, (hipStream_t)stream);
+}
+
+int conv2d_num_tile_configs(void) { return kNumGemmConfigs; }
+
+int conv2d_tile_config(int cfg, int* bk, int* bt, int* threads) {
+    if (cfg < 0 || cfg >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
+    if (bk) *bk = kGemmConfigs[cfg].bk;
+    if (bt) *bt = kGemmConfigs[cfg].bt;
+    if (threads) *threads = kGemmConfigs[cfg].threads;
+    return CONV2D_OK;
+}
+
+int conv2d_select_tile_config(const problem_t* pb, int* cfg) {
+    if (!pb || !cfg) return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = require_winograd(pb, &g);
+    if (rc) return rc;
+    *cfg = select_config(pb, g);
+    return CONV2D_OK;
+}
+
+int conv2d_direct_f64(const problem_t* pb, double* out_f64, void* stream) {
+    if (!pb || !out_f64 || !pb->in || !pb->weight) return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = derive_geometry(pb, &g);
+    if (rc) return rc;
+    mykernelParamType a;
+    fill_param(pb, g, 0, &a);
+    const size_t total = (size_t)pb->n * pb->k * g.Oh * g.Ow;
+    size_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(conv_direct_f64_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, a, out_f64);
+    return hipGetLastError() == hipSuccess ? CONV2D_OK : CONV2D_ERR_HIP;
+}
+
+const char* conv2d_version(void) { return "winoconv-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// Internal host-side declarations shared by the plugin translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "conv2d.h"
+#include "mykernelParamType.h"
+
+namespace wino {
+
+struct Geometry {
+    int Oh, Ow;
+    bool winograd_ok;
+    int tiles_h, tiles_w;
+    long long T, NT, G16;
+    int CB, KB;
+    int band;
+    size_t v_bytes, u_bytes;
+};
+
+struct Launch {
+    const void* kernel;
+    dim3 grid;
+    dim3 block;
+    unsigned lds;
+};
+
+int derive_geometry(const problem_t* pb, Geometry* g);
+int require_winograd(const problem_t* pb, Geometry* g);
+int heuristic_config(const problem_t* pb, const Geometry& g);
+int select_config(const problem_t* pb, const Geometry& g);
+void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a);
+Launch slot_filter(const Geometry& g);
+Launch slot_input(const problem_t* pb, const Geometry& g);
+Launch slot_gemm(const Geometry& g, int cfg);
+Launch slot_noop();
+int launch(const Launch& L, mykernelParamType* a, hipStream_t stream);
+
+// Tuned-table lookup (tuner.hip): configuration index for this exact shape, or -1.
+int tuned_config_lookup(const problem_t* pb);
+
+}  // namespace wino

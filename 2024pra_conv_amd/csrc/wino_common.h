@@ -1,0 +1,45 @@
+// Shared device-side definitions for the gfx950 Winograd F(2x2,3x3) kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mykernelParamType.h"
+
+typedef _Float16 half_t;
+typedef half_t half8 __attribute__((ext_vector_type(8)));
+typedef half_t half2_t __attribute__((ext_vector_type(2)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// 16 B vector of raw bits, used for global <-> LDS copies.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Elements of one packed MFMA fragment (16x32 FP16 operand = 64 lanes x 8 halves).
+constexpr int kFragHalves = 512;
+
+// Offset (in halves) of fragment (pos, cb, blk) in a packed U or V buffer whose
+// block dimension has nblk entries: layout [pos][cb][blk][lane][8].
+__host__ __device__ inline size_t frag_offset(int pos, int cb, int blk, int CB, int nblk) {
+    return ((size_t)(pos * CB + cb) * (size_t)nblk + (size_t)blk) * kFragHalves;
+}
+
+// ---- kernels (host stubs are taken by getkernelInfo to fill the launch slots) ----
+__global__ void wino_filter_transform(mykernelParamType prm);
+__global__ void wino_input_transform(mykernelParamType prm);
+__global__ void wino_noop(mykernelParamType prm);
+__global__ void conv_direct_f64_kernel(mykernelParamType prm, double* out);
+
+// Input-transform LDS staging: a workgroup stages `band` tile rows of one image and one
+// 32-channel block. LDS row pitch in halves: column index = input column + q + 8, so the
+// 4x4 window of tile column tw starts at the even column 2*tw + 8 (4-byte aligned reads)
+// and 16-byte global chunks (input columns 8j..8j+7) map to LDS columns 8j+q+8.
+__host__ __device__ inline int input_lds_pitch(int tiles_w, int W, int q) {
+    int a = 2 * tiles_w + 10;
+    int b = W + q + 8;
+    int m = a > b ? a : b;
+    return (m + 1) & ~1;
+}
+__host__ __device__ inline int input_lds_rows(int band) { return 2 * band + 2; }
+__host__ __device__ inline size_t input_lds_bytes(int band, int tiles_w, int W, int q) {
+    return (size_t)32 * input_lds_rows(band) * input_lds_pitch(tiles_w, W, q) * sizeof(uint16_t);
+}

@@ -1,0 +1,204 @@
+// S3: the 16-position batched GEMM on CDNA4 FP16 MFMA with the output transform fused
+// into the epilogue.
+//
+// Reference counterpart: matrixMul1/3/4/5/6 (matrixmul.cuh:139-556), which run the DCU
+// v_mmac_f32_16x16x16_f16 and store every per-position product M to HBM in FP16, then the
+// separate store* kernels (conv2d.cpp:17-250) read M back and apply
+//   A^T = [[1,1,1,0],[0,1,-1,-1]]     (conv2d.cpp:18-22)
+// Here, for each Winograd position (a,b) a workgroup reduces over all channels into FP32
+// MFMA accumulators, then folds them at once into the four output accumulators of the
+// 2x2 tile: Y[i][j] += A^T[i][a] * A^T[j][b] * M(a,b). M never leaves registers, and the
+// FP16 output is rounded once from FP32.
+//
+// Workgroup tile: BK output channels x BT tiles (GEMM columns g = image*T + tile). Waves
+// form a WK x WT grid; a wave owns (BK/WK) x (BT/WT) and issues
+// v_mfma_f32_16x16x32_f16 on 16x16 sub-tiles. Each pipeline step moves one 32-channel
+// chunk of U (BK/16 fragments) and V (BT/16 fragments) -- contiguous 1 KiB fragments in
+// the packed layouts -- into one of two LDS buffers while the other is consumed.
+#include "wino_common.h"
+#include "wino_gemm.h"
+
+namespace {
+
+constexpr int kAT[2][4] = {{1, 1, 1, 0}, {0, 1, -1, -1}};
+
+template <int BK, int BT, int WK, int WT>
+struct GemmShape {
+    static constexpr int kThreads = WK * WT * 64;
+    static constexpr int MI = BK / WK / 16;  // 16-row sub-tiles per wave
+    static constexpr int NI = BT / WT / 16;  // 16-column sub-tiles per wave
+    static constexpr int UF = BK / 16;       // U fragments per step
+    static constexpr int VF = BT / 16;       // V fragments per step
+    static constexpr int kPieces = (UF + VF) * 64;  // 16 B pieces per step
+    static constexpr int kLoads = (kPieces + kThreads - 1) / kThreads;
+    static constexpr int kStageHalves = (UF + VF) * kFragHalves;
+    static_assert(MI >= 1 && NI >= 1, "wave tile must be at least 16x16");
+    static_assert(BK % (WK * 16) == 0 && BT % (WT * 16) == 0, "tile must divide evenly");
+};
+
+}  // namespace
+
+template <int BK, int BT, int WK, int WT>
+__global__ void __launch_bounds__(WK * WT * 64) wino_gemm_kernel(mykernelParamType prm) {
+    using S = GemmShape<BK, BT, WK, WT>;
+    constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF;
+    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wk = wave % WK;
+    const int wt = wave / WK;
+    const int gb0 = blockIdx.x * VF;  // first 16-column block of V
+    const int kb0 = blockIdx.y * UF;  // first 16-row block of U
+    const int KB = prm.KB, G16 = prm.G16, CB = prm.CB;
+    const int nsteps = 16 * CB;
+    const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
+    const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
+
+    // Per-thread copy slots: piece i of a step is U piece i (i < UF*64) or V piece
+    // i - UF*64. A piece whose fragment lies beyond the matrix is not loaded; the stale LDS
+    // bytes it leaves only feed accumulator rows/columns that are never stored.
+    const half_t* src[S::kLoads];
+    size_t sstride[S::kLoads];
+    bool sval[S::kLoads];
+#pragma unroll
+    for (int i = 0; i < S::kLoads; ++i) {
+        const int pc = tid + i * S::kThreads;
+        if (pc < UF * 64) {
+            src[i] = U + (size_t)kb0 * kFragHalves + (size_t)pc * 8;
+            sstride[i] = (size_t)KB * kFragHalves;
+            sval[i] = (kb0 + pc / 64) < KB;
+        } else if (pc < S::kPieces) {
+            const int pv = pc - UF * 64;
+            src[i] = V + (size_t)gb0 * kFragHalves + (size_t)pv * 8;
+            sstride[i] = (size_t)G16 * kFragHalves;
+            sval[i] = (gb0 + pv / 64) < G16;
+        } else {
+            src[i] = U;
+            sstride[i] = 0;
+            sval[i] = false;
+        }
+    }
+
+    half8 pre[S::kLoads];
+    auto gload = [&](int step) {
+#pragma unroll
+        for (int i = 0; i < S::kLoads; ++i)
+            if (sval[i]) pre[i] = *(const half8*)(src[i] + (size_t)step * sstride[i]);
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < S::kLoads; ++i) {
+            const int pc = tid + i * S::kThreads;
+            if (sval[i]) *(half8*)(smem + buf * S::kStageHalves + pc * 8) = pre[i];
+        }
+    };
+
+    floatx4 acc[MI][NI];
+    floatx4 Y[4][MI][NI];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni) Y[q][mi][ni] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    gload(0);
+    lstore(0);
+    __syncthreads();
+
+    // The position loop stays rolled: unrolling all 16 positions makes the register
+    // allocator spill. The fold coefficients are then wave-uniform runtime values in
+    // {-1, 0, 1}; zero ones are skipped with a uniform branch.
+#pragma unroll 1
+    for (int pos = 0; pos < 16; ++pos) {
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+        for (int cb = 0; cb < CB; ++cb) {
+            const int step = pos * CB + cb;
+            const bool has_next = step + 1 < nsteps;
+            if (has_next) gload(step + 1);
+            const half_t* sb = smem + (step & 1) * S::kStageHalves;
+            half8 a[MI], b[NI];
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi)
+                a[mi] = *(const half8*)(sb + ((wk * MI + mi) * 64 + lane) * 8);
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni)
+                b[ni] = *(const half8*)(sb + UF * kFragHalves + ((wt * NI + ni) * 64 + lane) * 8);
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
+            if (has_next) lstore((step + 1) & 1);
+            __syncthreads();
+        }
+
+        // fold M(a,b) into the 2x2 output accumulators
+        const int pa = pos >> 2, pb = pos & 3;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int coef = kAT[i][pa] * kAT[j][pb];
+                if (coef == 0) continue;
+                const float cf = (float)coef;
+#pragma unroll
+                for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < NI; ++ni) Y[i * 2 + j][mi][ni] += cf * acc[mi][ni];
+            }
+    }
+
+    // epilogue: 2x2 output tile per (k, g), FP32 -> FP16, ragged edges guarded
+    const int T = prm.T, tiles_w = prm.tiles_w, K = prm.k, Oh = prm.Oh, Ow = prm.Ow;
+    const int NT = prm.NT;
+    half_t* __restrict__ out = (half_t*)prm.pout;
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+        const int g = (gb0 + wt * NI + ni) * 16 + (lane & 15);
+        if (g >= NT) continue;
+        const int img = g / T;
+        const int t = g - img * T;
+        const int th = t / tiles_w;
+        const int tw = t - th * tiles_w;
+        const int oh = 2 * th, ow = 2 * tw;
+        const bool r1 = oh + 1 < Oh, c1 = ow + 1 < Ow;
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = (kb0 + wk * MI + mi) * 16 + 4 * (lane >> 4) + j;
+                if (k >= K) continue;
+                half_t* o = out + (((size_t)img * K + k) * Oh + oh) * Ow + ow;
+                o[0] = (half_t)Y[0][mi][ni][j];
+                if (c1) o[1] = (half_t)Y[1][mi][ni][j];
+                if (r1) {
+                    o[Ow] = (half_t)Y[2][mi][ni][j];
+                    if (c1) o[Ow + 1] = (half_t)Y[3][mi][ni][j];
+                }
+            }
+    }
+}
+
+// ---- configuration table --------------------------------------------------------------
+
+#define WINO_CFG(BK, BT, WK, WT) \
+    { BK, BT, WK * WT * 64, (const void*)&wino_gemm_kernel<BK, BT, WK, WT> }
+
+const GemmConfig kGemmConfigs[] = {
+    WINO_CFG(128, 128, 2, 4),  // 0: wave 64x32, 8 waves
+    WINO_CFG(64, 128, 1, 4),   // 1: wave 64x32, 4 waves
+    WINO_CFG(128, 64, 2, 2),   // 2: wave 64x32, 4 waves
+    WINO_CFG(64, 64, 1, 2),    // 3: wave 64x32, 2 waves
+    WINO_CFG(32, 128, 1, 4),   // 4: wave 32x32, 4 waves
+    WINO_CFG(256, 64, 4, 2),   // 5: wave 64x32, 8 waves
+    WINO_CFG(16, 256, 1, 8),   // 6: wave 16x32, 8 waves (tiny K)
+    WINO_CFG(64, 256, 1, 8),   // 7: wave 64x32, 8 waves
+};
+const int kNumGemmConfigs = (int)(sizeof(kGemmConfigs) / sizeof(kGemmConfigs[0]));

@@ -1,0 +1,118 @@
+"""Drives the C-ABI library on PyTorch-allocated device buffers (tests and benchmarks).
+
+PyTorch provides device memory and the stream handle only; every kernel is the library's.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import KernelInfo, ParamType, check, load_library, make_problem, workspace_size
+
+
+def _stream_handle(stream: torch.cuda.Stream | None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class WinogradConv:
+    """One convolution problem with its own output and workspaces.
+
+    x: (N, C, H, W) float16 on cuda; w: (K, C, 3, 3) float16 on cuda.
+    """
+
+    def __init__(self, x: torch.Tensor, w: torch.Tensor, pad=(1, 1), out: torch.Tensor | None = None,
+                 workspace: tuple[torch.Tensor, torch.Tensor] | None = None):
+        assert x.dtype == torch.float16 and w.dtype == torch.float16
+        assert x.is_cuda and w.is_cuda and x.is_contiguous() and w.is_contiguous()
+        self.L = load_library()
+        n, c, h, wd = x.shape
+        k, c2, r, s = w.shape
+        assert c == c2
+        self.x, self.w = x, w
+        self.pad = pad
+        self.shape = (n, c, h, wd, k, r, s, 1, 1, pad[0], pad[1])
+        self.oh = (h - r + 2 * pad[0]) + 1
+        self.ow = (wd - s + 2 * pad[1]) + 1
+        probe = make_problem(*self.shape)
+        vb, ub, _ = workspace_size(probe)
+        self.v_bytes, self.u_bytes = vb, ub
+        dev = x.device
+        self.out = out if out is not None else torch.empty((n, k, self.oh, self.ow), dtype=torch.float16, device=dev)
+        if workspace is not None:
+            self.V, self.U = workspace
+        else:
+            self.V = torch.empty(max(vb, 2), dtype=torch.uint8, device=dev)
+            self.U = torch.empty(max(ub, 2), dtype=torch.uint8, device=dev)
+        self.pb = make_problem(*self.shape, ptrs={
+            "in_": x.data_ptr(), "weight": w.data_ptr(), "out": self.out.data_ptr(),
+            "transform_pin": self.V.data_ptr(), "transform_filter": self.U.data_ptr(), "mid_result": None})
+        self._ki = None
+        self._param = None
+
+    # --- reference protocol: getParamsize / getkernelInfo / four slots -----------------
+    def kernel_info(self):
+        if self._ki is None:
+            size = ctypes.c_int()
+            check(self.L.getParamsize(ctypes.byref(self.pb), ctypes.byref(size)), "getParamsize")
+            assert size.value == ctypes.sizeof(ParamType)
+            self._param = ctypes.create_string_buffer(size.value)
+            ki = KernelInfo()
+            check(self.L.getkernelInfo(ctypes.byref(self.pb), ctypes.byref(ki), self._param), "getkernelInfo")
+            self._ki = ki
+        return self._ki, self._param
+
+    def param(self) -> ParamType:
+        _, buf = self.kernel_info()
+        return ParamType.from_buffer_copy(buf)
+
+    def launch_slots(self, stream=None) -> None:
+        ki, param = self.kernel_info()
+        check(self.L.conv2d_launch(ctypes.byref(ki), param, _stream_handle(stream)), "conv2d_launch")
+
+    # --- stage entry points -------------------------------------------------------------
+    def prepare_weights(self, stream=None) -> None:
+        check(self.L.conv2d_prepare_weights(ctypes.byref(self.pb), _stream_handle(stream)), "conv2d_prepare_weights")
+
+    def input_transform(self, stream=None) -> None:
+        check(self.L.conv2d_input_transform(ctypes.byref(self.pb), _stream_handle(stream)), "conv2d_input_transform")
+
+    def gemm(self, cfg: int = -1, stream=None) -> None:
+        check(self.L.conv2d_gemm(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_gemm")
+
+    def forward(self, cfg: int = -1, stream=None) -> None:
+        check(self.L.conv2d_forward(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_forward")
+
+    def selected_config(self) -> int:
+        c = ctypes.c_int()
+        check(self.L.conv2d_select_tile_config(ctypes.byref(self.pb), ctypes.byref(c)), "conv2d_select_tile_config")
+        return c.value
+
+    def direct_f64(self, stream=None) -> torch.Tensor:
+        ref = torch.empty(self.out.shape, dtype=torch.float64, device=self.x.device)
+        check(self.L.conv2d_direct_f64(ctypes.byref(self.pb), ctypes.c_void_p(ref.data_ptr()), _stream_handle(stream)),
+              "conv2d_direct_f64")
+        return ref
+
+    # --- raw workspace views --------------------------------------------------------------
+    def u_halves(self) -> torch.Tensor:
+        return self.U[: self.u_bytes].view(torch.float16)
+
+    def v_halves(self) -> torch.Tensor:
+        return self.V[: self.v_bytes].view(torch.float16)
+
+
+def direct_f64(x: torch.Tensor, w: torch.Tensor, stride=(1, 1), pad=(1, 1)) -> torch.Tensor:
+    """FP64 validation conv of any geometry through the library's GPU kernel."""
+    L = load_library()
+    n, c, h, wd = x.shape
+    k, _, r, s = w.shape
+    oh = (h - r + 2 * pad[0]) // stride[0] + 1
+    ow = (wd - s + 2 * pad[1]) // stride[1] + 1
+    pb = make_problem(n, c, h, wd, k, r, s, stride[0], stride[1], pad[0], pad[1],
+                      ptrs={"in_": x.data_ptr(), "weight": w.data_ptr()})
+    ref = torch.empty((n, k, oh, ow), dtype=torch.float64, device=x.device)
+    check(L.conv2d_direct_f64(ctypes.byref(pb), ctypes.c_void_p(ref.data_ptr()), _stream_handle(None)),
+          "conv2d_direct_f64")
+    return ref

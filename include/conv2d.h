@@ -1,0 +1,177 @@
+/*
+ * conv2d.h — C-ABI of the MI355X (gfx950) FP16 Winograd F(2x2,3x3) convolution plugin.
+ *
+ * Drop-in for the contest plugin interface of the reference
+ * (include/conv2d.h:8-70 in Yhorm26/2024PRA_CONV): the same problem_t and kernelInfo_t
+ * layouts (field order and types unchanged) and the same getParamsize / getkernelInfo
+ * signatures, now with C linkage. Every function below returns 0 on success and a
+ * CONV2D_ERR_* code otherwise; nothing is launched on error.
+ *
+ * Tensor layouts follow the reference's spec (readme.txt:21-24): input NCHW, weight KCRS,
+ * output N,K,Oh,Ow, all FP16, with Oh = (h - r + 2p)/u + 1 and Ow = (w - s + 2q)/v + 1
+ * (readme.txt:17-19).
+ *
+ * The Winograd path supports r = s = 3, u = v = 1 and any pad p, q in [0, 2]. Other
+ * geometries are rejected with CONV2D_ERR_UNSUPPORTED (the reference computed garbage
+ * for them, conv2d.cpp:254-494).
+ */
+#ifndef CONV2D_MI355X_H_
+#define CONV2D_MI355X_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define __in__
+#define __out__
+#define __in_out__
+
+#if defined(__clang__) || (defined(__GNUC__) && __GNUC__ >= 12)
+typedef _Float16 conv2d_half_t;
+#else
+typedef uint16_t conv2d_half_t; /* same size and alignment as _Float16 */
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+enum {
+    CONV2D_OK = 0,
+    CONV2D_ERR_NULL = 1,        /* a required pointer is NULL */
+    CONV2D_ERR_UNSUPPORTED = 2, /* geometry the Winograd path does not implement */
+    CONV2D_ERR_OVERFLOW = 3,    /* a dimension does not fit the kernel argument (short) */
+    CONV2D_ERR_WORKSPACE = 4,   /* caller-declared workspace smaller than required */
+    CONV2D_ERR_HIP = 5,         /* a HIP runtime call failed */
+    CONV2D_ERR_INVALID = 6      /* bad argument (zero dimension, unknown config, ...) */
+};
+
+/* Problem descriptor. Same layout as the reference's problem_t (conv2d.h:8-27). The three
+ * workspaces are caller-owned device buffers:
+ *   transform_pin    : transformed input V (size from conv2d_workspace_size)
+ *   transform_filter : transformed filter U (size from conv2d_workspace_size)
+ *   mid_result       : unused by this implementation (the output transform is fused into
+ *                      the GEMM, so the per-position products M never reach HBM); may be NULL. */
+typedef struct {
+    conv2d_half_t* in;
+    conv2d_half_t* weight;
+    conv2d_half_t* out;
+    conv2d_half_t* transform_pin;
+    conv2d_half_t* transform_filter;
+    conv2d_half_t* mid_result;
+    unsigned int n;
+    unsigned int c;
+    unsigned int h;
+    unsigned int w;
+    unsigned int k;
+    unsigned int r;
+    unsigned int s;
+    unsigned int u;
+    unsigned int v;
+    unsigned int p;
+    unsigned int q;
+} problem_t;
+
+/* Four launch slots, as in the reference (conv2d.h:29-66). This implementation fills:
+ *   slot 1: filter transform  U = G g G^T        (wino_filter_transform)
+ *   slot 2: input transform   V = B^T d B        (wino_input_transform)
+ *   slot 3: 16-position FP16 MFMA GEMM with the output transform A^T M A fused into its
+ *           epilogue (wino_gemm_*)
+ *   slot 4: an empty one-workgroup kernel (the fourth reference stage, conv2d.cpp:17-250,
+ *           no longer exists as a separate pass), kept so callers that launch all four
+ *           slots unconditionally stay valid. */
+typedef struct {
+    unsigned int blockx1, blocky1, blockz1;
+    unsigned int threadx1, thready1, threadz1;
+    unsigned int dynmicLdsSize1;
+    void* kernelPtr1;
+
+    unsigned int blockx2, blocky2, blockz2;
+    unsigned int threadx2, thready2, threadz2;
+    unsigned int dynmicLdsSize2;
+    void* kernelPtr2;
+
+    unsigned int blockx3, blocky3, blockz3;
+    unsigned int threadx3, thready3, threadz3;
+    unsigned int dynmicLdsSize3;
+    void* kernelPtr3;
+
+    unsigned int blockx4, blocky4, blockz4;
+    unsigned int threadx4, thready4, threadz4;
+    unsigned int dynmicLdsSize4;
+    void* kernelPtr4;
+} kernelInfo_t;
+
+/* ---- Reference entry points (drop-in) ------------------------------------------------ */
+
+/* Replaces getParamsize (reference conv2d.h:69, conv2d.cpp:9-14): writes
+ * sizeof(mykernelParamType) of this implementation (the reference struct plus appended
+ * fields). */
+int getParamsize(__in__ problem_t* problem, __out__ int* paramSize);
+
+/* Replaces getkernelInfo (reference conv2d.h:70, conv2d.cpp:254-517): validates the
+ * geometry, selects a GEMM tile configuration (the tuned table if one is loaded, else a
+ * shape heuristic), fills the four launch slots and packs the kernel argument blob.
+ * The caller must have allocated the workspaces at least as large as
+ * conv2d_workspace_size reports. Each slot's kernel takes the blob by value as its only
+ * argument, exactly as the reference harness launches them (main.cpp:105-108). */
+int getkernelInfo(__in__ problem_t* problem, __out__ kernelInfo_t* kernelInfo,
+                  __in_out__ void* param);
+
+/* ---- Additive entry points (no counterpart in the reference) ------------------------- */
+
+/* Workspace sizes in bytes for transform_pin (V), transform_filter (U) and mid_result
+ * (always 0). The reference harness sized them as 4*h*h*n*c, 16*k*c and 4*k*h*w*n halves
+ * (main.cpp:33-38); those sizes suffice when h == w is even, c % 32 == 0, k % 16 == 0 and
+ * n*(h/2)*(w/2) % 16 == 0, and are too small otherwise (ragged or padded shapes). */
+int conv2d_workspace_size(const problem_t* problem, size_t* pin_bytes,
+                          size_t* filter_bytes, size_t* mid_bytes);
+
+/* Launches the four slots in order on a HIP stream (NULL = null stream) with
+ * hipExtLaunchKernel, as the reference harness does (main.cpp:105-108). */
+int conv2d_launch(const kernelInfo_t* kernelInfo, void* param, void* stream);
+
+/* S1 only: U = G g G^T into problem->transform_filter. Run once per weight set. */
+int conv2d_prepare_weights(const problem_t* problem, void* stream);
+
+/* S2 only: V = B^T d B into problem->transform_pin. */
+int conv2d_input_transform(const problem_t* problem, void* stream);
+
+/* S3 only (GEMM + fused output transform) with tile configuration cfg
+ * (-1 = the same choice getkernelInfo makes). Needs U and V prepared. */
+int conv2d_gemm(const problem_t* problem, int cfg, void* stream);
+
+/* S2 + S3 (steady state: U prepared once). cfg as for conv2d_gemm. */
+int conv2d_forward(const problem_t* problem, int cfg, void* stream);
+
+/* Tile configurations of the GEMM stage. */
+int conv2d_num_tile_configs(void);
+/* Describes configuration cfg: rows of k (bk) and columns of tiles (bt) per workgroup,
+ * threads per workgroup. */
+int conv2d_tile_config(int cfg, int* bk, int* bt, int* threads);
+/* The configuration getkernelInfo would select for this problem. */
+int conv2d_select_tile_config(const problem_t* problem, int* cfg);
+
+/* Validation-only FP64 direct convolution on the GPU (any r, s, u, v, p, q): every
+ * product of two FP16 values is formed exactly in FP64 and summed in FP64, written as
+ * N,K,Oh,Ow doubles to out_f64 (device memory). The only FP64 arithmetic on the GPU;
+ * no production stage uses it. */
+int conv2d_direct_f64(const problem_t* problem, double* out_f64, void* stream);
+
+/* ---- Tuned tile-configuration table (replaces the reference's hard-coded (n, c) dispatch,
+ * conv2d.cpp:273-494). getkernelInfo consults it before the shape heuristic. ---- */
+int conv2d_tuned_set(const problem_t* problem, int cfg);
+int conv2d_tuned_clear(void);
+int conv2d_tuned_count(void);
+/* Text file, one line per shape: "n c h w k p q cfg". */
+int conv2d_tuned_save(const char* path);
+int conv2d_tuned_load(const char* path);
+
+/* Library version string. */
+const char* conv2d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CONV2D_MI355X_H_ */

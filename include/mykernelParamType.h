@@ -1,0 +1,51 @@
+/*
+ * mykernelParamType.h — by-value kernel argument of every kernel this plugin puts in a
+ * launch slot.
+ *
+ * The first 19 fields keep the reference's struct (src/mykernelParamType.cuh:5-26): six
+ * FP16 pointers then thirteen `short` dimensions. The derived Winograd geometry is
+ * appended after them; getParamsize reports the full size, so callers that malloc the blob
+ * from that size (reference main.cpp:84-86) keep working.
+ */
+#ifndef MYKERNELPARAMTYPE_MI355X_H_
+#define MYKERNELPARAMTYPE_MI355X_H_
+
+#include "conv2d.h"
+
+typedef struct mykernelParamType {
+    conv2d_half_t* __restrict__ pin;              /* input, NCHW */
+    conv2d_half_t* __restrict__ pweight;          /* weight, KCRS */
+    conv2d_half_t* __restrict__ pout;             /* output, N K Oh Ow */
+    conv2d_half_t* __restrict__ transform_pin;    /* V, packed (see below) */
+    conv2d_half_t* __restrict__ transform_filter; /* U, packed (see below) */
+    conv2d_half_t* __restrict__ mid_result;       /* unused */
+    short n, c, h, w, k, r, s, u, v, p, q, Oh, Ow;
+
+    /* ---- appended: derived geometry, filled by getkernelInfo ---- */
+    int tiles_h; /* ceil(Oh / 2) */
+    int tiles_w; /* ceil(Ow / 2) */
+    int T;       /* tiles_h * tiles_w, 2x2 output tiles per image */
+    int NT;      /* n * T, GEMM columns (tile index g = image * T + tile) */
+    int G16;     /* ceil(NT / 16), 16-column blocks of V */
+    int CB;      /* ceil(c / 32), 32-channel blocks (MFMA reduction chunks) */
+    int KB;      /* ceil(k / 16), 16-row blocks of U */
+    int band;    /* tile rows per input-transform workgroup */
+    int cfg;     /* GEMM tile configuration index */
+    int flags;   /* bit 0: input rows are 16-byte aligned (w % 8 == 0 and aligned base) */
+} mykernelParamType;
+
+/*
+ * Packed layouts (element = one FP16 value). Both are the operand layouts of
+ * v_mfma_f32_16x16x32_f16, so a GEMM workgroup copies 1 KiB fragments verbatim into LDS
+ * and each lane then reads its 16 bytes.
+ *
+ *   U (transform_filter): U[pos][cb][kb][lane][e], lane = (c % 32 / 8) * 16 + k % 16,
+ *                         e = c % 8; pos = 4*a + b is the Winograd position (row a, col b).
+ *                         Rows k >= K and channels c >= C are zero.
+ *   V (transform_pin):    V[pos][cb][gb][lane][e], lane = (c % 32 / 8) * 16 + g % 16,
+ *                         e = c % 8, g = image * T + tile_row * tiles_w + tile_col.
+ *                         Channels c >= C and columns g >= NT are zero.
+ * Sizes: U = 16 * CB * KB * 512 halves, V = 16 * CB * G16 * 512 halves.
+ */
+
+#endif /* MYKERNELPARAMTYPE_MI355X_H_ */

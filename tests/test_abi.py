@@ -1,0 +1,172 @@
+"""C-ABI boundary tests (CPU only: no call here launches a kernel)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "conv2d.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol(wc):
+    L = wc.load_library()
+    names = _declared_functions()
+    assert "getParamsize" in names and "getkernelInfo" in names and len(names) >= 18
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", wc.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (\w+)$", out, flags=re.M))
+    assert set(names) <= exported
+    # the bindings cover exactly the declared API
+    assert set(wc.exported_symbols()) == set(names)
+
+
+def test_struct_layouts_match_c_headers(wc, tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "mykernelParamType.h"
+int main(void) {
+  printf("%zu %zu %zu\n", sizeof(problem_t), sizeof(kernelInfo_t), sizeof(mykernelParamType));
+  printf("%zu %zu %zu %zu\n", offsetof(problem_t, n), offsetof(problem_t, q), offsetof(kernelInfo_t, kernelPtr1), offsetof(kernelInfo_t, blockx4));
+  printf("%zu %zu %zu %zu %zu\n", offsetof(mykernelParamType, n), offsetof(mykernelParamType, Ow),
+         offsetof(mykernelParamType, tiles_h), offsetof(mykernelParamType, cfg), offsetof(mykernelParamType, flags));
+  return 0; }
+''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-Wall", "-Werror", f"-I{ROOT}/include", "-o", str(exe), str(src)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    assert list(map(int, lines[0].split())) == [ctypes.sizeof(wc.Problem), ctypes.sizeof(wc.KernelInfo),
+                                                ctypes.sizeof(wc.ParamType)]
+    P, K, M = wc.Problem, wc.KernelInfo, wc.ParamType
+    assert list(map(int, lines[1].split())) == [P.n.offset, P.q.offset, K.kernelPtr1.offset, K.blockx4.offset]
+    assert list(map(int, lines[2].split())) == [M.n.offset, M.Ow.offset, M.tiles_h.offset, M.cfg.offset, M.flags.offset]
+    # the reference layouts (probed: problem_t 96 B, kernelInfo_t 160 B; param: pointers 0-47,
+    # the 13 shorts at 48-73)
+    assert ctypes.sizeof(P) == 96 and ctypes.sizeof(K) == 160
+    assert M.n.offset == 48 and M.Ow.offset == 72
+
+
+def test_getparamsize(wc):
+    L = wc.load_library()
+    pb = wc.make_problem(1, 64, 56, 56, 64)
+    sz = ctypes.c_int()
+    assert L.getParamsize(ctypes.byref(pb), ctypes.byref(sz)) == wc.OK
+    assert sz.value == ctypes.sizeof(wc.ParamType)
+    assert L.getParamsize(ctypes.byref(pb), None) == wc.ERR_NULL
+
+
+def _ki(wc, pb):
+    L = wc.load_library()
+    ki = wc.KernelInfo()
+    buf = ctypes.create_string_buffer(ctypes.sizeof(wc.ParamType))
+    rc = L.getkernelInfo(ctypes.byref(pb), ctypes.byref(ki), buf)
+    return rc, ki, wc.ParamType.from_buffer_copy(buf)
+
+
+FAKE = {"in_": 256, "weight": 256, "out": 256, "transform_pin": 256, "transform_filter": 256}
+
+
+def test_getkernelinfo_fills_slots_and_param(wc):
+    pb = wc.make_problem(32, 256, 56, 56, 256, ptrs=FAKE)
+    rc, ki, prm = _ki(wc, pb)
+    assert rc == wc.OK
+    assert (prm.n, prm.c, prm.h, prm.w, prm.k, prm.r, prm.s, prm.u, prm.v, prm.p, prm.q) == (32, 256, 56, 56, 256, 3, 3, 1, 1, 1, 1)
+    assert (prm.Oh, prm.Ow, prm.tiles_h, prm.tiles_w, prm.T, prm.NT) == (56, 56, 28, 28, 784, 25088)
+    assert (prm.G16, prm.CB, prm.KB) == (1568, 8, 16)
+    assert prm.pin == 256 and prm.transform_filter == 256 and prm.flags == 1
+    for i in range(1, 5):
+        s = ki.slot(i)
+        assert s["kernel"] and all(s["grid"]) and all(s["block"])
+    assert ki.slot(4)["grid"] == (1, 1, 1)
+    cfg = wc.tile_configs()[prm.cfg]
+    assert ki.slot(3)["block"][0] == cfg["threads"]
+    assert ki.slot(3)["grid"] == (-(-prm.G16 // (cfg["bt"] // 16)), -(-prm.KB // (cfg["bk"] // 16)), 1)
+    assert ki.slot(2)["grid"] == (-(-28 // prm.band), 8, 32)
+    assert ki.slot(1)["grid"][0] * 256 >= prm.KB * prm.CB * 64
+
+
+@pytest.mark.parametrize("change,code", [
+    ({"r": 5, "s": 5}, "ERR_UNSUPPORTED"), ({"u": 2, "v": 2}, "ERR_UNSUPPORTED"), ({"p": 3}, "ERR_UNSUPPORTED"),
+    ({"n": 0}, "ERR_INVALID"), ({"c": 40000}, "ERR_OVERFLOW"), ({"h": 1, "p": 0}, "ERR_INVALID"),
+])
+def test_getkernelinfo_rejects(wc, change, code):
+    args = dict(n=1, c=16, h=8, w=8, k=16, r=3, s=3, u=1, v=1, p=1, q=1)
+    args.update(change)
+    pb = wc.make_problem(**args, ptrs=FAKE)
+    rc, _, _ = _ki(wc, pb)
+    assert rc == getattr(wc, code)
+
+
+def test_getkernelinfo_null_pointers(wc):
+    L = wc.load_library()
+    pb = wc.make_problem(1, 16, 8, 8, 16)  # no device pointers
+    rc, _, _ = _ki(wc, pb)
+    assert rc == wc.ERR_NULL
+    assert L.getkernelInfo(None, None, None) == wc.ERR_NULL
+    assert L.conv2d_launch(None, None, None) == wc.ERR_NULL
+    ki = wc.KernelInfo()  # all-zero slots: refused before any launch
+    assert L.conv2d_launch(ctypes.byref(ki), ctypes.c_void_p(1), None) == wc.ERR_NULL
+
+
+def test_workspace_sizes_and_legacy_sizing(wc):
+    # legacy harness sizes (main.cpp:36-38): V 4*h*h*n*c, U 16*k*c halves
+    for (n, c, h, k) in [(32, 256, 56, 256), (1, 64, 56, 64), (64, 512, 28, 512), (32, 96, 55, 96),
+                         (1, 3, 13, 32), (16, 128, 64, 27), (2, 40, 10, 20)]:
+        pb = wc.make_problem(n, c, h, h, k)
+        vb, ub, mb = wc.workspace_size(pb)
+        tiles = (h + 1) // 2
+        g16 = -(-(n * tiles * tiles) // 16)
+        cb, kb = -(-c // 32), -(-k // 16)
+        assert vb == 16 * cb * g16 * 512 * 2 and ub == 16 * cb * kb * 512 * 2 and mb == 0
+        legacy_ok = vb <= 4 * h * h * n * c * 2 and ub <= 16 * k * c * 2
+        predicted = h % 2 == 0 and c % 32 == 0 and k % 16 == 0 and (n * tiles * tiles) % 16 == 0
+        assert legacy_ok == predicted, (n, c, h, k)
+
+
+def test_tile_configs_and_selection(wc):
+    cfgs = wc.tile_configs()
+    assert len(cfgs) >= 8
+    L = wc.load_library()
+    assert L.conv2d_tile_config(len(cfgs), None, None, None) == wc.ERR_INVALID
+    for c in cfgs:
+        assert c["bk"] % 16 == 0 and c["bt"] % 16 == 0 and c["threads"] % 64 == 0 and c["threads"] <= 1024
+
+
+def test_tuned_table_roundtrip(wc, tmp_path):
+    L = wc.load_library()
+    L.conv2d_tuned_clear()
+    pb = wc.make_problem(4, 64, 28, 28, 128, ptrs=FAKE)
+    c = ctypes.c_int()
+    assert L.conv2d_select_tile_config(ctypes.byref(pb), ctypes.byref(c)) == wc.OK
+    default = c.value
+    other = (default + 1) % L.conv2d_num_tile_configs()
+    assert L.conv2d_tuned_set(ctypes.byref(pb), other) == wc.OK
+    assert L.conv2d_select_tile_config(ctypes.byref(pb), ctypes.byref(c)) == wc.OK and c.value == other
+    rc, _, prm = _ki(wc, pb)
+    assert rc == wc.OK and prm.cfg == other
+    path = str(tmp_path / "tuned.txt").encode()
+    assert L.conv2d_tuned_save(path) == wc.OK
+    L.conv2d_tuned_clear()
+    assert L.conv2d_tuned_count() == 0
+    assert L.conv2d_tuned_load(path) == wc.OK and L.conv2d_tuned_count() == 1
+    assert L.conv2d_select_tile_config(ctypes.byref(pb), ctypes.byref(c)) == wc.OK and c.value == other
+    assert L.conv2d_tuned_set(ctypes.byref(pb), 999) == wc.ERR_INVALID
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1 2 3\n")
+    assert L.conv2d_tuned_load(str(bad).encode()) == wc.ERR_INVALID
+    L.conv2d_tuned_clear()
+
+
+def test_missing_library_fails_loudly(wc, tmp_path):
+    with pytest.raises(RuntimeError):
+        wc.load_library(str(tmp_path / "nope.so"))

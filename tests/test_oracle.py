@@ -1,0 +1,242 @@
+"""Pins the oracle (CPU only): every oracle function is checked against something other
+than its own formula — closed forms, an independent library (torch, via committed
+fixtures), the reference's own checker, brute force on small tiles, or invariants."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import cdirect
+from oracle import winograd as wg
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---- FP16 rounding ----------------------------------------------------------------------
+def test_round_half_matches_numpy_rne():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([
+        rng.standard_normal(20000) * 10.0 ** rng.integers(-9, 5, 20000),
+        # exact ties between FP16 neighbours must go to even
+        (np.arange(1, 2000) + 0.5) * 2.0 ** -10,
+        np.array([65504.0, 65519.99, 65520.0, 1e6, -1e6, 2.0 ** -25, 3 * 2.0 ** -26, 0.0]),
+    ])
+    for x in xs:
+        assert cdirect.round_half(x) == float(np.float16(x)) or (np.isinf(np.float16(x)) and np.isinf(cdirect.round_half(x)))
+
+
+def test_splitmix64_known_answer():
+    # SplitMix64 with state 0: the first output is 0xE220A8397B1DCDAF (Steele et al. 2014)
+    assert int(wg.splitmix64(0, np.array([0], dtype=np.uint64))[0]) == 0xE220A8397B1DCDAF
+    a = wg.uniform_pm1(5, (3, 4))
+    assert a.dtype == np.float16 and np.array_equal(a, wg.uniform_pm1(5, (3, 4)))
+    assert np.all(np.abs(a.astype(np.float64)) <= 1.0)
+    z = wg.normal(3, (200000,)).astype(np.float64)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
+
+
+def test_outliers_fraction():
+    x = np.ones(200000, dtype=np.float16)
+    y = wg.with_outliers(x, 9)
+    frac = np.count_nonzero(y == 50.0) / x.size
+    assert 0.009 < frac < 0.011
+
+
+# ---- direct convolution -------------------------------------------------------------------
+def test_direct_all_ones_closed_form():
+    # all-ones input and filter, pad 1: output = C * (valid taps) = 4C corners, 6C edges, 9C inside
+    c = 7
+    x = np.ones((1, c, 4, 5), dtype=np.float16)
+    w = np.ones((2, c, 3, 3), dtype=np.float16)
+    y = cdirect.conv2d_f64(x, w)
+    taps = np.array([[4, 6, 6, 6, 4], [6, 9, 9, 9, 6], [6, 9, 9, 9, 6], [4, 6, 6, 6, 4]], dtype=np.float64)
+    assert np.array_equal(y[0, 0], c * taps) and np.array_equal(y[0, 1], c * taps)
+
+
+def test_direct_matches_torch_fixtures():
+    d = np.load(os.path.join(GOLDEN, "torch_conv_f64.npz"))
+    ncase = len([k for k in d.files if k.endswith("_shape")])
+    assert ncase >= 7
+    for i in range(ncase):
+        n, c, h, w, k, p, q = d[f"case{i}_shape"]
+        x = d[f"case{i}_x"].view(np.float16)
+        wt = d[f"case{i}_w"].view(np.float16)
+        y = cdirect.conv2d_f64(x, wt, pad=(int(p), int(q)))
+        np.testing.assert_allclose(y, d[f"case{i}_y"], rtol=1e-12, atol=1e-12)
+
+
+def test_direct_general_geometry_vs_brute_force():
+    # stride 2, 5x5 filter, pad 2: independent numpy brute force on a tiny case
+    x = wg.uniform_pm1(1, (1, 2, 7, 6))
+    w = wg.uniform_pm1(2, (3, 2, 5, 5))
+    y = cdirect.conv2d_f64(x, w, stride=(2, 2), pad=(2, 2))
+    xp = np.pad(x.astype(np.float64), ((0, 0), (0, 0), (2, 2), (2, 2)))
+    ref = np.zeros(y.shape)
+    for k in range(3):
+        for i in range(y.shape[2]):
+            for j in range(y.shape[3]):
+                ref[0, k, i, j] = np.sum(xp[0, :, 2 * i:2 * i + 5, 2 * j:2 * j + 5] * w[k].astype(np.float64))
+    np.testing.assert_allclose(y, ref, rtol=1e-13, atol=1e-13)
+
+
+def test_direct_linearity_exact():
+    # small integers: all sums exact in FP64, so conv(x1 + x2) == conv(x1) + conv(x2) bit for bit
+    rng = np.random.default_rng(4)
+    x1 = rng.integers(-8, 8, (2, 5, 6, 7)).astype(np.float16)
+    x2 = rng.integers(-8, 8, (2, 5, 6, 7)).astype(np.float16)
+    w = rng.integers(-4, 4, (3, 5, 3, 3)).astype(np.float16)
+    assert np.array_equal(cdirect.conv2d_f64(x1 + x2, w), cdirect.conv2d_f64(x1, w) + cdirect.conv2d_f64(x2, w))
+
+
+def test_threaded_direct_bit_identical():
+    x = wg.normal(7, (2, 9, 11, 10))
+    w = wg.normal(8, (13, 9, 3, 3), sigma=0.2)
+    assert np.array_equal(cdirect.conv2d_f64(x, w), cdirect.conv2d_f64_threads(x, w, threads=4))
+
+
+# ---- faithful restatement of the reference checker ----------------------------------------
+def test_faithful_restatement_matches_reference_checker_fixtures():
+    d = np.load(os.path.join(GOLDEN, "ref_checker.npz"))
+    ncase = len([k for k in d.files if k.endswith("_shape")])
+    assert ncase >= 4
+    for i in range(ncase):
+        n, c, h, w, k, r, s, u, v, p, q = (int(t) for t in d[f"case{i}_shape"])
+        x = d[f"case{i}_x"].view(np.float16)
+        wt = d[f"case{i}_w"].view(np.float16)
+        y = cdirect.conv2dcpu_faithful(x, wt, stride=(u, v), pad=(p, q))
+        assert np.array_equal(y.view(np.uint16), d[f"case{i}_y"]), f"case {i}"
+
+
+@pytest.mark.skipif(cdirect.ref_lib() is None, reason="reference checker not built (no reference source here)")
+def test_faithful_restatement_matches_reference_checker_live():
+    for seed, shape in ((31, (1, 7, 6, 9, 5)), (32, (2, 3, 5, 5, 4))):
+        n, c, h, w, k = shape
+        x = wg.normal(seed, (n, c, h, w))
+        wt = wg.normal(seed + 1, (k, c, 3, 3))
+        assert np.array_equal(cdirect.conv2dcpu_faithful(x, wt).view(np.uint16),
+                              cdirect.ref_conv2dcpu(x, wt).view(np.uint16))
+
+
+def test_faithful_rounds_products():
+    # (1 + 2^-10)^2 = 1 + 2^-9 + 2^-20: the FP16-rounded product is 1 + 2^-9 (verfiy.h:32)
+    a = np.full((1, 1, 1, 1), 1 + 2 ** -10, dtype=np.float16)
+    w = np.zeros((1, 1, 3, 3), dtype=np.float16)
+    w[0, 0, 1, 1] = a[0, 0, 0, 0]
+    y = cdirect.conv2dcpu_faithful(a, w)
+    assert float(y[0, 0, 0, 0]) == 1 + 2 ** -9
+    assert cdirect.conv2d_f64(a, w)[0, 0, 0, 0] == (1 + 2 ** -10) ** 2
+
+
+# ---- Winograd model -------------------------------------------------------------------------
+def _correlate_tile(d, g):
+    y = np.zeros((2, 2))
+    for i in range(2):
+        for j in range(2):
+            for r in range(3):
+                for s in range(3):
+                    y[i, j] += d[i + r, j + s] * g[r, s]
+    return y
+
+
+def test_winograd_identity_brute_force_exact():
+    # A^T [(G g G^T) . (B^T d B)] A == 2x2 valid correlation, exactly, on dyadic inputs
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        d = rng.integers(-64, 64, (4, 4)) / 8.0
+        g = rng.integers(-64, 64, (3, 3)) / 16.0
+        u = wg.G @ g @ wg.G.T
+        v = wg.BT @ d @ wg.BT.T
+        y = wg.AT @ (u * v) @ wg.AT.T
+        assert np.array_equal(y, _correlate_tile(d, g))
+
+
+def test_transform_functions_match_matrix_definition():
+    rng = np.random.default_rng(2)
+    w = rng.integers(-50, 50, (3, 4, 3, 3)).astype(np.float16) / np.float16(8)
+    u = wg.filter_transform_f64(w)
+    for k in range(3):
+        for c in range(4):
+            ref = wg.G @ w[k, c].astype(np.float64) @ wg.G.T
+            assert np.array_equal(u[:, k, c].reshape(4, 4), ref)
+    geo = wg.Geometry(1, 2, 5, 6, 1)
+    x = rng.integers(-50, 50, (1, 2, 5, 6)).astype(np.float16)
+    v = wg.input_transform_f64(x, geo)
+    xp = np.zeros((1, 2, 2 * geo.tiles_h + 2, 2 * geo.tiles_w + 2))
+    xp[:, :, 1:6, 1:7] = x
+    for c in range(2):
+        for th in range(geo.tiles_h):
+            for tw in range(geo.tiles_w):
+                d = xp[0, c, 2 * th:2 * th + 4, 2 * tw:2 * tw + 4]
+                ref = wg.BT @ d @ wg.BT.T
+                assert np.array_equal(v[:, c, th * geo.tiles_w + tw].reshape(4, 4), ref)
+
+
+@pytest.mark.parametrize("shape", [
+    (1, 64, 8, 8, 64, 1, 1), (2, 3, 7, 7, 5, 1, 1), (1, 33, 9, 6, 17, 1, 1),
+    (1, 8, 6, 6, 8, 0, 0), (1, 8, 5, 7, 8, 2, 2), (3, 16, 5, 5, 16, 1, 1)])
+def test_winograd_f64_equals_direct(shape):
+    n, c, h, w, k, p, q = shape
+    geo = wg.Geometry(n, c, h, w, k, p, q)
+    x = wg.uniform_pm1(40 + n, (n, c, h, w))
+    wt = wg.uniform_pm1(50 + c, (k, c, 3, 3))
+    y = wg.winograd_f64(x, wt, geo)
+    ref = cdirect.conv2d_f64(x, wt, pad=(p, q))
+    assert y.shape == ref.shape
+    np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
+
+
+def test_fp32_transforms_within_one_ulp_of_exact():
+    w = wg.normal(3, (64, 48, 3, 3), sigma=0.3)
+    u32 = wg.filter_transform_fp32(w).astype(np.float64)
+    u64 = wg.filter_transform_f64(w).astype(np.float16).astype(np.float64)
+    ulp = np.spacing(np.abs(u64).astype(np.float16)).astype(np.float64)
+    assert np.all(np.abs(u32 - u64) <= ulp)
+    assert np.mean(u32 == u64) > 0.999
+    geo = wg.Geometry(2, 20, 11, 9, 4)
+    x = wg.normal(4, (2, 20, 11, 9))
+    v32 = wg.input_transform_fp32(x, geo).astype(np.float64)
+    v64 = wg.input_transform_f64(x, geo).astype(np.float16).astype(np.float64)
+    ulp = np.spacing(np.abs(v64).astype(np.float16)).astype(np.float64)
+    assert np.all(np.abs(v32 - v64) <= ulp)
+    assert np.mean(v32 == v64) > 0.999
+
+
+def test_pack_roundtrip_and_known_offsets():
+    geo = wg.Geometry(2, 40, 7, 9, 20)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((16, geo.k, geo.c)).astype(np.float16)
+    v = rng.standard_normal((16, geo.c, geo.NT)).astype(np.float16)
+    pu, pv = wg.pack_U(u, geo), wg.pack_V(v, geo)
+    assert pu.size == geo.u_halves and pv.size == geo.v_halves
+    assert np.array_equal(wg.unpack_U(pu, geo), u) and np.array_equal(wg.unpack_V(pv, geo), v)
+    # hand-computed offsets: layout [pos][cb][blk][lane][e], lane = (c%32//8)*16 + r%16
+    pos, k, c = 3, 17, 38
+    off = ((pos * geo.CB + c // 32) * geo.KB + k // 16) * 512 + (((c % 32) // 8) * 16 + k % 16) * 8 + c % 8
+    assert pu[off] == u[pos, k, c]
+    pos, g, c = 14, 37, 9
+    off = ((pos * geo.CB + c // 32) * geo.G16 + g // 16) * 512 + (((c % 32) // 8) * 16 + g % 16) * 8 + c % 8
+    assert pv[off] == v[pos, c, g]
+    # padding (channels >= C, rows >= K, columns >= NT) is zero
+    assert np.count_nonzero(pu) == np.count_nonzero(u)
+    assert np.count_nonzero(pv) == np.count_nonzero(v)
+
+
+def test_fp16_method_error_is_small_cfg1():
+    # the FP16 data path (U, V rounded to FP16) stays far inside 1e-2 on the small config
+    geo = wg.Geometry(1, 64, 56, 56, 64)
+    x = wg.uniform_pm1(1, (1, 64, 56, 56))
+    w = wg.uniform_pm1(2, (64, 64, 3, 3))
+    y, _ = wg.winograd_gpu_model(x, w, geo)
+    ref = cdirect.conv2d_f64_threads(x, w)
+    m = wg.error_metrics(y.astype(np.float16), ref)
+    assert m["finite"] and m["max_rel"] < 2e-3 and m["rms_rel"] < 1e-3
+
+
+def test_error_metrics_definitions():
+    ref = np.array([1.0, -2.0, 4.0])
+    y = np.array([1.5, -2.0, 4.0])
+    m = wg.error_metrics(y, ref)
+    assert m["max_rel"] == 0.5 / 4.0
+    assert abs(m["rms_rel"] - 0.5 / np.sqrt(21.0)) < 1e-15
+    # the reference's elementwise criterion (main.cpp:144) flags > 1% with the GPU value as denominator
+    assert wg.reference_criterion_failures(np.array([1.0, 1.02], dtype=np.float16), np.array([1.0, 1.0])) == 1
