@@ -97,6 +97,8 @@ _SIGS = {
     "conv2d_tuned_count": ([], ctypes.c_int),
     "conv2d_tuned_save": ([ctypes.c_char_p], ctypes.c_int),
     "conv2d_tuned_load": ([ctypes.c_char_p], ctypes.c_int),
+    "conv2d_autotune": ([_P(Problem), ctypes.c_int, ctypes.c_void_p, _P(ctypes.c_int), _P(ctypes.c_float),
+                         ctypes.c_int], ctypes.c_int),
     "conv2d_version": ([], ctypes.c_char_p),
 }
 

@@ -139,7 +139,8 @@ Launch slot_gemm(const Geometry& g, int cfg) {
     Launch L;
     L.kernel = c.kernel;
     const int vf = c.bt / 16, uf = c.bk / 16;
-    L.grid = dim3((unsigned)((g.G16 + vf - 1) / vf), (unsigned)((g.KB + uf - 1) / uf), 1);
+    const unsigned ng = (unsigned)((g.G16 + vf - 1) / vf), nk = (unsigned)((g.KB + uf - 1) / uf);
+    L.grid = c.depth > 0 ? dim3(ng * nk, 1, 1) : dim3(ng, nk, 1);
     L.block = dim3((unsigned)c.threads, 1, 1);
     L.lds = 0;
     return L;

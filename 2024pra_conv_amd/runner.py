@@ -84,6 +84,15 @@ class WinogradConv:
     def forward(self, cfg: int = -1, stream=None) -> None:
         check(self.L.conv2d_forward(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_forward")
 
+    def autotune(self, reps: int = 5, stream=None):
+        """Runs the library's autotuner; returns (best_cfg, per-config best times in us, -1 = gated out)."""
+        n = self.L.conv2d_num_tile_configs()
+        times = (ctypes.c_float * n)()
+        best = ctypes.c_int(-1)
+        check(self.L.conv2d_autotune(ctypes.byref(self.pb), reps, _stream_handle(stream), ctypes.byref(best),
+                                     times, n), "conv2d_autotune")
+        return best.value, list(times)
+
     def selected_config(self) -> int:
         c = ctypes.c_int()
         check(self.L.conv2d_select_tile_config(ctypes.byref(self.pb), ctypes.byref(c)), "conv2d_select_tile_config")

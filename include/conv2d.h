@@ -167,6 +167,16 @@ int conv2d_tuned_count(void);
 int conv2d_tuned_save(const char* path);
 int conv2d_tuned_load(const char* path);
 
+/* Autotuner (replaces the reference's hand-picked per-shape kernels, conv2d.cpp:273-494):
+ * times every GEMM tile configuration on this problem (U and V must already be prepared;
+ * problem->out is overwritten) with a deterministic best-of-`reps` protocol on `stream`,
+ * gates each configuration on its output being bit-identical to configuration 0's, and
+ * records the fastest passing one in the tuned table. times_us[i] (i < ntimes) receives the
+ * best time of configuration i in microseconds, or -1 if it failed the gate. Allocates a
+ * temporary output-sized buffer. */
+int conv2d_autotune(const problem_t* problem, int reps, void* stream, int* best_cfg,
+                    float* times_us, int ntimes);
+
 /* Library version string. */
 const char* conv2d_version(void);
 

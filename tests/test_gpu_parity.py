@@ -237,3 +237,21 @@ def test_full_size_all_ones_closed_form(name, runner, cuda):
     expect = (rows[:, None] * cols[None, :]) * c * 2.0 ** -10
     out = conv.out.double().cpu()
     assert torch.equal(out, expect.expand(n, k, h, w))
+
+
+def test_autotuner_gates_times_and_records(runner, wc, cuda):
+    shape = (2, 96, 15, 15, 96, 1, 1)
+    _log(f"case {shape} autotune")
+    x, wt = _inputs(shape, 500)
+    conv = runner.WinogradConv(_to_dev(x, cuda), _to_dev(wt, cuda))
+    conv.L.conv2d_tuned_clear()
+    conv.prepare_weights()
+    conv.input_transform()
+    best, times = conv.autotune(reps=3)
+    assert all(t > 0 for t in times), times  # every configuration passed the parity gate
+    assert times[best] == min(times)
+    assert conv.selected_config() == best and conv.L.conv2d_tuned_count() == 1
+    # the tuned configuration is the one getkernelInfo now puts in slot 3
+    conv._ki = None
+    assert conv.param().cfg == best
+    conv.L.conv2d_tuned_clear()
