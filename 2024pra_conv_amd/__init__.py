@@ -99,6 +99,28 @@ _SIGS = {
     "conv2d_tuned_load": ([ctypes.c_char_p], ctypes.c_int),
     "conv2d_autotune": ([_P(Problem), ctypes.c_int, ctypes.c_void_p, _P(ctypes.c_int), _P(ctypes.c_float),
                          ctypes.c_int], ctypes.c_int),
+    "conv2d_batch_create": ([_P(Problem), ctypes.c_int, _P(ctypes.c_void_p)], ctypes.c_int),
+    "conv2d_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "conv2d_batch_prepare_weights": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_batch_forward": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_batch_forward_separate": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_batch_capture_graph": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_batch_launch_graph": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_batch_autotune": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_batch_layer_config": ([ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_int)], ctypes.c_int),
+    "conv2d_mlp_param_count": ([], ctypes.c_int),
+    "conv2d_mlp_features": ([_P(Problem), ctypes.c_int, _P(ctypes.c_float)], ctypes.c_int),
+    "conv2d_mlp_forward": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+                           ctypes.c_int),
+    "conv2d_mlp_loss_grad": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                              ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_mlp_adam_step": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                              ctypes.c_float, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_mlp_train": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                          ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P(ctypes.c_float),
+                          ctypes.c_void_p], ctypes.c_int),
+    "conv2d_mlp_select": ([ctypes.c_void_p, _P(Problem), _P(ctypes.c_int), _P(ctypes.c_float), ctypes.c_void_p],
+                          ctypes.c_int),
     "conv2d_version": ([], ctypes.c_char_p),
 }
 

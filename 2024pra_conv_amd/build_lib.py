@@ -25,6 +25,8 @@ LIB_SOURCES = [
     "direct_f64.hip",
     "conv2d_plugin.hip",
     "tuner.hip",
+    "batch.hip",
+    "mlp.hip",
 ]
 
 # Packed FP32 VALU forms are disabled for all device code (see DESIGN notes on gfx950).

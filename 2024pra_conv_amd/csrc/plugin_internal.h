@@ -40,4 +40,10 @@ int launch(const Launch& L, mykernelParamType* a, hipStream_t stream);
 // Tuned-table lookup (tuner.hip): configuration index for this exact shape, or -1.
 int tuned_config_lookup(const problem_t* pb);
 
+// Autotuner over a subset of GEMM configurations (tuner.hip): best-of-reps timing of each,
+// gated on bit-identity with the first one's output; does not touch the tuned table.
+// times_us[i] = best time of cfgs[i] (us) or -1 if gated out.
+int autotune_subset(const problem_t* pb, int reps, void* stream, const int* cfgs, int ncfg, int* best_cfg,
+                    float* times_us);
+
 }  // namespace wino

@@ -146,12 +146,16 @@ struct TunerBuffers {
 
 }  // namespace
 
-extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int* best_cfg,
-                               float* times_us, int ntimes) {
+namespace wino {
+
+int autotune_subset(const problem_t* pb, int reps, void* stream, const int* cfgs, int ncfg, int* best_cfg,
+                    float* times_us) {
     if (!pb || !pb->out || !pb->transform_pin || !pb->transform_filter) return CONV2D_ERR_NULL;
-    if (reps < 1) return CONV2D_ERR_INVALID;
-    wino::Geometry g;
-    int rc = wino::require_winograd(pb, &g);
+    if (reps < 1 || ncfg < 1 || !cfgs) return CONV2D_ERR_INVALID;
+    for (int i = 0; i < ncfg; ++i)
+        if (cfgs[i] < 0 || cfgs[i] >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
+    Geometry g;
+    int rc = require_winograd(pb, &g);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const size_t nout = (size_t)pb->n * pb->k * g.Oh * g.Ow;
@@ -162,12 +166,13 @@ extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int*
     mykernelParamType a;
     int best = -1;
     float best_t = 0.f;
-    for (int cfg = 0; cfg < kNumGemmConfigs; ++cfg) {
-        wino::fill_param(pb, g, cfg, &a);
-        const wino::Launch L = wino::slot_gemm(g, cfg);
-        if ((rc = wino::launch(L, &a, st))) return rc;  // warm-up, and the output to check
+    for (int ci = 0; ci < ncfg; ++ci) {
+        const int cfg = cfgs[ci];
+        fill_param(pb, g, cfg, &a);
+        const Launch L = slot_gemm(g, cfg);
+        if ((rc = launch(L, &a, st))) return rc;  // warm-up, and the output to check
         bool pass = true;
-        if (cfg == 0) {
+        if (ci == 0) {
             if (hipMemcpyAsync(B.golden, pb->out, nout * 2, hipMemcpyDeviceToDevice, st) != hipSuccess)
                 return CONV2D_ERR_HIP;
         } else {
@@ -185,7 +190,7 @@ extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int*
             for (int r = 0; r < reps; ++r) {
                 float ms = 0.f;
                 if (hipEventRecord(B.e0, st) != hipSuccess) return CONV2D_ERR_HIP;
-                if ((rc = wino::launch(L, &a, st))) return rc;
+                if ((rc = launch(L, &a, st))) return rc;
                 if (hipEventRecord(B.e1, st) != hipSuccess || hipEventSynchronize(B.e1) != hipSuccess ||
                     hipEventElapsedTime(&ms, B.e0, B.e1) != hipSuccess)
                     return CONV2D_ERR_HIP;
@@ -196,9 +201,25 @@ extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int*
                 best_t = t_min;
             }
         }
-        if (times_us && cfg < ntimes) times_us[cfg] = pass ? t_min * 1000.f : -1.f;
+        if (times_us) times_us[ci] = pass ? t_min * 1000.f : -1.f;
     }
     if (best < 0) return CONV2D_ERR_INVALID;
+    if (best_cfg) *best_cfg = best;
+    return CONV2D_OK;
+}
+
+}  // namespace wino
+
+extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int* best_cfg,
+                               float* times_us, int ntimes) {
+    std::vector<int> all(kNumGemmConfigs);
+    for (int i = 0; i < kNumGemmConfigs; ++i) all[i] = i;
+    std::vector<float> t(kNumGemmConfigs, -1.f);
+    int best = -1;
+    int rc = wino::autotune_subset(pb, reps, stream, all.data(), kNumGemmConfigs, &best, t.data());
+    if (rc) return rc;
+    if (times_us)
+        for (int i = 0; i < ntimes && i < kNumGemmConfigs; ++i) times_us[i] = t[i];
     if (best_cfg) *best_cfg = best;
     return conv2d_tuned_set(pb, best);
 }

@@ -29,6 +29,49 @@ __global__ void wino_input_transform(mykernelParamType prm);
 __global__ void wino_noop(mykernelParamType prm);
 __global__ void conv_direct_f64_kernel(mykernelParamType prm, double* out);
 
+// Per-item bodies shared by the per-layer kernels and the multi-layer persistent kernels.
+__device__ void input_transform_block(const mykernelParamType& prm, int n, int cb, int bi, half_t* lds);
+
+// ---- multi-layer (batched) form ---------------------------------------------------------
+// One layer of a batch, in device memory. Work items of a stage are numbered globally: layer
+// i owns [first, first + items). S2 item -> (band, channel block, image); S3 item ->
+// (k-block, column block) of the layer's batched GEMM configuration.
+struct BatchLayer {
+    mykernelParamType prm;
+    int s2_first, s2_items, nbands;
+    int s3_first, s3_items, nk;  // nk: k-blocks of the layer's GEMM tile (column blocks = items / nk)
+    int bcfg;                    // index into the batched GEMM configurations
+    int pad_;
+};
+
+// Batched GEMM configurations (all 256 threads, so one persistent kernel runs any of them).
+constexpr int kBatchCfgs = 3;
+__host__ __device__ inline int batch_cfg_bk(int b) { return b == 1 ? 128 : (b == 2 ? 32 : 64); }
+__host__ __device__ inline int batch_cfg_bt(int b) { return b == 1 ? 64 : 128; }
+
+__global__ void wino_batch_reset(unsigned* counters);
+__global__ void wino_batch_input_transform(const BatchLayer* __restrict__ layers, int nlayers,
+                                           int total, unsigned* counter);
+__global__ void wino_batch_gemm(const BatchLayer* __restrict__ layers, int nlayers, int total,
+                                unsigned* counter);
+
+// Work-queue helper: thread 0 takes the next item index, every thread gets it.
+__device__ inline int batch_next_item(unsigned* counter, int* slot) {
+    if (threadIdx.x == 0) *slot = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int item = *slot;
+    __syncthreads();
+    return item;
+}
+
+// Layer owning global item `item` (items are contiguous per layer, in layer order).
+__device__ inline int batch_find_layer(const BatchLayer* layers, int nlayers, int item, bool s3) {
+    int L = 0;
+    for (int i = 1; i < nlayers; ++i)
+        if ((s3 ? layers[i].s3_first : layers[i].s2_first) <= item) L = i;
+    return L;
+}
+
 // Input-transform LDS staging: a workgroup stages `band` tile rows of one image and one
 // 32-channel block. LDS row pitch in halves: column index = input column + q + 8, so the
 // 4x4 window of tile column tw starts at the even column 2*tw + 8 (4-byte aligned reads)

@@ -94,19 +94,21 @@ __device__ inline void fold_position(int pos, const floatx4 (&acc)[MI][NI], floa
         }
 }
 
+// One GEMM work item (k-block kblk x column block gblk), rolled position loop. smem must
+// hold 2 * GemmShape::kStageHalves halves. Ends after the epilogue stores; callers reusing
+// smem for another item must __syncthreads() first.
 template <int BK, int BT, int WK, int WT>
-__global__ void __launch_bounds__(WK * WT * 64) wino_gemm_kernel(mykernelParamType prm) {
+__device__ inline void gemm_body_rolled(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
     using S = GemmShape<BK, BT, WK, WT>;
     constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF;
-    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wk = wave % WK;
     const int wt = wave / WK;
-    const int gb0 = blockIdx.x * VF;  // first 16-column block of V
-    const int kb0 = blockIdx.y * UF;  // first 16-row block of U
+    const int gb0 = gblk * VF;  // first 16-column block of V
+    const int kb0 = kblk * UF;  // first 16-row block of U
     const int KB = prm.KB, G16 = prm.G16, CB = prm.CB;
     const int nsteps = 16 * CB;
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
@@ -201,6 +203,12 @@ __global__ void __launch_bounds__(WK * WT * 64) wino_gemm_kernel(mykernelParamTy
     store_tiles<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
 
+template <int BK, int BT, int WK, int WT>
+__global__ void __launch_bounds__(WK * WT * 64) wino_gemm_kernel(mykernelParamType prm) {
+    __shared__ __attribute__((aligned(16))) half_t smem[2 * GemmShape<BK, BT, WK, WT>::kStageHalves];
+    gemm_body_rolled<BK, BT, WK, WT>(prm, blockIdx.y, blockIdx.x, smem);
+}
+
 // ---------------------------------------------------------------------------------------
 // Ring variant: the same arithmetic (bit-identical output) with
 //   * D pipeline steps of global loads in flight in a register ring (one step = one
@@ -221,11 +229,10 @@ __device__ inline void xcd_remap(int L, int total, int nk, int* kblk, int* gblk)
 }
 
 template <int BK, int BT, int WK, int WT, int D>
-__global__ void __launch_bounds__(WK * WT * 64) wino_gemm_ring_kernel(mykernelParamType prm) {
+__device__ inline void gemm_body_ring(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
     using S = GemmShape<BK, BT, WK, WT>;
     constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF;
     static_assert(16 % D == 0, "ring depth must divide the 16 positions' step count");
-    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -233,10 +240,6 @@ __global__ void __launch_bounds__(WK * WT * 64) wino_gemm_ring_kernel(mykernelPa
     const int wk = wave % WK;
     const int wt = wave / WK;
     const int KB = prm.KB, G16 = prm.G16, CB = prm.CB;
-    const int nk = (KB + UF - 1) / UF;
-    const int ng = (G16 + VF - 1) / VF;
-    int kblk, gblk;
-    xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
     const int gb0 = gblk * VF;
     const int kb0 = kblk * UF;
     const int nsteps = 16 * CB;
@@ -335,6 +338,46 @@ __global__ void __launch_bounds__(WK * WT * 64) wino_gemm_ring_kernel(mykernelPa
     }
 
     store_tiles<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
+}
+
+template <int BK, int BT, int WK, int WT, int D>
+__global__ void __launch_bounds__(WK * WT * 64) wino_gemm_ring_kernel(mykernelParamType prm) {
+    using S = GemmShape<BK, BT, WK, WT>;
+    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
+    const int nk = (prm.KB + S::UF - 1) / S::UF;
+    const int ng = (prm.G16 + S::VF - 1) / S::VF;
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
+    gemm_body_ring<BK, BT, WK, WT, D>(prm, kblk, gblk, smem);
+}
+
+// ---------------------------------------------------------------------------------------
+// Multi-layer persistent GEMM: workgroups pull (layer, k-block, column block) items from
+// one queue; each layer runs the tile configuration chosen for it (256-thread shapes, so
+// one kernel covers all; the LDS buffer is sized for the largest).
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) wino_batch_gemm(const BatchLayer* __restrict__ layers, int nlayers,
+                                                       int total, unsigned* counter) {
+    constexpr int kSmem = 2 * GemmShape<64, 128, 1, 4>::kStageHalves;
+    static_assert(2 * GemmShape<128, 64, 2, 2>::kStageHalves <= kSmem, "LDS sizing");
+    static_assert(2 * GemmShape<32, 128, 1, 4>::kStageHalves <= kSmem, "LDS sizing");
+    __shared__ __attribute__((aligned(16))) half_t smem[kSmem];
+    __shared__ int slot;
+    for (;;) {
+        const int item = batch_next_item(counter, &slot);
+        if (item >= total) break;
+        const int L = batch_find_layer(layers, nlayers, item, true);
+        const BatchLayer& bl = layers[L];
+        const int local = item - bl.s3_first;
+        const int kblk = local % bl.nk;
+        const int gblk = local / bl.nk;
+        switch (bl.bcfg) {
+            case 1: gemm_body_rolled<128, 64, 2, 2>(bl.prm, kblk, gblk, smem); break;
+            case 2: gemm_body_rolled<32, 128, 1, 4>(bl.prm, kblk, gblk, smem); break;
+            default: gemm_body_rolled<64, 128, 1, 4>(bl.prm, kblk, gblk, smem); break;
+        }
+        __syncthreads();
+    }
 }
 
 // ---- configuration table --------------------------------------------------------------

@@ -103,7 +103,14 @@ __device__ inline void input_tile_fp32(const float d[16], float v[16]) {
 
 __global__ void __launch_bounds__(256) wino_input_transform(mykernelParamType prm) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
-    const int n = blockIdx.z, cb = blockIdx.y, bi = blockIdx.x;
+    input_transform_block(prm, blockIdx.z, blockIdx.y, blockIdx.x, lds);
+}
+
+// One input-transform work item: tile-row band bi of image n, channel block cb. Called by
+// the per-layer kernel above and by the multi-layer persistent kernel. `lds` must hold
+// input_lds_bytes(prm.band, ...) bytes. Ends with no barrier pending; callers that reuse
+// `lds` for another item must __syncthreads() first.
+__device__ void input_transform_block(const mykernelParamType& prm, int n, int cb, int bi, half_t* lds) {
     const int TH = prm.band;
     const int th0 = bi * TH;
     const int tiles_w = prm.tiles_w, tiles_h = prm.tiles_h;
@@ -213,3 +220,30 @@ __global__ void __launch_bounds__(256) wino_input_transform(mykernelParamType pr
 
 // Slot-4 placeholder: the output transform runs inside the GEMM epilogue.
 __global__ void wino_noop(mykernelParamType) {}
+
+// ---------------------------------------------------------------------------------------
+// Multi-layer persistent input transform: workgroups pull (layer, band, channel block,
+// image) items from one queue until it is empty. Dynamic LDS = the largest layer's staging.
+// ---------------------------------------------------------------------------------------
+__global__ void wino_batch_reset(unsigned* counters) {
+    if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+}
+
+__global__ void __launch_bounds__(256) wino_batch_input_transform(const BatchLayer* __restrict__ layers,
+                                                                  int nlayers, int total, unsigned* counter) {
+    extern __shared__ __attribute__((aligned(16))) half_t lds[];
+    __shared__ int slot;
+    for (;;) {
+        const int item = batch_next_item(counter, &slot);
+        if (item >= total) break;
+        const int L = batch_find_layer(layers, nlayers, item, false);
+        const BatchLayer& bl = layers[L];
+        const int local = item - bl.s2_first;
+        const int CB = bl.prm.CB;
+        const int bi = local % bl.nbands;
+        const int cb = (local / bl.nbands) % CB;
+        const int n = local / (bl.nbands * CB);
+        input_transform_block(bl.prm, n, cb, bi, lds);
+        __syncthreads();
+    }
+}

@@ -125,3 +125,63 @@ def direct_f64(x: torch.Tensor, w: torch.Tensor, stride=(1, 1), pad=(1, 1)) -> t
     check(L.conv2d_direct_f64(ctypes.byref(pb), ctypes.c_void_p(ref.data_ptr()), _stream_handle(None)),
           "conv2d_direct_f64")
     return ref
+
+
+class Batch:
+    """The multi-layer form over several WinogradConv problems (each keeps its own buffers)."""
+
+    def __init__(self, convs: list[WinogradConv]):
+        self.L = load_library()
+        self.convs = convs
+        arr = (type(convs[0].pb) * len(convs))(*[c.pb for c in convs])
+        self._arr = arr
+        h = ctypes.c_void_p()
+        check(self.L.conv2d_batch_create(arr, len(convs), ctypes.byref(h)), "conv2d_batch_create")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            check(self.L.conv2d_batch_destroy(self.h), "conv2d_batch_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare_weights(self, stream=None):
+        check(self.L.conv2d_batch_prepare_weights(self.h, _stream_handle(stream)), "conv2d_batch_prepare_weights")
+
+    def forward(self, stream=None):
+        check(self.L.conv2d_batch_forward(self.h, _stream_handle(stream)), "conv2d_batch_forward")
+
+    def forward_separate(self, stream=None):
+        check(self.L.conv2d_batch_forward_separate(self.h, _stream_handle(stream)), "conv2d_batch_forward_separate")
+
+    def capture_graph(self, stream):
+        check(self.L.conv2d_batch_capture_graph(self.h, _stream_handle(stream)), "conv2d_batch_capture_graph")
+
+    def launch_graph(self, stream=None):
+        check(self.L.conv2d_batch_launch_graph(self.h, _stream_handle(stream)), "conv2d_batch_launch_graph")
+
+    def autotune(self, reps: int = 3, stream=None):
+        check(self.L.conv2d_batch_autotune(self.h, reps, _stream_handle(stream)), "conv2d_batch_autotune")
+
+    def layer_configs(self) -> list[int]:
+        out = []
+        for i in range(len(self.convs)):
+            c = ctypes.c_int()
+            check(self.L.conv2d_batch_layer_config(self.h, i, ctypes.byref(c)), "conv2d_batch_layer_config")
+            out.append(c.value)
+        return out
+
+
+def batch_layer_shapes(seed: int = 4, count: int = 24):
+    """The 24-layer batched workload: seeded shapes N in {1,4,16}, C,K in
+    {3,32,64,96,128,256,512}, H=W in {7,13,14,27,28,56,112}, 3x3 stride 1 pad 1."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns, cs, hs = [1, 4, 16], [3, 32, 64, 96, 128, 256, 512], [7, 13, 14, 27, 28, 56, 112]
+    return [(int(rng.choice(ns)), int(rng.choice(cs)), int(rng.choice(hs)), int(rng.choice(cs)))
+            for _ in range(count)]

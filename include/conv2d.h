@@ -177,6 +177,54 @@ int conv2d_tuned_load(const char* path);
 int conv2d_autotune(const problem_t* problem, int reps, void* stream, int* best_cfg,
                     float* times_us, int ntimes);
 
+/* ---- Multi-layer (batched) form: no counterpart in the reference (its harness runs one
+ * shape per process, main.cpp:8-166). Many layers of uneven shape run as one persistent
+ * input-transform launch and one persistent GEMM launch over (layer, tile-block) work
+ * queues; the pair can be captured once into a hipGraph and replayed with one launch. ---- */
+typedef struct conv2d_batch conv2d_batch;
+/* Validates every layer (all pointers set, Winograd geometry) and uploads descriptors.
+ * Per-layer tile configurations come from the tuned table or the shape heuristic. */
+int conv2d_batch_create(const problem_t* layers, int nlayers, conv2d_batch** batch);
+int conv2d_batch_destroy(conv2d_batch* batch);
+/* Filter transforms of every layer (once per weight set). */
+int conv2d_batch_prepare_weights(conv2d_batch* batch, void* stream);
+/* Input transforms + GEMMs of all layers as two persistent launches. */
+int conv2d_batch_forward(conv2d_batch* batch, void* stream);
+/* The same work as separate per-layer launches with the same tile configurations. */
+int conv2d_batch_forward_separate(conv2d_batch* batch, void* stream);
+/* Captures conv2d_batch_forward into a hipGraph on a non-null stream / replays it. */
+int conv2d_batch_capture_graph(conv2d_batch* batch, void* stream);
+int conv2d_batch_launch_graph(conv2d_batch* batch, void* stream);
+/* Autotunes each layer over the batched configurations (U and V prepared; outputs
+ * overwritten). Invalidates a captured graph. */
+int conv2d_batch_autotune(conv2d_batch* batch, int reps, void* stream);
+/* Tile configuration (global index) used for layer i. */
+int conv2d_batch_layer_config(const conv2d_batch* batch, int i, int* cfg);
+
+/* ---- Learned cost model (no counterpart in the reference): a 21 -> 64 -> 64 -> 1 FP32 MLP
+ * predicting log(GEMM time, us) from log2(n, c, h, w, k) and a one-hot of the tile
+ * configuration. Parameters: float[conv2d_mlp_param_count()] laid out
+ * W1[64][21] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3. All array arguments of the
+ * forward / training calls are device pointers. ---- */
+int conv2d_mlp_param_count(void);
+/* Host: the 21 features of (shape, cfg), cfg in [0, 16). */
+int conv2d_mlp_features(const problem_t* problem, int cfg, float* out21);
+/* y[b] = MLP(X[b]), X: B x 21. */
+int conv2d_mlp_forward(const float* params, const float* X, int B, float* y, void* stream);
+/* Mean-squared-error loss over B samples with targets T and its gradient (5633 floats). */
+int conv2d_mlp_loss_grad(const float* params, const float* X, const float* T, int B, float* grad,
+                         float* loss, void* stream);
+/* One bias-corrected Adam step t >= 1 (beta1 0.9, beta2 0.999, eps 1e-8). */
+int conv2d_mlp_adam_step(float* params, const float* grad, float* m, float* v, int t, float lr,
+                         void* stream);
+/* `steps` full-batch Adam steps continuing from step t0; final loss to *loss_out (host). */
+int conv2d_mlp_train(float* params, float* m, float* v, int t0, const float* X, const float* T,
+                     int B, int steps, float lr, float* loss_out, void* stream);
+/* The configuration with the smallest predicted time (GPU inference over all
+ * configurations); pred_us (host, optional) receives each prediction. */
+int conv2d_mlp_select(const float* params, const problem_t* problem, int* best_cfg,
+                      float* pred_us, void* stream);
+
 /* Library version string. */
 const char* conv2d_version(void);
 

@@ -255,3 +255,70 @@ def test_autotuner_gates_times_and_records(runner, wc, cuda):
     conv._ki = None
     assert conv.param().cfg == best
     conv.L.conv2d_tuned_clear()
+
+
+# ---- multi-layer (batched) form --------------------------------------------------------------
+BATCH_SMALL = [(1, 3, 7, 5), (4, 32, 13, 64), (2, 64, 14, 32), (1, 96, 27, 128), (2, 128, 7, 3),
+               (1, 40, 9, 20), (3, 130, 12, 140), (1, 16, 28, 16)]
+
+
+def _batch_convs(runner, cuda, shapes, seed=600):
+    convs = []
+    for i, (n, c, h, k) in enumerate(shapes):
+        x, wt = _inputs((n, c, h, h, k, 1, 1), seed + i)
+        convs.append(runner.WinogradConv(_to_dev(x, cuda), _to_dev(wt, cuda)))
+    return convs
+
+
+def test_batched_equals_separate_and_graph_replays(runner, cuda):
+    _log("case batched small")
+    convs = _batch_convs(runner, cuda, BATCH_SMALL)
+    b = runner.Batch(convs)
+    b.prepare_weights()
+    b.forward_separate()
+    torch.cuda.synchronize()
+    ref = [c.out.clone() for c in convs]
+    for c in convs:
+        c.out.fill_(float("nan"))
+        c.V.fill_(0x55)
+    b.forward()
+    torch.cuda.synchronize()
+    for i, c in enumerate(convs):
+        assert torch.equal(c.out.view(torch.int16), ref[i].view(torch.int16)), f"layer {i}"
+    # per-layer accuracy against the FP64 direct conv
+    for i, c in enumerate(convs):
+        y64 = c.direct_f64()
+        d = (c.out.double() - y64).abs()
+        assert (d.max() / y64.abs().max()).item() <= 1e-2, f"layer {i}"
+    # hipGraph: capture once, replay several times with eager work in between
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b.capture_graph(s)
+    for rep in range(3):
+        for c in convs:
+            c.out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        b.launch_graph(s)
+        s.synchronize()
+        for i, c in enumerate(convs):
+            assert torch.equal(c.out.view(torch.int16), ref[i].view(torch.int16)), f"replay {rep} layer {i}"
+        _ = torch.ones(1000, device=cuda).sum().item()  # eager work between replays
+    b.close()
+
+
+def test_batched_autotune_keeps_results(runner, cuda):
+    _log("case batched autotune")
+    convs = _batch_convs(runner, cuda, BATCH_SMALL[:4], seed=700)
+    b = runner.Batch(convs)
+    b.prepare_weights()
+    b.forward_separate()
+    torch.cuda.synchronize()
+    ref = [c.out.clone() for c in convs]
+    b.autotune(reps=2)
+    cfgs = b.layer_configs()
+    assert all(c in (1, 2, 4) for c in cfgs)
+    b.forward()
+    torch.cuda.synchronize()
+    for i, c in enumerate(convs):
+        assert torch.equal(c.out.view(torch.int16), ref[i].view(torch.int16)), f"layer {i}"
+    b.close()
