@@ -5,7 +5,7 @@ struct GemmConfig {
     int bk;              // output channels per workgroup
     int bt;              // tiles (GEMM columns) per workgroup
     int threads;         // threads per workgroup
-    int depth;           // 0: rolled kernel, 2-D grid; D > 0: ring kernel with D steps in flight, 1-D XCD-remapped grid
+    int depth;           // global-load steps in flight (register ring); 0 = 2-D grid without XCD remap
     const void* kernel;  // host stub of wino_gemm_kernel<...>
 };
 

@@ -1,0 +1,170 @@
+// Device helpers shared by the FP16 and FP8 kernels (header-only: no relocatable device
+// code is needed across translation units).
+#pragma once
+
+#include "wino_common.h"
+
+// A^T of F(2,3) (reference conv2d.cpp:18-22).
+static constexpr int kWinoAT[2][4] = {{1, 1, 1, 0}, {0, 1, -1, -1}};
+
+// ---- input transform pieces --------------------------------------------------------------
+
+// V = B^T d B for one 4x4 window, FP32, fixed operation order (restated by the oracle).
+//   B^T = [[1,0,-1,0],[0,1,1,0],[0,-1,1,0],[0,1,0,-1]]  (reference preprocessing.cuh:50-54)
+__device__ inline void input_tile_fp32(const float d[16], float v[16]) {
+    float s[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float d0 = d[0 * 4 + j], d1 = d[1 * 4 + j], d2 = d[2 * 4 + j], d3 = d[3 * 4 + j];
+        s[0][j] = d0 - d2;
+        s[1][j] = d1 + d2;
+        s[2][j] = d2 - d1;
+        s[3][j] = d1 - d3;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v[i * 4 + 0] = s[i][0] - s[i][2];
+        v[i * 4 + 1] = s[i][1] + s[i][2];
+        v[i * 4 + 2] = s[i][2] - s[i][1];
+        v[i * 4 + 3] = s[i][1] - s[i][3];
+    }
+}
+
+// The 4x4 window of staged channel `ch` whose top-left LDS element is (lr0, lc0), as FP32.
+// lc0 is even, so each window row is two 4-byte LDS reads.
+__device__ inline void load_window(const half_t* lds, int LR, int LWp, int lr0, int lc0, int ch, float d[16]) {
+    const half_t* base = lds + (ch * LR + lr0) * LWp + lc0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const half2_t lo = *(const half2_t*)(base + i * LWp);
+        const half2_t hi = *(const half2_t*)(base + i * LWp + 2);
+        d[i * 4 + 0] = (float)lo[0];
+        d[i * 4 + 1] = (float)lo[1];
+        d[i * 4 + 2] = (float)hi[0];
+        d[i * 4 + 3] = (float)hi[1];
+    }
+}
+
+// Stages the input rows of tile-row band bi (image n, 32-channel block cb) in LDS: zero fill
+// (padding halo, rows outside the image, channels >= C), then coalesced row loads. Ends with
+// a barrier, so the staging is visible to every wave.
+__device__ inline void stage_input_band(const mykernelParamType& prm, int n, int cb, int bi, half_t* lds) {
+    const int TH = prm.band;
+    const int th0 = bi * TH;
+    const int H = prm.h, W = prm.w, C = prm.c;
+    const int qq = prm.q;
+    const int LR = input_lds_rows(TH);
+    const int LWp = input_lds_pitch(prm.tiles_w, W, qq);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nwaves = blockDim.x >> 6;
+    {
+        const int total16 = (32 * LR * LWp) >> 3;
+        u32x4* l16 = (u32x4*)lds;
+        const u32x4 z = {0u, 0u, 0u, 0u};
+        for (int i = tid; i < total16; i += blockDim.x) l16[i] = z;
+    }
+    __syncthreads();
+    const half_t* in_n = (const half_t*)prm.pin + (size_t)n * C * H * W;
+    const int ih0 = 2 * th0 - prm.p;
+    const int njobs = 32 * LR;
+    if (prm.flags & 1) {
+        const int rsub = lane >> 3, ck = lane & 7, nck = W >> 3;
+        for (int job = wave * 8 + rsub; job < njobs; job += nwaves * 8) {
+            const int ch = job / LR;
+            const int lr = job - ch * LR;
+            const int c = cb * 32 + ch;
+            const int ih = ih0 + lr;
+            if (c >= C || ih < 0 || ih >= H) continue;
+            const half_t* src = in_n + ((size_t)c * H + ih) * W;
+            half_t* dst = lds + (ch * LR + lr) * LWp + qq + 8;
+            for (int j = ck; j < nck; j += 8) {
+                const half8 val = *(const half8*)(src + j * 8);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) dst[j * 8 + e] = val[e];
+            }
+        }
+    } else {
+        for (int job = wave; job < njobs; job += nwaves) {
+            const int ch = job / LR;
+            const int lr = job - ch * LR;
+            const int c = cb * 32 + ch;
+            const int ih = ih0 + lr;
+            if (c >= C || ih < 0 || ih >= H) continue;
+            const half_t* src = in_n + ((size_t)c * H + ih) * W;
+            half_t* dst = lds + (ch * LR + lr) * LWp + qq + 8;
+            for (int iw = lane; iw < W; iw += 64) dst[iw] = src[iw];
+        }
+    }
+    __syncthreads();
+}
+
+// ---- GEMM epilogue pieces -----------------------------------------------------------------
+
+// Folds the FP32 accumulators of position pos into the four output accumulators:
+// Y[i][j] += A^T[i][a] A^T[j][b] M(a,b), pos = 4a + b. Coefficients are wave-uniform values
+// in {-1, 0, 1}; zero ones are skipped with a uniform branch.
+template <int MI, int NI>
+__device__ inline void fold_position(int pos, const floatx4 (&acc)[MI][NI], floatx4 (&Y)[4][MI][NI]) {
+    const int pa = pos >> 2, pb = pos & 3;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int coef = kWinoAT[i][pa] * kWinoAT[j][pb];
+            if (coef == 0) continue;
+            const float cf = (float)coef;
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni) Y[i * 2 + j][mi][ni] += cf * acc[mi][ni];
+        }
+}
+
+// The 2x2 output tile of each (k, g) this lane holds, FP32 -> FP16, ragged bottom/right
+// edges guarded. gbw / kbw: the wave's first 16-column / 16-row block. MFMA 16x16
+// accumulator layout: element j of lane l is row 4*(l/16) + j, column l % 16.
+template <int MI, int NI>
+__device__ inline void store_tiles(const mykernelParamType& prm, const floatx4 (&Y)[4][MI][NI], int gbw, int kbw,
+                                   int lane) {
+    const int T = prm.T, tiles_w = prm.tiles_w, K = prm.k, Oh = prm.Oh, Ow = prm.Ow;
+    const int NT = prm.NT;
+    half_t* __restrict__ out = (half_t*)prm.pout;
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+        const int g = (gbw + ni) * 16 + (lane & 15);
+        if (g >= NT) continue;
+        const int img = g / T;
+        const int t = g - img * T;
+        const int th = t / tiles_w;
+        const int tw = t - th * tiles_w;
+        const int oh = 2 * th, ow = 2 * tw;
+        const bool r1 = oh + 1 < Oh, c1 = ow + 1 < Ow;
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = (kbw + mi) * 16 + 4 * (lane >> 4) + j;
+                if (k >= K) continue;
+                half_t* o = out + (((size_t)img * K + k) * Oh + oh) * Ow + ow;
+                o[0] = (half_t)Y[0][mi][ni][j];
+                if (c1) o[1] = (half_t)Y[1][mi][ni][j];
+                if (r1) {
+                    o[Ow] = (half_t)Y[2][mi][ni][j];
+                    if (c1) o[Ow + 1] = (half_t)Y[3][mi][ni][j];
+                }
+            }
+    }
+}
+
+// Hardware dispatches workgroup ids round-robin over the 8 XCDs; this remaps id L so every
+// XCD gets a contiguous run of the (k-block fastest) work list, keeping the k-blocks that
+// share one column block of V adjacent on one XCD (same L2).
+__device__ inline void xcd_remap(int L, int total, int nk, int* kblk, int* gblk) {
+    const int xcd = L & 7, j = L >> 3;
+    const int base = total >> 3, rem = total & 7;
+    const int m = xcd * base + min(xcd, rem) + j;
+    *kblk = m % nk;
+    *gblk = m / nk;
+}

@@ -197,8 +197,9 @@ def input_transform_f64(x16: np.ndarray, geo: Geometry) -> np.ndarray:
     return _to_g(v.reshape(16, *v.shape[2:]), geo)
 
 
-def input_transform_fp32(x16: np.ndarray, geo: Geometry) -> np.ndarray:
-    """The kernel's FP32 operation order (wino_transforms.hip input_tile_fp32), FP16 out."""
+def input_transform_fp32(x16: np.ndarray, geo: Geometry, keep_fp32: bool = False) -> np.ndarray:
+    """The kernel's FP32 operation order (wino_transforms.hip input_tile_fp32), FP16 out
+    (FP32 out with keep_fp32, as the FP8 input transform consumes it)."""
     d = input_windows(x16, geo).astype(np.float32)
     s = [[None] * 4 for _ in range(4)]
     for j in range(4):
@@ -207,12 +208,12 @@ def input_transform_fp32(x16: np.ndarray, geo: Geometry) -> np.ndarray:
         s[1][j] = d1 + d2
         s[2][j] = d2 - d1
         s[3][j] = d1 - d3
-    v = np.empty((16,) + d.shape[:4], dtype=np.float16)
+    v = np.empty((16,) + d.shape[:4], dtype=np.float32 if keep_fp32 else np.float16)
     for i in range(4):
-        v[i * 4 + 0] = (s[i][0] - s[i][2]).astype(np.float16)
-        v[i * 4 + 1] = (s[i][1] + s[i][2]).astype(np.float16)
-        v[i * 4 + 2] = (s[i][2] - s[i][1]).astype(np.float16)
-        v[i * 4 + 3] = (s[i][1] - s[i][3]).astype(np.float16)
+        v[i * 4 + 0] = (s[i][0] - s[i][2]).astype(v.dtype)
+        v[i * 4 + 1] = (s[i][1] + s[i][2]).astype(v.dtype)
+        v[i * 4 + 2] = (s[i][2] - s[i][1]).astype(v.dtype)
+        v[i * 4 + 3] = (s[i][1] - s[i][3]).astype(v.dtype)
     return _to_g(v, geo)
 
 

@@ -90,7 +90,8 @@ def test_getkernelinfo_fills_slots_and_param(wc):
     assert ki.slot(4)["grid"] == (1, 1, 1)
     cfg = wc.tile_configs()[prm.cfg]
     assert ki.slot(3)["block"][0] == cfg["threads"]
-    assert ki.slot(3)["grid"] == (-(-prm.G16 // (cfg["bt"] // 16)), -(-prm.KB // (cfg["bk"] // 16)), 1)
+    # one-dimensional grid over (column block, k-block) pairs, remapped per XCD in the kernel
+    assert ki.slot(3)["grid"] == (-(-prm.G16 // (cfg["bt"] // 16)) * -(-prm.KB // (cfg["bk"] // 16)), 1, 1)
     assert ki.slot(2)["grid"] == (-(-28 // prm.band), 8, 32)
     assert ki.slot(1)["grid"][0] * 256 >= prm.KB * prm.CB * 64
 
