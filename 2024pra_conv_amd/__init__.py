@@ -72,7 +72,7 @@ class ParamType(ctypes.Structure):
                [(f, ctypes.c_short) for f in
                 ("n", "c", "h", "w", "k", "r", "s", "u", "v", "p", "q", "Oh", "Ow")] + \
                [(f, ctypes.c_int) for f in
-                ("tiles_h", "tiles_w", "T", "NT", "G16", "CB", "KB", "band", "cfg", "flags")]
+                ("tiles_h", "tiles_w", "T", "NT", "G32", "CB", "KB32", "band", "cfg", "flags")]
 
 
 _lib = None

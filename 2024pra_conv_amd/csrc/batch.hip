@@ -70,9 +70,9 @@ int upload(conv2d_batch* b) {
         bl.s2_first = s2;
         bl.s2_items = bl.nbands * g.CB * (int)pb.n;
         s2 += bl.s2_items;
-        const int uf = batch_cfg_bk(b->bcfg[i]) / 16, vf = batch_cfg_bt(b->bcfg[i]) / 16;
-        bl.nk = (g.KB + uf - 1) / uf;
-        const int ng = (int)((g.G16 + vf - 1) / vf);
+        const int uf = batch_cfg_bk(b->bcfg[i]) / 32, vf = batch_cfg_bt(b->bcfg[i]) / 32;
+        bl.nk = (g.KB32 + uf - 1) / uf;
+        const int ng = (int)((g.G32 + vf - 1) / vf);
         bl.s3_first = s3;
         bl.s3_items = bl.nk * ng;
         s3 += bl.s3_items;

@@ -42,11 +42,11 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
     g->T = (long long)g->tiles_h * g->tiles_w;
     g->NT = g->T * pb->n;
     if (g->NT > (1LL << 30)) return CONV2D_ERR_OVERFLOW;
-    g->G16 = (g->NT + 15) / 16;
+    g->G32 = (g->NT + 31) / 32;
     g->CB = ((int)pb->c + 31) / 32;
-    g->KB = ((int)pb->k + 15) / 16;
-    g->v_bytes = (size_t)16 * g->CB * (size_t)g->G16 * kFragHalves * 2;
-    g->u_bytes = (size_t)16 * g->CB * (size_t)g->KB * kFragHalves * 2;
+    g->KB32 = ((int)pb->k + 31) / 32;
+    g->v_bytes = (size_t)16 * 2 * g->CB * (size_t)g->G32 * kFragHalves * 2;
+    g->u_bytes = (size_t)16 * 2 * g->CB * (size_t)g->KB32 * kFragHalves * 2;
     // input-transform band: largest of 8/4/2/1 tile rows whose LDS staging fits in 64 KiB,
     // preferring 4 (keeps several workgroups per CU)
     g->band = 0;
@@ -107,9 +107,9 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
     a->tiles_w = g.tiles_w;
     a->T = (int)g.T;
     a->NT = (int)g.NT;
-    a->G16 = (int)g.G16;
+    a->G32 = (int)g.G32;
     a->CB = g.CB;
-    a->KB = g.KB;
+    a->KB32 = g.KB32;
     a->band = g.band;
     a->cfg = cfg;
     a->flags = ((pb->w % 8) == 0 && (((uintptr_t)pb->in) & 15) == 0) ? 1 : 0;
@@ -118,7 +118,7 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
 Launch slot_filter(const Geometry& g) {
     Launch L;
     L.kernel = (const void*)&wino_filter_transform;
-    const long long threads = (long long)g.KB * g.CB * 64;
+    const long long threads = (long long)g.KB32 * 2 * g.CB * 64;
     L.grid = dim3((unsigned)((threads + 255) / 256), 1, 1);
     L.block = dim3(256, 1, 1);
     L.lds = 0;
@@ -138,8 +138,8 @@ Launch slot_gemm(const Geometry& g, int cfg) {
     const GemmConfig& c = kGemmConfigs[cfg];
     Launch L;
     L.kernel = c.kernel;
-    const int vf = c.bt / 16, uf = c.bk / 16;
-    const unsigned ng = (unsigned)((g.G16 + vf - 1) / vf), nk = (unsigned)((g.KB + uf - 1) / uf);
+    const int vf = c.bt / 32, uf = c.bk / 32;
+    const unsigned ng = (unsigned)((g.G32 + vf - 1) / vf), nk = (unsigned)((g.KB32 + uf - 1) / uf);
     L.grid = c.depth > 0 ? dim3(ng * nk, 1, 1) : dim3(ng, nk, 1);
     L.block = dim3((unsigned)c.threads, 1, 1);
     L.lds = 0;

@@ -13,8 +13,8 @@ struct Geometry {
     int Oh, Ow;
     bool winograd_ok;
     int tiles_h, tiles_w;
-    long long T, NT, G16;
-    int CB, KB;
+    long long T, NT, G32;  // G32: 32-column blocks of V
+    int CB, KB32;          // CB: 32-channel blocks; KB32: 32-row blocks of U
     int band;
     size_t v_bytes, u_bytes;
 };

@@ -10,6 +10,7 @@ typedef _Float16 half_t;
 typedef half_t half8 __attribute__((ext_vector_type(8)));
 typedef half_t half2_t __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // 16 B vector of raw bits, used for global <-> LDS copies.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));

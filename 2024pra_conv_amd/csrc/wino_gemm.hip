@@ -11,10 +11,12 @@
 // FP16 output is rounded once from FP32.
 //
 // Workgroup tile: BK output channels x BT tiles (GEMM columns g = image*T + tile). Waves
-// form a WK x WT grid; a wave owns (BK/WK) x (BT/WT) as 16x16 sub-tiles of
-// v_mfma_f32_16x16x32_f16. One pipeline step = one 32-channel chunk of one position: BK/16
-// fragments of U and BT/16 of V, contiguous 1 KiB blocks in the packed layouts, copied into
-// one of two LDS buffers while the other feeds the MFMAs.
+// form a WK x WT grid; a wave owns (BK/WK) x (BT/WT) as 32x32 sub-tiles of
+// v_mfma_f32_32x32x16_f16 (measured on MI355X: ~2.1 PFLOP/s sustained in a register loop,
+// against ~1.2 for 16x16x32 with the same accumulator count). One pipeline step = one
+// 32-channel chunk of one position = two 16-channel MFMA k-steps: 2 x BK/32 fragments of U
+// and 2 x BT/32 of V, contiguous 1 KiB blocks in the packed layouts, copied into one of two
+// LDS buffers while the other feeds the MFMAs.
 //
 // Global loads run D steps ahead in a register ring. The copies are branch-free: a piece
 // whose fragment lies beyond the matrix reads an in-bounds dummy fragment instead (its
@@ -27,58 +29,67 @@
 
 namespace {
 
+// KS 16-channel substeps per pipeline step (one v_mfma_f32_32x32x16_f16 k-step each).
+constexpr int KS = 2;
+
 template <int BK, int BT, int WK, int WT>
 struct GemmShape {
     static constexpr int kThreads = WK * WT * 64;
-    static constexpr int MI = BK / WK / 16;  // 16-row sub-tiles per wave
-    static constexpr int NI = BT / WT / 16;  // 16-column sub-tiles per wave
-    static constexpr int UF = BK / 16;       // U fragments per step
-    static constexpr int VF = BT / 16;       // V fragments per step
-    static constexpr int kPieces = (UF + VF) * 64;  // 16 B pieces per step
+    static constexpr int MI = BK / WK / 32;  // 32-row sub-tiles per wave
+    static constexpr int NI = BT / WT / 32;  // 32-column sub-tiles per wave
+    static constexpr int UF = BK / 32;       // U fragments per substep
+    static constexpr int VF = BT / 32;       // V fragments per substep
+    static constexpr int kPieces = KS * (UF + VF) * 64;  // 16 B pieces per step
     static constexpr int kLoads = (kPieces + kThreads - 1) / kThreads;
     // stage size, padded so every thread copies kLoads pieces (the padding is never read)
     static constexpr int kStageHalves = kLoads * kThreads * 8;
-    static_assert(MI >= 1 && NI >= 1, "wave tile must be at least 16x16");
-    static_assert(BK % (WK * 16) == 0 && BT % (WT * 16) == 0, "tile must divide evenly");
+    static_assert(MI >= 1 && NI >= 1, "wave tile must be at least 32x32");
+    static_assert(BK % (WK * 32) == 0 && BT % (WT * 32) == 0, "tile must divide evenly");
 };
 
 }  // namespace
 
 // One GEMM work item (k-block kblk x column block gblk). smem holds 2 stages. Ends after
 // the epilogue stores; callers reusing smem for another item must __syncthreads() first.
+// Stage layout: substep kk = 0..KS-1, each [UF U fragments][VF V fragments].
 template <int BK, int BT, int WK, int WT, int D>
 __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
     using S = GemmShape<BK, BT, WK, WT>;
     constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF, NL = S::kLoads;
-    static_assert(16 % D == 0, "ring depth must divide the step count (a multiple of 16)");
+    static_assert(16 % D == 0 && D % 2 == 0, "ring depth: even, dividing the step count (a multiple of 16)");
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wk = wave % WK;
     const int wt = wave / WK;
-    const int KB = prm.KB, G16 = prm.G16, CB = prm.CB;
-    const int gb0 = gblk * VF;  // first 16-column block of V
-    const int kb0 = kblk * UF;  // first 16-row block of U
-    const int nsteps = 16 * CB;
+    const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
+    const int gb0 = gblk * VF;  // first 32-column block of V
+    const int kb0 = kblk * UF;  // first 32-row block of U
+    const int nsteps = 16 * CB;  // (position, 32-channel chunk) pairs
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
 
-    // copy slot i of this thread: piece pc = tid + i * threads of every step
+    // copy slot i of this thread: piece pc = tid + i * threads of every step. Step s covers
+    // 16-channel blocks 2s and 2s+1 of the [pos][c16] sequence (CB 32-channel chunks per
+    // position), so fragment addresses advance by 2 blocks per step.
     const half_t* src[NL];
     size_t stride[NL];
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
         const int pc = tid + i * S::kThreads;
-        if (pc < UF * 64) {
-            const bool ok = kb0 + pc / 64 < KB;
-            src[i] = U + (ok ? (size_t)kb0 * kFragHalves + (size_t)pc * 8 : (size_t)(pc % 64) * 8);
-            stride[i] = (size_t)KB * kFragHalves;
+        const int p = pc < S::kPieces ? pc : 0;  // padding slots copy a dummy piece
+        const int kk = p / ((UF + VF) * 64);
+        const int q = p % ((UF + VF) * 64);
+        if (q < UF * 64) {
+            const bool ok = pc < S::kPieces && kb0 + q / 64 < KB;
+            src[i] = U + (ok ? ((size_t)kk * KB + kb0) * kFragHalves + (size_t)q * 8 : (size_t)(q % 64) * 8);
+            stride[i] = (size_t)2 * KB * kFragHalves;
         } else {
-            const int pv = pc < S::kPieces ? pc - UF * 64 : 0;  // padding slots copy a dummy piece
-            const bool ok = pc < S::kPieces && gb0 + pv / 64 < G16;
-            src[i] = V + (ok ? (size_t)gb0 * kFragHalves + (size_t)pv * 8 : (size_t)(pv % 64) * 8);
-            stride[i] = (size_t)G16 * kFragHalves;
+            const int qv = q - UF * 64;
+            const bool ok = pc < S::kPieces && gb0 + qv / 64 < G32;
+            src[i] = V + (ok ? ((size_t)kk * G32 + gb0) * kFragHalves + (size_t)qv * 8 : (size_t)(qv % 64) * 8);
+            stride[i] = (size_t)2 * G32 * kFragHalves;
         }
     }
 
@@ -93,24 +104,47 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
         for (int i = 0; i < NL; ++i) *(half8*)(smem + buf * S::kStageHalves + (tid + i * S::kThreads) * 8) = slot[i];
     };
 
-    floatx4 acc[MI][NI];
-    floatx4 Y[4][MI][NI];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int mi = 0; mi < MI; ++mi)
-#pragma unroll
-            for (int ni = 0; ni < NI; ++ni) Y[q][mi][ni] = floatx4{0.f, 0.f, 0.f, 0.f};
+    floatx16 acc[MI][NI];
+    floatx16 Y[4][MI][NI];
 #pragma unroll
     for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int ni = 0; ni < NI; ++ni) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Y[q][mi][ni] = acc[mi][ni];
+        }
 
-    // prologue: step 0 into LDS buffer 0, steps 1..D into the ring (slot = step % D)
+    // MFMA operands of one step, double-buffered in registers
+    half8 ra[2][KS][MI], rb[2][KS][NI];
+    auto lread = [&](half8 (&A)[KS][MI], half8 (&B)[KS][NI], int buf) {
+        const half_t* sb = smem + buf * S::kStageHalves;
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const half_t* sk = sb + kk * (UF + VF) * kFragHalves;
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi) A[kk][mi] = *(const half8*)(sk + ((wk * MI + mi) * 64 + lane) * 8);
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni)
+                B[kk][ni] = *(const half8*)(sk + UF * kFragHalves + ((wt * NI + ni) * 64 + lane) * 8);
+        }
+    };
+
+    // Pipeline (slot = step % D in the register ring; LDS buffer = step & 1):
+    //   prologue: LDS0 <- step 0, regs <- LDS0, LDS1 <- step 1, ring <- steps 2..D+1
+    //   iteration s: MFMAs on regs(s) (asynchronous) | regs(s+1) <- LDS[(s+1)&1]
+    //                | LDS[s&1] <- step s+2 (that buffer was read into regs(s) before the
+    //                previous barrier) | ring refill with step s+2+D | barrier
     gload(ring[0], 0);
     lstore(ring[0], 0);
+    gload(ring[0], D);
 #pragma unroll
-    for (int d = 1; d <= D; ++d) gload(ring[d % D], d);
+    for (int d = 1; d < D; ++d) gload(ring[d], d);
+    __syncthreads();
+    lread(ra[0], rb[0], 0);
+    lstore(ring[1 % D], 1);
+    gload(ring[1 % D], 1 + D);
     __syncthreads();
 
     int pos = 0, cb = 0;
@@ -118,43 +152,49 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const int s = s0 + d;
-            const half_t* sb = smem + (s & 1) * S::kStageHalves;
-            half8 a[MI], b[NI];
+            const int cur = d & 1;  // D is even, so s & 1 == d & 1
 #pragma unroll
-            for (int mi = 0; mi < MI; ++mi) a[mi] = *(const half8*)(sb + ((wk * MI + mi) * 64 + lane) * 8);
-#pragma unroll
-            for (int ni = 0; ni < NI; ++ni)
-                b[ni] = *(const half8*)(sb + UF * kFragHalves + ((wt * NI + ni) * 64 + lane) * 8);
-#pragma unroll
-            for (int mi = 0; mi < MI; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < NI; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
-            // step s+1 (ring slot (d+1)%D) -> the other LDS buffer, then refill that slot
-            lstore(ring[(d + 1) % D], (s + 1) & 1);
-            gload(ring[(d + 1) % D], s + 1 + D);
-            __syncthreads();
-            if (++cb == CB) {
-                fold_position<MI, NI>(pos, acc, Y);
+            for (int kk = 0; kk < KS; ++kk)
 #pragma unroll
                 for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-                    for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = floatx4{0.f, 0.f, 0.f, 0.f};
+                    for (int ni = 0; ni < NI; ++ni)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[cur][kk][mi], rb[cur][kk][ni],
+                                                                             acc[mi][ni], 0, 0, 0);
+            lread(ra[cur ^ 1], rb[cur ^ 1], cur ^ 1);
+            lstore(ring[(d + 2) % D], cur);
+            gload(ring[(d + 2) % D], s + 2 + D);
+            __syncthreads();
+            if (++cb == CB) {
+                fold_position(pos, acc, Y);
+#pragma unroll
+                for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
                 cb = 0;
                 ++pos;
             }
         }
     }
 
-    store_tiles<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
+    store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
 
+// Waves per SIMD the register allocation must allow: two for 64x32 wave tiles (so two waves
+// of a SIMD overlap LDS traffic with MFMAs), one for 64x64 tiles (their output accumulators
+// alone take 256 registers).
+template <int BK, int BT, int WK, int WT>
+constexpr int kWavesPerEU = (BK / WK / 32) * (BT / WT / 32) >= 4 ? 1 : 2;
+
 template <int BK, int BT, int WK, int WT, int D>
-__global__ void __launch_bounds__(WK * WT * 64) wino_gemm_kernel(mykernelParamType prm) {
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
+wino_gemm_kernel(mykernelParamType prm) {
     using S = GemmShape<BK, BT, WK, WT>;
     __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
-    const int nk = (prm.KB + S::UF - 1) / S::UF;
-    const int ng = (prm.G16 + S::VF - 1) / S::VF;
+    const int nk = (prm.KB32 + S::UF - 1) / S::UF;
+    const int ng = (prm.G32 + S::VF - 1) / S::VF;
     int kblk, gblk;
     xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
     gemm_body<BK, BT, WK, WT, D>(prm, kblk, gblk, smem);
@@ -201,15 +241,15 @@ const GemmConfig kGemmConfigs[] = {
     WINO_CFG(64, 64, 1, 2, 2),    // 3: wave 64x32, 2 waves
     WINO_CFG(32, 128, 1, 4, 2),   // 4: wave 32x32, 4 waves (batched set)
     WINO_CFG(256, 64, 4, 2, 2),   // 5: wave 64x32, 8 waves
-    WINO_CFG(16, 256, 1, 8, 2),   // 6: wave 16x32, 8 waves (tiny K)
+    WINO_CFG(32, 256, 1, 8, 2),   // 6: wave 32x32, 8 waves (small K)
     WINO_CFG(64, 256, 1, 8, 2),   // 7: wave 64x32, 8 waves
     WINO_CFG(128, 128, 2, 4, 4),  // 8
-    WINO_CFG(64, 128, 1, 4, 4),   // 9
-    WINO_CFG(128, 64, 2, 2, 4),   // 10
+    WINO_CFG(128, 128, 2, 2, 2),  // 9: wave 64x64, 4 waves (one per SIMD)
+    WINO_CFG(32, 64, 1, 2, 2),    // 10: wave 32x32, 2 waves (small problems)
     WINO_CFG(256, 64, 4, 2, 4),   // 11
-    WINO_CFG(128, 128, 2, 2, 2),  // 12: wave 64x64, 4 waves (one wave per SIMD)
-    WINO_CFG(128, 128, 2, 2, 4),  // 13
-    WINO_CFG(32, 128, 1, 4, 4),   // 14
-    WINO_CFG(64, 64, 1, 2, 4),    // 15
+    WINO_CFG(64, 128, 1, 2, 2),   // 12: wave 64x64, 2 waves
+    WINO_CFG(64, 128, 1, 4, 4),   // 13
+    WINO_CFG(128, 64, 2, 2, 4),   // 14
+    WINO_CFG(32, 128, 1, 4, 4),   // 15
 };
 const int kNumGemmConfigs = (int)(sizeof(kGemmConfigs) / sizeof(kGemmConfigs[0]));

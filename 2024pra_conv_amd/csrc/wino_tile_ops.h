@@ -4,8 +4,6 @@
 
 #include "wino_common.h"
 
-// A^T of F(2,3) (reference conv2d.cpp:18-22).
-static constexpr int kWinoAT[2][4] = {{1, 1, 1, 0}, {0, 1, -1, -1}};
 
 // ---- input transform pieces --------------------------------------------------------------
 
@@ -105,14 +103,19 @@ __device__ inline void stage_input_band(const mykernelParamType& prm, int n, int
 // Folds the FP32 accumulators of position pos into the four output accumulators:
 // Y[i][j] += A^T[i][a] A^T[j][b] M(a,b), pos = 4a + b. Coefficients are wave-uniform values
 // in {-1, 0, 1}; zero ones are skipped with a uniform branch.
-template <int MI, int NI>
-__device__ inline void fold_position(int pos, const floatx4 (&acc)[MI][NI], floatx4 (&Y)[4][MI][NI]) {
+// A^T[i][a] of F(2,3), A^T = [[1,1,1,0],[0,1,-1,-1]] (reference conv2d.cpp:18-22), computed
+// arithmetically (scalar ALU): reading the table from memory would issue
+// scalar loads whose s_waitcnt lgkmcnt(0) also waits for every LDS operation in flight.
+__device__ inline int wino_at(int i, int a) { return i == 0 ? (a != 3 ? 1 : 0) : (a == 0 ? 0 : (a == 1 ? 1 : -1)); }
+
+template <typename Acc, int MI, int NI>
+__device__ inline void fold_position(int pos, const Acc (&acc)[MI][NI], Acc (&Y)[4][MI][NI]) {
     const int pa = pos >> 2, pb = pos & 3;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int coef = kWinoAT[i][pa] * kWinoAT[j][pb];
+            const int coef = wino_at(i, pa) * wino_at(j, pb);
             if (coef == 0) continue;
             const float cf = (float)coef;
 #pragma unroll
@@ -122,38 +125,76 @@ __device__ inline void fold_position(int pos, const floatx4 (&acc)[MI][NI], floa
         }
 }
 
-// The 2x2 output tile of each (k, g) this lane holds, FP32 -> FP16, ragged bottom/right
-// edges guarded. gbw / kbw: the wave's first 16-column / 16-row block. MFMA 16x16
-// accumulator layout: element j of lane l is row 4*(l/16) + j, column l % 16.
+// Output position of GEMM column g: image, and top-left output row/column of its 2x2 tile.
+struct TileCoord {
+    int img, oh, ow;
+    bool r1, c1;  // second row / column inside the image (ragged bottom / right edge)
+};
+__device__ inline TileCoord tile_coord(const mykernelParamType& prm, int g) {
+    TileCoord tc;
+    tc.img = g / prm.T;
+    const int t = g - tc.img * prm.T;
+    const int th = t / prm.tiles_w;
+    const int tw = t - th * prm.tiles_w;
+    tc.oh = 2 * th;
+    tc.ow = 2 * tw;
+    tc.r1 = tc.oh + 1 < prm.Oh;
+    tc.c1 = tc.ow + 1 < prm.Ow;
+    return tc;
+}
+
+__device__ inline void store_tile(const mykernelParamType& prm, const TileCoord& tc, int k, float y00, float y01,
+                                  float y10, float y11) {
+    const int Oh = prm.Oh, Ow = prm.Ow;
+    half_t* o = (half_t*)prm.pout + (((size_t)tc.img * prm.k + k) * Oh + tc.oh) * Ow + tc.ow;
+    o[0] = (half_t)y00;
+    if (tc.c1) o[1] = (half_t)y01;
+    if (tc.r1) {
+        o[Ow] = (half_t)y10;
+        if (tc.c1) o[Ow + 1] = (half_t)y11;
+    }
+}
+
+// Epilogue for 32x32 MFMA accumulators: the 2x2 output tile of each (k, g) this lane
+// holds, FP32 -> FP16, ragged edges guarded. gbw / kbw: the wave's first 32-column / 32-row
+// block. v_mfma_f32_32x32x16 accumulator layout: element j of lane l is row
+// 8*(j/4) + 4*(l/32) + j%4, column l%32.
 template <int MI, int NI>
-__device__ inline void store_tiles(const mykernelParamType& prm, const floatx4 (&Y)[4][MI][NI], int gbw, int kbw,
-                                   int lane) {
-    const int T = prm.T, tiles_w = prm.tiles_w, K = prm.k, Oh = prm.Oh, Ow = prm.Ow;
-    const int NT = prm.NT;
-    half_t* __restrict__ out = (half_t*)prm.pout;
+__device__ inline void store_tiles32(const mykernelParamType& prm, const floatx16 (&Y)[4][MI][NI], int gbw,
+                                     int kbw, int lane) {
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+        const int g = (gbw + ni) * 32 + (lane & 31);
+        if (g >= prm.NT) continue;
+        const TileCoord tc = tile_coord(prm, g);
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int k = (kbw + mi) * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                if (k >= prm.k) continue;
+                store_tile(prm, tc, k, Y[0][mi][ni][j], Y[1][mi][ni][j], Y[2][mi][ni][j], Y[3][mi][ni][j]);
+            }
+    }
+}
+
+// Epilogue for 16x16 MFMA accumulators (element j of lane l is row 4*(l/16) + j, column
+// l%16); gbw / kbw count 16-column / 16-row blocks.
+template <int MI, int NI>
+__device__ inline void store_tiles16(const mykernelParamType& prm, const floatx4 (&Y)[4][MI][NI], int gbw,
+                                     int kbw, int lane) {
 #pragma unroll
     for (int ni = 0; ni < NI; ++ni) {
         const int g = (gbw + ni) * 16 + (lane & 15);
-        if (g >= NT) continue;
-        const int img = g / T;
-        const int t = g - img * T;
-        const int th = t / tiles_w;
-        const int tw = t - th * tiles_w;
-        const int oh = 2 * th, ow = 2 * tw;
-        const bool r1 = oh + 1 < Oh, c1 = ow + 1 < Ow;
+        if (g >= prm.NT) continue;
+        const TileCoord tc = tile_coord(prm, g);
 #pragma unroll
         for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int k = (kbw + mi) * 16 + 4 * (lane >> 4) + j;
-                if (k >= K) continue;
-                half_t* o = out + (((size_t)img * K + k) * Oh + oh) * Ow + ow;
-                o[0] = (half_t)Y[0][mi][ni][j];
-                if (c1) o[1] = (half_t)Y[1][mi][ni][j];
-                if (r1) {
-                    o[Ow] = (half_t)Y[2][mi][ni][j];
-                    if (c1) o[Ow + 1] = (half_t)Y[3][mi][ni][j];
-                }
+                if (k >= prm.k) continue;
+                store_tile(prm, tc, k, Y[0][mi][ni][j], Y[1][mi][ni][j], Y[2][mi][ni][j], Y[3][mi][ni][j]);
             }
     }
 }

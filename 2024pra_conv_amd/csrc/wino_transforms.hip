@@ -13,7 +13,7 @@
 //   G = [[1,0,0],[1/2,1/2,1/2],[1/2,-1/2,1/2],[0,0,1]]   (reference preprocessing.cuh:77-81)
 // One lane produces one (k, 8-channel group) for all 16 positions, so each of its 16
 // stores is one full 16-byte lane slot of a fragment and a wave stores whole 1 KiB
-// fragments.
+// fragments (32 rows x 16 channels).
 // ---------------------------------------------------------------------------------------
 __device__ inline void filter_tile_fp32(const float g[9], float u[16]) {
     float t[4][3];
@@ -39,12 +39,12 @@ __global__ void __launch_bounds__(256) wino_filter_transform(mykernelParamType p
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int frag = gtid >> 6;
     const int lane = gtid & 63;
-    const int KB = prm.KB, CB = prm.CB;
-    if (frag >= KB * CB) return;
+    const int KB = prm.KB32, CB16 = 2 * prm.CB;
+    if (frag >= KB * CB16) return;
     const int kb = frag % KB;
-    const int cb = frag / KB;
-    const int k = kb * 16 + (lane & 15);
-    const int c0 = cb * 32 + (lane >> 4) * 8;
+    const int cb = frag / KB;  // 16-channel block
+    const int k = kb * 32 + (lane & 31);
+    const int c0 = cb * 16 + (lane >> 5) * 8;
     const int K = prm.k, C = prm.c;
 
     half8 outv[16];
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(256) wino_filter_transform(mykernelParamType p
     half_t* U = (half_t*)prm.transform_filter;
 #pragma unroll
     for (int pos = 0; pos < 16; ++pos) {
-        half8* dst = (half8*)(U + frag_offset(pos, cb, kb, CB, KB) + lane * 8);
+        half8* dst = (half8*)(U + frag_offset(pos, cb, kb, CB16, KB) + lane * 8);
         *dst = outv[pos];
     }
 }
@@ -78,8 +78,8 @@ __global__ void __launch_bounds__(256) wino_filter_transform(mykernelParamType p
 // Workgroup = (band of prm.band tile rows, 32-channel block, image). The band's input rows
 // (with halo and zero padding, ragged right/bottom edges included) are staged in LDS
 // with coalesced row loads; then each lane builds the 4x4 windows of one tile for 8
-// channels and stores 16 full fragment lane slots. A wave covers 16 consecutive tiles x
-// 32 channels, i.e. whole 1 KiB fragments per position.
+// channels and stores 16 full fragment lane slots. A wave covers 32 consecutive tiles x
+// 16 channels, i.e. whole 1 KiB fragments per position.
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) wino_input_transform(mykernelParamType prm) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
@@ -99,16 +99,21 @@ __device__ void input_transform_block(const mykernelParamType& prm, int n, int c
     const int LR = input_lds_rows(TH);
     const int LWp = input_lds_pitch(tiles_w, prm.w, prm.q);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    const int T = prm.T, NT = prm.NT, G16 = prm.G16, CB = prm.CB;
+    const int T = prm.T, NT = prm.NT, G32 = prm.G32, CB16 = 2 * prm.CB;
     const int gs = n * T + th0 * tiles_w;
     const int ge = n * T + th1 * tiles_w;
-    const int cg = lane >> 4;
-    const size_t pos_stride = (size_t)CB * G16 * kFragHalves;
-    half_t* V = (half_t*)prm.transform_pin + (size_t)cb * G16 * kFragHalves + (size_t)(cg * 16 + (lane & 15)) * 8;
-    for (int grp = (gs >> 4) + wave; grp <= ((ge - 1) >> 4); grp += nwaves) {
-        const int g = grp * 16 + (lane & 15);
+    const int kg = lane >> 5;  // 8-channel group within the 16-channel block
+    const size_t pos_stride = (size_t)CB16 * G32 * kFragHalves;
+    half_t* V = (half_t*)prm.transform_pin + (size_t)lane * 8;
+    // work unit = (32-column block, 16-channel half of the staged 32 channels): one wave
+    // writes one whole 1 KiB fragment per position
+    const int grp0 = gs >> 5;
+    const int nunits = 2 * (((ge - 1) >> 5) - grp0 + 1);
+    for (int u = wave; u < nunits; u += nwaves) {
+        const int grp = grp0 + (u >> 1), h = u & 1;
+        const int g = grp * 32 + (lane & 31);
         const bool valid = g >= gs && g < ge;
-        const bool tail = (ge == NT) && g >= NT && g < G16 * 16;
+        const bool tail = (ge == NT) && g >= NT && g < G32 * 32;
         if (!valid && !tail) continue;
         half8 outv[16];
         if (valid) {
@@ -118,7 +123,7 @@ __device__ void input_transform_block(const mykernelParamType& prm, int n, int c
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 float d[16], v[16];
-                load_window(lds, LR, LWp, 2 * (th - th0), 2 * tw + 8, cg * 8 + e, d);
+                load_window(lds, LR, LWp, 2 * (th - th0), 2 * tw + 8, h * 16 + kg * 8 + e, d);
                 input_tile_fp32(d, v);
 #pragma unroll
                 for (int pos = 0; pos < 16; ++pos) outv[pos][e] = (half_t)v[pos];
@@ -129,7 +134,7 @@ __device__ void input_transform_block(const mykernelParamType& prm, int n, int c
 #pragma unroll
                 for (int e = 0; e < 8; ++e) outv[pos][e] = (half_t)0.0f;
         }
-        half_t* dst = V + (size_t)grp * kFragHalves;
+        half_t* dst = V + ((size_t)(2 * cb + h) * G32 + grp) * kFragHalves;
 #pragma unroll
         for (int pos = 0; pos < 16; ++pos) *(half8*)(dst + pos * pos_stride) = outv[pos];
     }

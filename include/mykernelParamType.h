@@ -26,9 +26,9 @@ typedef struct mykernelParamType {
     int tiles_w; /* ceil(Ow / 2) */
     int T;       /* tiles_h * tiles_w, 2x2 output tiles per image */
     int NT;      /* n * T, GEMM columns (tile index g = image * T + tile) */
-    int G16;     /* ceil(NT / 16), 16-column blocks of V */
-    int CB;      /* ceil(c / 32), 32-channel blocks (MFMA reduction chunks) */
-    int KB;      /* ceil(k / 16), 16-row blocks of U */
+    int G32;     /* ceil(NT / 32), 32-column blocks of V */
+    int CB;      /* ceil(c / 32); channels are padded to 32 * CB (2 * CB 16-channel blocks) */
+    int KB32;    /* ceil(k / 32), 32-row blocks of U */
     int band;    /* tile rows per input-transform workgroup */
     int cfg;     /* GEMM tile configuration index */
     int flags;   /* bit 0: input rows are 16-byte aligned (w % 8 == 0 and aligned base) */
@@ -36,16 +36,19 @@ typedef struct mykernelParamType {
 
 /*
  * Packed layouts (element = one FP16 value). Both are the operand layouts of
- * v_mfma_f32_16x16x32_f16, so a GEMM workgroup copies 1 KiB fragments verbatim into LDS
- * and each lane then reads its 16 bytes.
+ * v_mfma_f32_32x32x16_f16 (32 rows x 16 channels per 1 KiB fragment), so a GEMM workgroup
+ * copies fragments verbatim into LDS and each lane then reads its 16 contiguous bytes.
  *
- *   U (transform_filter): U[pos][cb][kb][lane][e], lane = (c % 32 / 8) * 16 + k % 16,
- *                         e = c % 8; pos = 4*a + b is the Winograd position (row a, col b).
+ *   U (transform_filter): U[pos][c16][kb][lane][e], c16 = c / 16, kb = k / 32,
+ *                         lane = (c % 16 / 8) * 32 + k % 32, e = c % 8;
+ *                         pos = 4*a + b is the Winograd position (row a, col b).
  *                         Rows k >= K and channels c >= C are zero.
- *   V (transform_pin):    V[pos][cb][gb][lane][e], lane = (c % 32 / 8) * 16 + g % 16,
- *                         e = c % 8, g = image * T + tile_row * tiles_w + tile_col.
+ *   V (transform_pin):    V[pos][c16][gb][lane][e], gb = g / 32,
+ *                         lane = (c % 16 / 8) * 32 + g % 32, e = c % 8,
+ *                         g = image * T + tile_row * tiles_w + tile_col.
  *                         Channels c >= C and columns g >= NT are zero.
- * Sizes: U = 16 * CB * KB * 512 halves, V = 16 * CB * G16 * 512 halves.
+ * Sizes (2 * CB 16-channel blocks): U = 16 * 2CB * KB32 * 512 halves,
+ * V = 16 * 2CB * G32 * 512 halves.
  */
 
 #endif /* MYKERNELPARAMTYPE_MI355X_H_ */

@@ -67,24 +67,24 @@ class Geometry:
         return self.n * self.T
 
     @property
-    def G16(self) -> int:
-        return (self.NT + 15) // 16
+    def G32(self) -> int:
+        return (self.NT + 31) // 32
 
     @property
     def CB(self) -> int:
         return (self.c + 31) // 32
 
     @property
-    def KB(self) -> int:
-        return (self.k + 15) // 16
+    def KB32(self) -> int:
+        return (self.k + 31) // 32
 
     @property
     def v_halves(self) -> int:
-        return 16 * self.CB * self.G16 * FRAG
+        return 16 * 2 * self.CB * self.G32 * FRAG
 
     @property
     def u_halves(self) -> int:
-        return 16 * self.CB * self.KB * FRAG
+        return 16 * 2 * self.CB * self.KB32 * FRAG
 
     def direct_flops(self) -> float:
         return 2.0 * self.n * self.k * self.c * 9 * self.oh * self.ow
@@ -218,44 +218,45 @@ def input_transform_fp32(x16: np.ndarray, geo: Geometry, keep_fp32: bool = False
 
 
 # ---------------------------------------------------------------------------------------
-# Packed layouts (the MFMA operand layouts the kernels use)
-#   U[pos][cb][kb][lane][e], lane = (c%32//8)*16 + k%16, e = c%8
-#   V[pos][cb][gb][lane][e], lane = (c%32//8)*16 + g%16, e = c%8
+# Packed layouts (the v_mfma_f32_32x32x16_f16 operand layouts the kernels use)
+#   U[pos][c16][kb][lane][e], kb = k//32, lane = (c%16//8)*32 + k%32, e = c%8
+#   V[pos][c16][gb][lane][e], gb = g//32, lane = (c%16//8)*32 + g%32, e = c%8
 # ---------------------------------------------------------------------------------------
 def _pack(a: np.ndarray, rows_blocks: int) -> np.ndarray:
-    """a: (16, R, C) with R = row dim (k or g). Returns the flat packed array."""
+    """a: (16, R, C) with R = row dim (k or g). Returns the flat packed array: fragments of
+    32 rows x 16 channels, [pos][c16][rb][lane][e], lane = (c % 16 // 8) * 32 + r % 32;
+    channels padded to a multiple of 32."""
     P, R, C = a.shape
-    CB = (C + 31) // 32
-    Rp = rows_blocks * 16
-    full = np.zeros((P, Rp, CB * 32), dtype=a.dtype)
+    C16 = 2 * ((C + 31) // 32)
+    full = np.zeros((P, rows_blocks * 32, C16 * 16), dtype=a.dtype)
     full[:, :R, :C] = a
-    # (P, rb, r16, cb, cg, e) -> (P, cb, rb, cg, r16, e)
-    x = full.reshape(P, rows_blocks, 16, CB, 4, 8).transpose(0, 3, 1, 4, 2, 5)
+    # (P, rb, r32, c16, cg, e) -> (P, c16, rb, cg, r32, e)
+    x = full.reshape(P, rows_blocks, 32, C16, 2, 8).transpose(0, 3, 1, 4, 2, 5)
     return np.ascontiguousarray(x).reshape(-1)
 
 
 def _unpack(flat: np.ndarray, P: int, R: int, C: int, rows_blocks: int) -> np.ndarray:
-    CB = (C + 31) // 32
-    x = flat.reshape(P, CB, rows_blocks, 4, 16, 8).transpose(0, 2, 4, 1, 3, 5)
-    return np.ascontiguousarray(x).reshape(P, rows_blocks * 16, CB * 32)[:, :R, :C]
+    C16 = 2 * ((C + 31) // 32)
+    x = flat.reshape(P, C16, rows_blocks, 2, 32, 8).transpose(0, 2, 4, 1, 3, 5)
+    return np.ascontiguousarray(x).reshape(P, rows_blocks * 32, C16 * 16)[:, :R, :C]
 
 
 def pack_U(u: np.ndarray, geo: Geometry) -> np.ndarray:
     """u: (16, K, C) -> flat packed U (length geo.u_halves)."""
-    return _pack(u, geo.KB)
+    return _pack(u, geo.KB32)
 
 
 def pack_V(v: np.ndarray, geo: Geometry) -> np.ndarray:
     """v: (16, C, NT) -> flat packed V (length geo.v_halves)."""
-    return _pack(np.ascontiguousarray(v.transpose(0, 2, 1)), geo.G16)
+    return _pack(np.ascontiguousarray(v.transpose(0, 2, 1)), geo.G32)
 
 
 def unpack_U(flat: np.ndarray, geo: Geometry) -> np.ndarray:
-    return _unpack(flat, 16, geo.k, geo.c, geo.KB)
+    return _unpack(flat, 16, geo.k, geo.c, geo.KB32)
 
 
 def unpack_V(flat: np.ndarray, geo: Geometry) -> np.ndarray:
-    return _unpack(flat, 16, geo.NT, geo.c, geo.G16).transpose(0, 2, 1)
+    return _unpack(flat, 16, geo.NT, geo.c, geo.G32).transpose(0, 2, 1)
 
 
 # ---------------------------------------------------------------------------------------

@@ -82,7 +82,7 @@ def test_getkernelinfo_fills_slots_and_param(wc):
     assert rc == wc.OK
     assert (prm.n, prm.c, prm.h, prm.w, prm.k, prm.r, prm.s, prm.u, prm.v, prm.p, prm.q) == (32, 256, 56, 56, 256, 3, 3, 1, 1, 1, 1)
     assert (prm.Oh, prm.Ow, prm.tiles_h, prm.tiles_w, prm.T, prm.NT) == (56, 56, 28, 28, 784, 25088)
-    assert (prm.G16, prm.CB, prm.KB) == (1568, 8, 16)
+    assert (prm.G32, prm.CB, prm.KB32) == (784, 8, 8)
     assert prm.pin == 256 and prm.transform_filter == 256 and prm.flags == 1
     for i in range(1, 5):
         s = ki.slot(i)
@@ -91,9 +91,9 @@ def test_getkernelinfo_fills_slots_and_param(wc):
     cfg = wc.tile_configs()[prm.cfg]
     assert ki.slot(3)["block"][0] == cfg["threads"]
     # one-dimensional grid over (column block, k-block) pairs, remapped per XCD in the kernel
-    assert ki.slot(3)["grid"] == (-(-prm.G16 // (cfg["bt"] // 16)) * -(-prm.KB // (cfg["bk"] // 16)), 1, 1)
+    assert ki.slot(3)["grid"] == (-(-prm.G32 // (cfg["bt"] // 32)) * -(-prm.KB32 // (cfg["bk"] // 32)), 1, 1)
     assert ki.slot(2)["grid"] == (-(-28 // prm.band), 8, 32)
-    assert ki.slot(1)["grid"][0] * 256 >= prm.KB * prm.CB * 64
+    assert ki.slot(1)["grid"][0] * 256 >= prm.KB32 * 2 * prm.CB * 64
 
 
 @pytest.mark.parametrize("change,code", [
@@ -126,11 +126,11 @@ def test_workspace_sizes_and_legacy_sizing(wc):
         pb = wc.make_problem(n, c, h, h, k)
         vb, ub, mb = wc.workspace_size(pb)
         tiles = (h + 1) // 2
-        g16 = -(-(n * tiles * tiles) // 16)
-        cb, kb = -(-c // 32), -(-k // 16)
-        assert vb == 16 * cb * g16 * 512 * 2 and ub == 16 * cb * kb * 512 * 2 and mb == 0
+        g32 = -(-(n * tiles * tiles) // 32)
+        cb, kb = -(-c // 32), -(-k // 32)
+        assert vb == 16 * 2 * cb * g32 * 512 * 2 and ub == 16 * 2 * cb * kb * 512 * 2 and mb == 0
         legacy_ok = vb <= 4 * h * h * n * c * 2 and ub <= 16 * k * c * 2
-        predicted = h % 2 == 0 and c % 32 == 0 and k % 16 == 0 and (n * tiles * tiles) % 16 == 0
+        predicted = h % 2 == 0 and c % 32 == 0 and k % 32 == 0 and (n * tiles * tiles) % 32 == 0
         assert legacy_ok == predicted, (n, c, h, k)
 
 

@@ -209,12 +209,13 @@ def test_pack_roundtrip_and_known_offsets():
     pu, pv = wg.pack_U(u, geo), wg.pack_V(v, geo)
     assert pu.size == geo.u_halves and pv.size == geo.v_halves
     assert np.array_equal(wg.unpack_U(pu, geo), u) and np.array_equal(wg.unpack_V(pv, geo), v)
-    # hand-computed offsets: layout [pos][cb][blk][lane][e], lane = (c%32//8)*16 + r%16
+    # hand-computed offsets: layout [pos][c16][blk][lane][e], lane = (c%16//8)*32 + r%32
+    C16 = 2 * geo.CB
     pos, k, c = 3, 17, 38
-    off = ((pos * geo.CB + c // 32) * geo.KB + k // 16) * 512 + (((c % 32) // 8) * 16 + k % 16) * 8 + c % 8
+    off = ((pos * C16 + c // 16) * geo.KB32 + k // 32) * 512 + (((c % 16) // 8) * 32 + k % 32) * 8 + c % 8
     assert pu[off] == u[pos, k, c]
     pos, g, c = 14, 37, 9
-    off = ((pos * geo.CB + c // 32) * geo.G16 + g // 16) * 512 + (((c % 32) // 8) * 16 + g % 16) * 8 + c % 8
+    off = ((pos * C16 + c // 16) * geo.G32 + g // 32) * 512 + (((c % 16) // 8) * 32 + g % 32) * 8 + c % 8
     assert pv[off] == v[pos, c, g]
     # padding (channels >= C, rows >= K, columns >= NT) is zero
     assert np.count_nonzero(pu) == np.count_nonzero(u)
