@@ -68,6 +68,24 @@ def cpu_baseline(n, c, h, w, k) -> dict:
             "seconds": dt}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "wino_gemm_n32_c256_h56_k256.md")
+
+
+def measured_traffic(cfg: int):
+    """HBM bytes per GEMM launch on this workload, from the committed counter profile
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by its copy-kernel calibration).
+    None when the profile is absent or was taken for another tile configuration."""
+    import re
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_per_launch: cfg=(\d+) bytes=(\d+)", text, flags=re.M)
+    if m is None or int(m.group(1)) != cfg:
+        return None
+    return int(m.group(2))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +93,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cfg", type=int, default=-1, help="GEMM tile configuration (-1 = library choice)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="before warm-up, run one device-to-device copy of this many MiB (untimed): the "
+                         "known-size kernel that hardware byte counters are calibrated against")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -105,6 +126,12 @@ def main() -> int:
 
     step = (lambda: conv.launch_slots()) if args.cfg < 0 else (
         lambda: (conv.prepare_weights(), conv.input_transform(), conv.gemm(cfg)))
+    if args.copy_mib > 0:
+        src = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -141,7 +168,7 @@ def main() -> int:
     else:
         roof = {"bound": "mfma", "achieved": round(s3_tflops, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(s3_tflops / PEAK_FP16_TFLOPS, 4)}
-    roof.update({"kernel": "wino_gemm (S3, fused output transform)", "traffic": None,
+    roof.update({"kernel": "wino_gemm (S3, fused output transform)", "traffic": measured_traffic(cfg),
                  "mfma_frac": round(s3_tflops / PEAK_FP16_TFLOPS, 4),
                  "hbm_frac": round(s3_gbs / PEAK_HBM_GBS, 4),
                  "algorithmic_bytes": s3_bytes, "algorithmic_flops": gemm_flops})
