@@ -92,6 +92,14 @@ _SIGS = {
     "conv2d_tile_config": ([ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_select_tile_config": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_direct_f64": ([_P(Problem), ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_fp8_workspace_size": ([_P(Problem), _P(ctypes.c_size_t), _P(ctypes.c_size_t)], ctypes.c_int),
+    "conv2d_fp8_prepare_weights": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_fp8_input_transform": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_fp8_gemm": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_fp8_forward": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_fp8_num_tile_configs": ([], ctypes.c_int),
+    "conv2d_fp8_tile_config": ([ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int)], ctypes.c_int),
+    "conv2d_fp8_select_tile_config": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_tuned_set": ([_P(Problem), ctypes.c_int], ctypes.c_int),
     "conv2d_tuned_clear": ([], ctypes.c_int),
     "conv2d_tuned_count": ([], ctypes.c_int),
@@ -172,6 +180,24 @@ def tile_configs():
     for i in range(L.conv2d_num_tile_configs()):
         bk, bt, th = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(L.conv2d_tile_config(i, ctypes.byref(bk), ctypes.byref(bt), ctypes.byref(th)), "conv2d_tile_config")
+        out.append({"cfg": i, "bk": bk.value, "bt": bt.value, "threads": th.value})
+    return out
+
+
+def fp8_workspace_size(pb: Problem):
+    L = load_library()
+    a, b = ctypes.c_size_t(), ctypes.c_size_t()
+    check(L.conv2d_fp8_workspace_size(ctypes.byref(pb), ctypes.byref(a), ctypes.byref(b)), "conv2d_fp8_workspace_size")
+    return a.value, b.value
+
+
+def fp8_tile_configs():
+    L = load_library()
+    out = []
+    for i in range(L.conv2d_fp8_num_tile_configs()):
+        bk, bt, th = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(L.conv2d_fp8_tile_config(i, ctypes.byref(bk), ctypes.byref(bt), ctypes.byref(th)),
+              "conv2d_fp8_tile_config")
         out.append({"cfg": i, "bk": bk.value, "bt": bt.value, "threads": th.value})
     return out
 

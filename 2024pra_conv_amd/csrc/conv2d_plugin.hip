@@ -13,6 +13,7 @@
 #include "conv2d.h"
 #include "mykernelParamType.h"
 #include "wino_common.h"
+#include "wino_fp8.h"
 #include "wino_gemm.h"
 #include "plugin_internal.h"
 
@@ -331,6 +332,120 @@ int conv2d_direct_f64(const problem_t* pb, double* out_f64, void* stream) {
     hipLaunchKernelGGL(conv_direct_f64_kernel, dim3((unsigned)blocks), dim3(256), 0,
                        (hipStream_t)stream, a, out_f64);
     return hipGetLastError() == hipSuccess ? CONV2D_OK : CONV2D_ERR_HIP;
+}
+
+// ---- FP8 compensated form ----------------------------------------------------------------
+
+static int fp8_heuristic_config(const problem_t* pb) {
+    const int kp = ((int)pb->k + 31) / 32 * 32;
+    if (kp >= 128) return 0;
+    if (kp >= 64) return 1;
+    return 3;
+}
+
+static int fp8_prepare(const problem_t* pb, int cfg, Geometry* g, mykernelParamType* a, int* out_cfg) {
+    int rc = require_winograd(pb, g);
+    if (rc) return rc;
+    if (cfg < 0) cfg = fp8_heuristic_config(pb);
+    if (cfg >= kNumFp8Configs) return CONV2D_ERR_INVALID;
+    fill_param(pb, *g, cfg, a);
+    if (out_cfg) *out_cfg = cfg;
+    return CONV2D_OK;
+}
+
+static Launch fp8_slot_filter(const problem_t* pb, const Geometry& g) {
+    Launch L;
+    L.kernel = (const void*)&wino_filter_fp8;
+    const long long threads = 16LL * 2 * fp8_cq((int)pb->c) * g.KB32 * 32;
+    L.grid = dim3((unsigned)((threads + 255) / 256), 1, 1);
+    L.block = dim3(256, 1, 1);
+    L.lds = 0;
+    return L;
+}
+
+static Launch fp8_slot_input(const problem_t* pb, const Geometry& g) {
+    Launch L = slot_input(pb, g);
+    L.kernel = (const void*)&wino_input_transform_fp8;
+    return L;
+}
+
+static Launch fp8_slot_gemm(const Geometry& g, int cfg) {
+    const GemmConfig& c = kFp8Configs[cfg];
+    Launch L;
+    L.kernel = c.kernel;
+    const int vf = c.bt / 32, uf = c.bk / 32;
+    const unsigned ng = (unsigned)((g.G32 + vf - 1) / vf), nk = (unsigned)((g.KB32 + uf - 1) / uf);
+    L.grid = dim3(ng * nk, 1, 1);
+    L.block = dim3((unsigned)c.threads, 1, 1);
+    L.lds = 0;
+    return L;
+}
+
+int conv2d_fp8_workspace_size(const problem_t* pb, size_t* pin_bytes, size_t* filter_bytes) {
+    if (!pb) return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = require_winograd(pb, &g);
+    if (rc) return rc;
+    if (pin_bytes) *pin_bytes = fp8_operand_bytes((int)pb->c, g.G32);
+    if (filter_bytes) *filter_bytes = fp8_u8_offset(g.CB, g.KB32) + fp8_operand_bytes((int)pb->c, g.KB32);
+    return CONV2D_OK;
+}
+
+int conv2d_fp8_prepare_weights(const problem_t* pb, void* stream) {
+    int rc = check_ptrs(pb, false, true, false, false, true);
+    if (rc) return rc;
+    Geometry g;
+    mykernelParamType a;
+    if ((rc = fp8_prepare(pb, 0, &g, &a, nullptr))) return rc;
+    if ((rc = launch(slot_filter(g), &a, (hipStream_t)stream))) return rc;
+    return launch(fp8_slot_filter(pb, g), &a, (hipStream_t)stream);
+}
+
+int conv2d_fp8_input_transform(const problem_t* pb, void* stream) {
+    int rc = check_ptrs(pb, true, false, false, true, false);
+    if (rc) return rc;
+    Geometry g;
+    mykernelParamType a;
+    if ((rc = fp8_prepare(pb, 0, &g, &a, nullptr))) return rc;
+    return launch(fp8_slot_input(pb, g), &a, (hipStream_t)stream);
+}
+
+int conv2d_fp8_gemm(const problem_t* pb, int cfg, void* stream) {
+    int rc = check_ptrs(pb, false, false, true, true, true);
+    if (rc) return rc;
+    Geometry g;
+    mykernelParamType a;
+    if ((rc = fp8_prepare(pb, cfg, &g, &a, &cfg))) return rc;
+    return launch(fp8_slot_gemm(g, cfg), &a, (hipStream_t)stream);
+}
+
+int conv2d_fp8_forward(const problem_t* pb, int cfg, void* stream) {
+    int rc = check_ptrs(pb, true, false, true, true, true);
+    if (rc) return rc;
+    Geometry g;
+    mykernelParamType a;
+    if ((rc = fp8_prepare(pb, cfg, &g, &a, &cfg))) return rc;
+    if ((rc = launch(fp8_slot_input(pb, g), &a, (hipStream_t)stream))) return rc;
+    return launch(fp8_slot_gemm(g, cfg), &a, (hipStream_t)stream);
+}
+
+int conv2d_fp8_num_tile_configs(void) { return kNumFp8Configs; }
+
+int conv2d_fp8_tile_config(int cfg, int* bk, int* bt, int* threads) {
+    if (cfg < 0 || cfg >= kNumFp8Configs) return CONV2D_ERR_INVALID;
+    if (bk) *bk = kFp8Configs[cfg].bk;
+    if (bt) *bt = kFp8Configs[cfg].bt;
+    if (threads) *threads = kFp8Configs[cfg].threads;
+    return CONV2D_OK;
+}
+
+int conv2d_fp8_select_tile_config(const problem_t* pb, int* cfg) {
+    if (!pb || !cfg) return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = require_winograd(pb, &g);
+    if (rc) return rc;
+    *cfg = fp8_heuristic_config(pb);
+    return CONV2D_OK;
 }
 
 const char* conv2d_version(void) { return "winoconv-mi355x 0.1 (gfx950)"; }

@@ -1,166 +1,146 @@
-// FP8 (OCP e4m3fn) compensated form of the GEMM stage.
+// FP8 (OCP e4m3fn) compensated form of the transform and GEMM stages.
 //
-// No counterpart in the reference (it extends the method). Operands are split into a high
-// and a low FP8 part, each with an E8M0 scale per (Winograd position, row or column,
-// 32-channel block) -- the microscaling granularity v_mfma_scale_f32_16x16x128_f8f6f4
-// applies in hardware -- and the product is recovered in three FP8 MFMA passes accumulated
-// in FP32:  U.V ~ U_hi.V_hi + U_hi.V_lo + U_lo.V_hi  (the dropped U_lo.V_lo term is ~2^-8 of
-// the block scale squared). Two passes (one-sided compensation) measurably miss a 1e-2
-// error bound; three meet it. The transforms stay FP16/FP32: U is quantised from the FP16
-// transformed filter, V from the FP32 transform before any FP16 rounding.
+// No counterpart in the reference (it extends the method). Each operand value v is split
+// into a high and a low e4m3 part, v ~= hi * 2^E_hi + lo * 2^E_lo, with one E8M0 exponent
+// pair per (Winograd position, row, 32-channel block) -- the microscaling block that
+// v_mfma_scale_f32_32x32x64_f8f6f4 applies in hardware. E_hi is the smallest exponent with
+// max|v| <= 448 * 2^E_hi; E_lo = E_hi - 4. The product is recovered in three FP8 MFMA passes
+// accumulated in FP32:  U.V ~ U_hi.V_hi + U_hi.V_lo + U_lo.V_hi  (the dropped U_lo.V_lo term is
+// below 2^-8 of the block scales' product). With the split of V alone (two passes) the
+// filter's own e4m3 rounding leaves a few percent of error; the third pass removes it.
 //
-// Packed layouts (bytes): fragment (pos, cq, blk) = 64 lanes x 32 bytes, lane =
-// (c % 128 / 32) * 16 + row % 16, byte = c % 32 (the 16x16x128 operand layout: a lane holds
-// 32 consecutive channels of one row); scale block (pos, cq, blk) = 64 E8M0 bytes, one per
-// lane. Workspace = hi codes | lo codes | hi scales | lo scales.
-#include "wino_common.h"
+// The transforms keep FP32 arithmetic: U is quantised from the FP16 transformed filter that
+// S1 writes, V straight from the FP32 input transform (no FP16 rounding of V). Every step of
+// the quantisation is exact in FP32 (power-of-two scalings; the hi residual is v minus v
+// rounded to 4 significant bits), then rounded once to e4m3 (nearest, ties to even).
 #include "wino_fp8.h"
+#include "wino_tile_ops.h"
 
+typedef int int4v __attribute__((ext_vector_type(4)));
 typedef int int8v __attribute__((ext_vector_type(8)));
 
 namespace {
 
-constexpr int kAT8[2][4] = {{1, 1, 1, 0}, {0, 1, -1, -1}};
-
-// Smallest E with m <= 448 * 2^E (m >= 0), clamped to [-127, 127]; E = -127 for m = 0.
+// Smallest E with m <= 448 * 2^E (m >= 0), clamped to [-127, 127]; -127 for m = 0.
+// 448 = 0.875 * 2^9, and m = f * 2^x with f in [0.5, 1).
 __device__ inline int block_exponent(float m) {
     if (!(m > 0.f)) return -127;
     int x;
-    const float f = frexpf(m, &x);  // m = f * 2^x, f in [0.5, 1)
-    int E = x - 9 + (f > 0.875f ? 1 : 0);
+    const float f = frexpf(m, &x);
+    const int E = x - 9 + (f > 0.875f ? 1 : 0);
     return E < -127 ? -127 : (E > 127 ? 127 : E);
 }
 
-// Packs 4 floats (already scaled into [-448, 448]) into 4 e4m3 bytes (RNE).
+// Four values (already scaled into [-448, 448]) -> four e4m3 codes, nearest-even.
 __device__ inline uint32_t pack4_e4m3(float a, float b, float c, float d) {
-    uint32_t w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    const uint32_t w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0u, false);
     return __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
 }
 
-// Value of an e4m3fn byte as float.
-__device__ inline float e4m3_value(uint32_t code) {
-    const uint32_t s = code & 0x80u, e = (code >> 3) & 0xFu, m = code & 7u;
-    float v = e ? ldexpf((float)(8u + m), (int)e - 10) : ldexpf((float)m, -9);
-    return s ? -v : v;
-}
-
-__device__ inline float scale_of(int E) { return ldexpf(1.0f, E); }
-
-// Quantises n <= 32 values held by a lane group into hi/lo codes given the group's block
-// exponents. vals: this lane's count values; out bytes via pack4.
+// Splits n values (multiple of 4) into hi / lo code words given the block exponents.
 template <int N>
-__device__ inline void quantize_lane(const float (&v)[N], int Ehi, int Elo, uint32_t (&hi)[N / 4],
-                                     uint32_t (&lo)[N / 4]) {
-    const float ih = scale_of(-Ehi), il = scale_of(-Elo), sh = scale_of(Ehi);
+__device__ inline void split_hi_lo(const float (&v)[N], int Ehi, int Elo, uint32_t (&hi)[N / 4],
+                                   uint32_t (&lo)[N / 4]) {
 #pragma unroll
     for (int q = 0; q < N / 4; ++q) {
-        float h[4], r[4];
+        hi[q] = pack4_e4m3(ldexpf(v[4 * q], -Ehi), ldexpf(v[4 * q + 1], -Ehi), ldexpf(v[4 * q + 2], -Ehi),
+                           ldexpf(v[4 * q + 3], -Ehi));
+        const float h[4] = {__builtin_amdgcn_cvt_f32_fp8((int)hi[q], 0), __builtin_amdgcn_cvt_f32_fp8((int)hi[q], 1),
+                            __builtin_amdgcn_cvt_f32_fp8((int)hi[q], 2), __builtin_amdgcn_cvt_f32_fp8((int)hi[q], 3)};
+        float r[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) h[e] = v[q * 4 + e] * ih;
-        hi[q] = pack4_e4m3(h[0], h[1], h[2], h[3]);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) r[e] = (v[q * 4 + e] - e4m3_value((hi[q] >> (8 * e)) & 0xFFu) * sh) * il;
+        for (int e = 0; e < 4; ++e) r[e] = ldexpf(v[4 * q + e] - ldexpf(h[e], Ehi), -Elo);
         lo[q] = pack4_e4m3(r[0], r[1], r[2], r[3]);
     }
 }
 
-template <int N>
-__device__ inline float max_abs(const float (&v)[N]) {
-    float m = 0.f;
-#pragma unroll
-    for (int i = 0; i < N; ++i) m = fmaxf(m, fabsf(v[i]));
-    return m;
-}
+__device__ inline int lo_exponent(int Ehi) { return max(Ehi - kFp8LoOffset, -127); }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
-// U (FP16 packed) -> U8 hi/lo + scales. One lane per (pos, cq, kb, lane): 32 channels of
-// one row k, gathered from the four FP16 lane slots that hold them.
+// U (FP16, packed by S1) -> FP8 records. One thread per (position, row, 32-channel block):
+// it gathers the block's 32 channels from the four FP16 lane slots that hold them, then
+// writes two 16-byte code runs per part and one scale pair. Consecutive threads take
+// consecutive rows, so the 16-byte stores of a wave are contiguous.
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) wino_filter_fp8(mykernelParamType prm, uint8_t* __restrict__ ws) {
-    const int KB = prm.KB, CB = prm.CB, CQ = (prm.c + 127) / 128;
-    const size_t nfrag = (size_t)16 * CQ * KB;
-    const size_t gt = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gt >= nfrag * 64) return;
-    const int lane = (int)(gt & 63);
-    const size_t frag = gt >> 6;
-    const int kb = (int)(frag % KB);
-    const int cq = (int)((frag / KB) % CQ);
-    const int pos = (int)(frag / ((size_t)KB * CQ));
-    const int cb32 = cq * 4 + (lane >> 4);
-    const int r = lane & 15;
+__global__ void __launch_bounds__(256) wino_filter_fp8(mykernelParamType prm) {
+    const int KB = prm.KB32, CB = prm.CB, CQ = fp8_cq(prm.c);
+    const int rows = KB * 32;
+    const long long total = 16LL * 2 * CQ * rows;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int r = (int)(t % rows);
+    const int cb = (int)((t / rows) % (2 * CQ));  // 32-channel block
+    const int pos = (int)(t / ((long long)rows * 2 * CQ));
     float v[32];
-    if (cb32 < CB) {
-        const half_t* U = (const half_t*)prm.transform_filter + frag_offset(pos, cb32, kb, CB, KB);
+    if (cb < CB) {
+        const half_t* U = (const half_t*)prm.transform_filter;
 #pragma unroll
-        for (int cg = 0; cg < 4; ++cg) {
-            const half8 h = *(const half8*)(U + (cg * 16 + r) * 8);
+        for (int hc = 0; hc < 2; ++hc)      // 16-channel block 2cb + hc
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[cg * 8 + e] = (float)h[e];
-        }
+            for (int cg = 0; cg < 2; ++cg) {  // 8-channel group (lane half) inside it
+                const half8 h =
+                    *(const half8*)(U + frag_offset(pos, 2 * cb + hc, r >> 5, 2 * CB, KB) + (cg * 32 + (r & 31)) * 8);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[hc * 16 + cg * 8 + e] = (float)h[e];
+            }
     } else {
 #pragma unroll
         for (int i = 0; i < 32; ++i) v[i] = 0.f;
     }
-    const int Ehi = block_exponent(max_abs(v));
+    float m = 0.f;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) m = fmaxf(m, fabsf(v[i]));
+    const int Ehi = block_exponent(m), Elo = lo_exponent(Ehi);
     uint32_t hi[8], lo[8];
-    // residual maximum needs the hi codes first
-    {
-        const float ih = scale_of(-Ehi), sh = scale_of(Ehi);
-        float rmax = 0.f;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            hi[q] = pack4_e4m3(v[q * 4] * ih, v[q * 4 + 1] * ih, v[q * 4 + 2] * ih, v[q * 4 + 3] * ih);
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                rmax = fmaxf(rmax, fabsf(v[q * 4 + e] - e4m3_value((hi[q] >> (8 * e)) & 0xFFu) * sh));
-        }
-        const int Elo = block_exponent(rmax);
-        quantize_lane<32>(v, Ehi, Elo, hi, lo);
-        const size_t codes = nfrag * 2048;
-        uint8_t* dst = ws + frag * 2048 + (size_t)lane * 32;
-        *(u32x4*)dst = u32x4{hi[0], hi[1], hi[2], hi[3]};
-        *(u32x4*)(dst + 16) = u32x4{hi[4], hi[5], hi[6], hi[7]};
-        *(u32x4*)(dst + codes) = u32x4{lo[0], lo[1], lo[2], lo[3]};
-        *(u32x4*)(dst + codes + 16) = u32x4{lo[4], lo[5], lo[6], lo[7]};
-        uint8_t* sc = ws + 2 * codes + frag * 64 + lane;
-        sc[0] = (uint8_t)(Ehi + 127);
-        sc[nfrag * 64] = (uint8_t)(Elo + 127);
-    }
+    split_hi_lo<32>(v, Ehi, Elo, hi, lo);
+    uint8_t* rec = (uint8_t*)prm.transform_filter + fp8_u8_offset(CB, KB) +
+                   ((size_t)(pos * CQ + (cb >> 1)) * KB + (r >> 5)) * kFp8RecBytes;
+    uint8_t* part = rec + (cb & 1) * 1024;
+    const int l0 = r & 31, l1 = 32 + (r & 31);  // lanes holding channels 0-15 / 16-31
+    *(u32x4*)(part + l0 * 16) = u32x4{hi[0], hi[1], hi[2], hi[3]};
+    *(u32x4*)(part + l1 * 16) = u32x4{hi[4], hi[5], hi[6], hi[7]};
+    *(u32x4*)(part + 2048 + l0 * 16) = u32x4{lo[0], lo[1], lo[2], lo[3]};
+    *(u32x4*)(part + 2048 + l1 * 16) = u32x4{lo[4], lo[5], lo[6], lo[7]};
+    *(uint16_t*)(rec + 4096 + ((cb & 1) * 32 + l0) * 2) = (uint16_t)((Ehi + 127) | ((Elo + 127) << 8));
 }
 
 // ---------------------------------------------------------------------------------------
-// Input transform straight to V8 hi/lo + scales. Same staging as the FP16 transform; a lane
-// computes 8 channels x 16 positions of one tile in FP32, the four lanes holding one tile's
-// 32-channel block agree on the block exponents by cross-lane max, then each writes its
-// 8 bytes of the 32-byte lane slot.
+// Input transform straight to FP8 records. Same workgroup decomposition and LDS staging as
+// the FP16 transform (band of tile rows x 32-channel block x image). Work unit = 16
+// consecutive tiles x the 32 staged channels; lane = (tile lane % 16, channel quarter
+// lane / 16) computes 8 channels x 16 positions in FP32; the four quarters of a tile agree
+// on the block exponents by a cross-lane maximum, then each stores its 8 code bytes per
+// part. A wave's stores cover two contiguous 256-byte runs per part and position.
 // ---------------------------------------------------------------------------------------
-__device__ void input_transform_fp32_tile(const half_t* lds, int LR, int LWp, int lr0, int lc0, int ch,
-                                          float v[16]);
-
-__global__ void __launch_bounds__(256) wino_input_transform_fp8(mykernelParamType prm, uint8_t* __restrict__ ws) {
+__global__ void __launch_bounds__(256) wino_input_transform_fp8(mykernelParamType prm) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
     const int n = blockIdx.z, cb = blockIdx.y, bi = blockIdx.x;
-    stage_input_band(prm, n, cb, bi, lds);  // zero + load, ends with a barrier
+    stage_input_band(prm, n, cb, bi, lds);
     const int TH = prm.band;
     const int th0 = bi * TH;
     const int tiles_w = prm.tiles_w;
     const int th1 = min(th0 + TH, prm.tiles_h);
     const int LR = input_lds_rows(TH);
     const int LWp = input_lds_pitch(tiles_w, prm.w, prm.q);
-    const int T = prm.T, NT = prm.NT, G16 = prm.G16, CQ = (prm.c + 127) / 128;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    const int cg = lane >> 4;
+    const int T = prm.T, NT = prm.NT, G32 = prm.G32, CB = prm.CB, CQ = fp8_cq(prm.c);
     const int gs = n * T + th0 * tiles_w;
     const int ge = n * T + th1 * tiles_w;
-    const size_t nfrag = (size_t)16 * CQ * G16;
-    const size_t codes = nfrag * 2048;
-    const int cq = cb >> 2, slot_lane = (cb & 3) * 16;
-    for (int grp = (gs >> 4) + wave; grp <= ((ge - 1) >> 4); grp += nwaves) {
-        const int g = grp * 16 + (lane & 15);
+    // the last image's last band also writes the zero columns up to the 32-column padding
+    const int gend = ge == NT ? G32 * 32 : ge;
+    const int q = lane >> 4;  // channel quarter: staged channels 8q .. 8q+7
+    const int part = cb & 1;  // 32-channel block inside the 64-channel record
+    // an odd channel-block count leaves the second half of the last record to this block
+    const bool pad_half = (cb == CB - 1) && (CB & 1);
+    uint8_t* V8 = (uint8_t*)prm.transform_pin;
+    const int grp0 = gs >> 4;
+    const int nunits = ((gend - 1) >> 4) - grp0 + 1;
+    for (int u = wave; u < nunits; u += nwaves) {
+        const int g = (grp0 + u) * 16 + (lane & 15);
         const bool valid = g >= gs && g < ge;
-        const bool tail = (ge == NT) && g >= NT && g < G16 * 16;
+        const bool store = g >= gs && g < gend;
         float v[16][8];
         if (valid) {
             const int t = g - n * T;
@@ -168,10 +148,11 @@ __global__ void __launch_bounds__(256) wino_input_transform_fp8(mykernelParamTyp
             const int tw = t - th * tiles_w;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                float vv[16];
-                input_transform_fp32_tile(lds, LR, LWp, 2 * (th - th0), 2 * tw + 8, cg * 8 + e, vv);
+                float d[16], w[16];
+                load_window(lds, LR, LWp, 2 * (th - th0), 2 * tw + 8, q * 8 + e, d);
+                input_tile_fp32(d, w);
 #pragma unroll
-                for (int pos = 0; pos < 16; ++pos) v[pos][e] = vv[pos];
+                for (int pos = 0; pos < 16; ++pos) v[pos][e] = w[pos];
             }
         } else {
 #pragma unroll
@@ -179,222 +160,226 @@ __global__ void __launch_bounds__(256) wino_input_transform_fp8(mykernelParamTyp
 #pragma unroll
                 for (int e = 0; e < 8; ++e) v[pos][e] = 0.f;
         }
-        const bool store = valid || tail;
+        const int lcode = (q >> 1) * 32 + (g & 31);  // operand lane holding these channels
+        const int lscale = part * 32 + (g & 31);     // lane holding this block's scales
 #pragma unroll
         for (int pos = 0; pos < 16; ++pos) {
-            float m = max_abs(v[pos]);
+            float m = 0.f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(v[pos][e]));
             m = fmaxf(m, __shfl_xor(m, 16));
             m = fmaxf(m, __shfl_xor(m, 32));
-            const int Ehi = block_exponent(m);
-            const float ih = scale_of(-Ehi), sh = scale_of(Ehi);
+            const int Ehi = block_exponent(m), Elo = lo_exponent(Ehi);
             uint32_t hi[2], lo[2];
-            float rm = 0.f;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                hi[q] = pack4_e4m3(v[pos][q * 4] * ih, v[pos][q * 4 + 1] * ih, v[pos][q * 4 + 2] * ih,
-                                   v[pos][q * 4 + 3] * ih);
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    rm = fmaxf(rm, fabsf(v[pos][q * 4 + e] - e4m3_value((hi[q] >> (8 * e)) & 0xFFu) * sh));
-            }
-            rm = fmaxf(rm, __shfl_xor(rm, 16));
-            rm = fmaxf(rm, __shfl_xor(rm, 32));
-            const int Elo = block_exponent(rm);
-            const float il = scale_of(-Elo);
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                float r[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    r[e] = (v[pos][q * 4 + e] - e4m3_value((hi[q] >> (8 * e)) & 0xFFu) * sh) * il;
-                lo[q] = pack4_e4m3(r[0], r[1], r[2], r[3]);
-            }
+            split_hi_lo<8>(v[pos], Ehi, Elo, hi, lo);
             if (store) {
-                const size_t frag = ((size_t)pos * CQ + cq) * G16 + grp;
-                const int L = slot_lane + (g & 15);
-                uint8_t* dst = ws + frag * 2048 + (size_t)L * 32 + cg * 8;
+                uint8_t* rec = V8 + ((size_t)(pos * CQ + (cb >> 1)) * G32 + (g >> 5)) * kFp8RecBytes;
+                uint8_t* dst = rec + part * 1024 + lcode * 16 + (q & 1) * 8;
                 *(uint2*)dst = make_uint2(hi[0], hi[1]);
-                *(uint2*)(dst + codes) = make_uint2(lo[0], lo[1]);
-                if (cg == 0) {
-                    uint8_t* sc = ws + 2 * codes + frag * 64 + L;
-                    sc[0] = (uint8_t)(Ehi + 127);
-                    sc[nfrag * 64] = (uint8_t)(Elo + 127);
+                *(uint2*)(dst + 2048) = make_uint2(lo[0], lo[1]);
+                if (q == 0) *(uint16_t*)(rec + 4096 + lscale * 2) = (uint16_t)((Ehi + 127) | ((Elo + 127) << 8));
+                if (pad_half) {
+                    uint8_t* z = rec + 1024 + lcode * 16 + (q & 1) * 8;
+                    *(uint2*)z = make_uint2(0u, 0u);
+                    *(uint2*)(z + 2048) = make_uint2(0u, 0u);
+                    if (q == 0) *(uint16_t*)(rec + 4096 + (32 + (g & 31)) * 2) = (uint16_t)0;
                 }
             }
         }
     }
 }
 
-// Zero fill of the V8 fragments of 32-channel blocks that lie entirely beyond C in the last
-// 128-channel block (codes 0, scale bytes 0 = 2^-127): grid.y covers them too, so every
-// byte of the workspace is written.
+// ---------------------------------------------------------------------------------------
+// Three-pass FP8 GEMM with the fused output transform. Same structure as the FP16 GEMM
+// (workgroup tile BK x BT, waves WK x WT with 32x32 sub-tiles, branch-free copies with a
+// register ring D steps deep, two LDS stages, register double-buffered operands); one
+// pipeline step = one 64-channel block of one position = UF + VF records.
+// ---------------------------------------------------------------------------------------
+namespace {
 
-// ---------------------------------------------------------------------------------------
-// Three-pass FP8 GEMM with the fused output transform. Workgroup tile BK x BT, waves
-// WK x WT, 16x16 sub-tiles; one pipeline step = one 128-channel block of one position.
-// LDS stage: U hi | U lo | V hi | V lo codes, then U hi | U lo | V hi | V lo scales.
-// ---------------------------------------------------------------------------------------
 template <int BK, int BT, int WK, int WT>
-__global__ void __launch_bounds__(WK * WT * 64) wino_gemm_fp8_kernel(mykernelParamType prm) {
-    constexpr int kThreads = WK * WT * 64;
-    constexpr int MI = BK / WK / 16, NI = BT / WT / 16, UF = BK / 16, VF = BT / 16;
-    constexpr int kCodeBytes = 2 * (UF + VF) * 2048;
-    constexpr int kScaleBytes = 2 * (UF + VF) * 64;
-    constexpr int kStage = kCodeBytes + kScaleBytes;
-    constexpr int kCodePieces = kCodeBytes / 16, kPieces = kStage / 16;
-    constexpr int kLoads = (kPieces + kThreads - 1) / kThreads;
-    __shared__ __attribute__((aligned(16))) uint8_t smem[2 * kStage];
+struct Fp8Shape {
+    static constexpr int kThreads = WK * WT * 64;
+    static constexpr int MI = BK / WK / 32;
+    static constexpr int NI = BT / WT / 32;
+    static constexpr int UF = BK / 32;
+    static constexpr int VF = BT / 32;
+    static constexpr int kRecPieces = kFp8RecBytes / 16;
+    static constexpr int kPieces = (UF + VF) * kRecPieces;
+    static constexpr int kLoads = (kPieces + kThreads - 1) / kThreads;
+    static constexpr int kStageBytes = kLoads * kThreads * 16;
+    static_assert(MI >= 1 && NI >= 1, "wave tile must be at least 32x32");
+    static_assert(BK % (WK * 32) == 0 && BT % (WT * 32) == 0, "tile must divide evenly");
+};
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wk = wave % WK, wt = wave / WK;
-    const int gb0 = blockIdx.x * VF, kb0 = blockIdx.y * UF;
-    const int KB = prm.KB, G16 = prm.G16, CQ = (prm.c + 127) / 128;
-    const int nsteps = 16 * CQ;
-    const uint8_t* U8 = (const uint8_t*)prm.transform_filter;
-    const uint8_t* V8 = (const uint8_t*)prm.transform_pin;
-    const size_t ufr = (size_t)16 * CQ * KB, vfr = (size_t)16 * CQ * G16;
+struct Fp8Frag {
+    int8v hi, lo;
+    int sc;  // byte 0: E_hi + 127, byte 1: E_lo + 127
+};
 
-    // copy slots: piece p of a stage -> (global base, per-step stride, validity)
-    const uint8_t* src[kLoads];
-    size_t sstride[kLoads];
-    bool sval[kLoads];
+__device__ inline Fp8Frag read_frag(const uint8_t* rec, int lane) {
+    Fp8Frag f;
+    const int4v h0 = *(const int4v*)(rec + lane * 16);
+    const int4v h1 = *(const int4v*)(rec + 1024 + lane * 16);
+    const int4v l0 = *(const int4v*)(rec + 2048 + lane * 16);
+    const int4v l1 = *(const int4v*)(rec + 3072 + lane * 16);
+    f.hi = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
+    f.lo = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
+    f.sc = *(const uint16_t*)(rec + 4096 + lane * 2);
+    return f;
+}
+
+}  // namespace
+
+template <int BK, int BT, int WK, int WT, int D>
+__device__ inline void gemm_fp8_body(const mykernelParamType& prm, int kblk, int gblk, uint8_t* smem) {
+    using S = Fp8Shape<BK, BT, WK, WT>;
+    constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF, NL = S::kLoads;
+    constexpr int RP = S::kRecPieces;
+    static_assert(16 % D == 0 && D % 2 == 0, "ring depth: even, dividing the step count (a multiple of 16)");
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wk = wave % WK;
+    const int wt = wave / WK;
+    const int KB = prm.KB32, G32 = prm.G32, CQ = fp8_cq(prm.c);
+    const int gb0 = gblk * VF;
+    const int kb0 = kblk * UF;
+    const int nsteps = 16 * CQ;  // (position, 64-channel block) pairs
+    const uint8_t* __restrict__ U8 = (const uint8_t*)prm.transform_filter + fp8_u8_offset(prm.CB, KB);
+    const uint8_t* __restrict__ V8 = (const uint8_t*)prm.transform_pin;
+
+    // copy slot i: piece pc = tid + i * threads of every step; record (pos, cq, rb) of step
+    // s = pos * CQ + cq lies at s * RB + rb records, so addresses advance by RB records per step
+    const uint8_t* src[NL];
+    size_t stride[NL];
 #pragma unroll
-    for (int i = 0; i < kLoads; ++i) {
-        const int pc = tid + i * kThreads;
-        src[i] = U8;
-        sstride[i] = 0;
-        sval[i] = false;
-        if (pc < kCodePieces) {
-            const int part = pc / (128 * (UF + VF));  // 0 hi, 1 lo
-            const int rem = pc % (128 * (UF + VF));
-            const int f = rem / 128, within = rem % 128;
-            if (f < UF) {
-                src[i] = U8 + part * ufr * 2048 + ((size_t)kb0 + f) * 2048 + within * 16;
-                sstride[i] = (size_t)KB * 2048;
-                sval[i] = kb0 + f < KB;
-            } else {
-                src[i] = V8 + part * vfr * 2048 + ((size_t)gb0 + f - UF) * 2048 + within * 16;
-                sstride[i] = (size_t)G16 * 2048;
-                sval[i] = gb0 + f - UF < G16;
-            }
-        } else if (pc < kPieces) {
-            const int sp = pc - kCodePieces;  // 4 pieces (64 B) per fragment scale block
-            const int part = sp / (4 * (UF + VF));
-            const int rem = sp % (4 * (UF + VF));
-            const int f = rem / 4, within = rem % 4;
-            if (f < UF) {
-                src[i] = U8 + 2 * ufr * 2048 + part * ufr * 64 + ((size_t)kb0 + f) * 64 + within * 16;
-                sstride[i] = (size_t)KB * 64;
-                sval[i] = kb0 + f < KB;
-            } else {
-                src[i] = V8 + 2 * vfr * 2048 + part * vfr * 64 + ((size_t)gb0 + f - UF) * 64 + within * 16;
-                sstride[i] = (size_t)G16 * 64;
-                sval[i] = gb0 + f - UF < G16;
-            }
+    for (int i = 0; i < NL; ++i) {
+        const int pc = tid + i * S::kThreads;
+        const int p = pc < S::kPieces ? pc : 0;
+        const int rec = p / RP, within = p % RP;
+        if (rec < UF) {
+            const bool ok = pc < S::kPieces && kb0 + rec < KB;
+            src[i] = U8 + (ok ? (size_t)(kb0 + rec) * kFp8RecBytes : 0) + (size_t)within * 16;
+            stride[i] = (size_t)KB * kFp8RecBytes;
+        } else {
+            const int rv = rec - UF;
+            const bool ok = pc < S::kPieces && gb0 + rv < G32;
+            src[i] = V8 + (ok ? (size_t)(gb0 + rv) * kFp8RecBytes : 0) + (size_t)within * 16;
+            stride[i] = (size_t)G32 * kFp8RecBytes;
         }
     }
-    u32x4 pre[kLoads];
-    auto gload = [&](int step) {
+
+    u32x4 ring[D][NL];
+    auto gload = [&](u32x4 (&slot)[NL], int step) {
+        const size_t s = (size_t)min(step, nsteps - 1);
 #pragma unroll
-        for (int i = 0; i < kLoads; ++i)
-            if (sval[i]) pre[i] = *(const u32x4*)(src[i] + (size_t)step * sstride[i]);
+        for (int i = 0; i < NL; ++i) slot[i] = *(const u32x4*)(src[i] + s * stride[i]);
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](const u32x4 (&slot)[NL], int buf) {
 #pragma unroll
-        for (int i = 0; i < kLoads; ++i) {
-            const int pc = tid + i * kThreads;
-            if (sval[i]) *(u32x4*)(smem + buf * kStage + pc * 16) = pre[i];
+        for (int i = 0; i < NL; ++i) *(u32x4*)(smem + buf * S::kStageBytes + (tid + i * S::kThreads) * 16) = slot[i];
+    };
+
+    floatx16 acc[MI][NI];
+    floatx16 Y[4][MI][NI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Y[q][mi][ni] = acc[mi][ni];
         }
+
+    Fp8Frag ra[2][MI], rb[2][NI];
+    auto lread = [&](Fp8Frag (&A)[MI], Fp8Frag (&B)[NI], int buf) {
+        const uint8_t* sb = smem + buf * S::kStageBytes;
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi) A[mi] = read_frag(sb + (wk * MI + mi) * kFp8RecBytes, lane);
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) B[ni] = read_frag(sb + (UF + wt * NI + ni) * kFp8RecBytes, lane);
     };
 
-    floatx4 acc[MI][NI], Y[4][MI][NI];
+    // pipeline as in the FP16 GEMM (regs(s) feed the MFMAs while regs(s+1) are read from LDS,
+    // LDS[s & 1] is refilled with step s+2, and the ring reloads step s+2+D)
+    gload(ring[0], 0);
+    lstore(ring[0], 0);
+    gload(ring[0], D);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int mi = 0; mi < MI; ++mi)
-#pragma unroll
-            for (int ni = 0; ni < NI; ++ni) Y[q][mi][ni] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-    gload(0);
-    lstore(0);
+    for (int d = 1; d < D; ++d) gload(ring[d], d);
     __syncthreads();
-#pragma unroll 1
-    for (int pos = 0; pos < 16; ++pos) {
+    lread(ra[0], rb[0], 0);
+    lstore(ring[1 % D], 1);
+    gload(ring[1 % D], 1 + D);
+    __syncthreads();
+
+    int pos = 0, cq = 0;
+    for (int s0 = 0; s0 < nsteps; s0 += D) {
 #pragma unroll
-        for (int mi = 0; mi < MI; ++mi)
+        for (int d = 0; d < D; ++d) {
+            const int s = s0 + d;
+            const int cur = d & 1;
 #pragma unroll
-            for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = floatx4{0.f, 0.f, 0.f, 0.f};
-        for (int cq = 0; cq < CQ; ++cq) {
-            const int step = pos * CQ + cq;
-            const bool has_next = step + 1 < nsteps;
-            if (has_next) gload(step + 1);
-            const uint8_t* sb = smem + (step & 1) * kStage;
-            const uint8_t* ush = sb;                       // U hi codes
-            const uint8_t* usl = sb + UF * 2048;           // U lo codes
-            const uint8_t* vsh = sb + 2 * UF * 2048;       // V hi codes
-            const uint8_t* vsl = vsh + VF * 2048;          // V lo codes
-            const uint8_t* sc = sb + kCodeBytes;           // scales: U hi | V hi | U lo | V lo
-            int8v bh[NI], bl[NI];
-            int sbh[NI], sbl[NI];
-#pragma unroll
-            for (int ni = 0; ni < NI; ++ni) {
-                const int f = wt * NI + ni;
-                bh[ni] = *(const int8v*)(vsh + f * 2048 + lane * 32);
-                bl[ni] = *(const int8v*)(vsl + f * 2048 + lane * 32);
-                sbh[ni] = sc[(UF + f) * 64 + lane];
-                sbl[ni] = sc[(UF + VF) * 64 + (UF + f) * 64 + lane];
-            }
-#pragma unroll
-            for (int mi = 0; mi < MI; ++mi) {
-                const int f = wk * MI + mi;
-                const int8v ah = *(const int8v*)(ush + f * 2048 + lane * 32);
-                const int8v al = *(const int8v*)(usl + f * 2048 + lane * 32);
-                const int sah = sc[f * 64 + lane];
-                const int sal = sc[(UF + VF) * 64 + f * 64 + lane];
+            for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
                 for (int ni = 0; ni < NI; ++ni) {
-                    floatx4 c = acc[mi][ni];
-                    c = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(ah, bh[ni], c, 0, 0, 0, sah, 0, sbh[ni]);
-                    c = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(ah, bl[ni], c, 0, 0, 0, sah, 0, sbl[ni]);
-                    c = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(al, bh[ni], c, 0, 0, 0, sal, 0, sbh[ni]);
+                    const Fp8Frag& a = ra[cur][mi];
+                    const Fp8Frag& b = rb[cur][ni];
+                    floatx16 c = acc[mi][ni];
+                    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a.hi, b.hi, c, 0, 0, 0, a.sc, 0, b.sc);
+                    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a.hi, b.lo, c, 0, 0, 0, a.sc, 1, b.sc);
+                    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a.lo, b.hi, c, 0, 0, 1, a.sc, 0, b.sc);
                     acc[mi][ni] = c;
                 }
-            }
-            if (has_next) lstore((step + 1) & 1);
+            lread(ra[cur ^ 1], rb[cur ^ 1], cur ^ 1);
+            lstore(ring[(d + 2) % D], cur);
+            gload(ring[(d + 2) % D], s + 2 + D);
             __syncthreads();
-        }
-        const int pa = pos >> 2, pb = pos & 3;
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int coef = kAT8[i][pa] * kAT8[j][pb];
-                if (coef == 0) continue;
-                const float cf = (float)coef;
+            if (++cq == CQ) {
+                fold_position(pos, acc, Y);
 #pragma unroll
                 for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-                    for (int ni = 0; ni < NI; ++ni) Y[i * 2 + j][mi][ni] += cf * acc[mi][ni];
+                    for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+                cq = 0;
+                ++pos;
             }
+        }
     }
-    store_tiles_fp8<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
+
+    store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
 
-#define FP8_CFG(BK, BT, WK, WT) \
-    { BK, BT, WK * WT * 64, 0, (const void*)&wino_gemm_fp8_kernel<BK, BT, WK, WT> }
+template <int BK, int BT, int WK, int WT>
+constexpr int kFp8WavesPerEU = (BK / WK / 32) * (BT / WT / 32) >= 4 || WK * WT <= 4 ? 1 : 2;
+
+template <int BK, int BT, int WK, int WT, int D>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kFp8WavesPerEU<BK, BT, WK, WT>)))
+wino_gemm_fp8_kernel(mykernelParamType prm) {
+    using S = Fp8Shape<BK, BT, WK, WT>;
+    __shared__ __attribute__((aligned(16))) uint8_t smem[2 * S::kStageBytes];
+    const int nk = (prm.KB32 + S::UF - 1) / S::UF;
+    const int ng = (prm.G32 + S::VF - 1) / S::VF;
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
+    gemm_fp8_body<BK, BT, WK, WT, D>(prm, kblk, gblk, smem);
+}
+
+#define FP8_CFG(BK, BT, WK, WT, D) \
+    { BK, BT, WK * WT * 64, D, (const void*)&wino_gemm_fp8_kernel<BK, BT, WK, WT, D> }
 
 const GemmConfig kFp8Configs[] = {
-    FP8_CFG(64, 64, 2, 2),    // 0: wave 32x32, 4 waves
-    FP8_CFG(128, 64, 2, 2),   // 1: wave 64x32, 4 waves
-    FP8_CFG(64, 128, 2, 4),   // 2: wave 32x32, 8 waves
+    FP8_CFG(128, 64, 2, 2, 2),   // 0: wave 64x32, 4 waves
+    FP8_CFG(64, 128, 1, 4, 2),   // 1: wave 64x32, 4 waves
+    FP8_CFG(64, 128, 2, 2, 2),   // 2: wave 32x64, 4 waves
+    FP8_CFG(32, 128, 1, 4, 2),   // 3: wave 32x32, 4 waves (small K)
+    FP8_CFG(64, 64, 1, 2, 2),    // 4: wave 64x32, 2 waves (small problems)
+    FP8_CFG(128, 64, 2, 2, 4),   // 5
+    FP8_CFG(32, 64, 1, 2, 2),    // 6: wave 32x32, 2 waves (small problems)
 };
 const int kNumFp8Configs = (int)(sizeof(kFp8Configs) / sizeof(kFp8Configs[0]));
-
-size_t fp8_u_bytes(int c, int k) {
-    const size_t CQ = (size_t)(c + 127) / 128, KB = (size_t)(k + 15) / 16;
-    return 16 * CQ * KB * (2 * 2048 + 2 * 64);
-}
-size_t fp8_v_bytes(int c, long long G16) {
-    const size_t CQ = (size_t)(c + 127) / 128;
-    return 16 * CQ * (size_t)G16 * (2 * 2048 + 2 * 64);
-}

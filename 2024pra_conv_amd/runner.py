@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from . import KernelInfo, ParamType, check, load_library, make_problem, workspace_size
+from . import KernelInfo, ParamType, check, fp8_workspace_size, load_library, make_problem, workspace_size
 
 
 def _stream_handle(stream: torch.cuda.Stream | None):
@@ -110,6 +110,70 @@ class WinogradConv:
 
     def v_halves(self) -> torch.Tensor:
         return self.V[: self.v_bytes].view(torch.float16)
+
+
+class WinogradConvFP8:
+    """The FP8 compensated form of one problem (own output and FP8 workspaces).
+
+    transform_filter holds the FP16 transformed filter (u16_bytes) followed by the FP8 filter
+    records; transform_pin holds the FP8 input records."""
+
+    def __init__(self, x: torch.Tensor, w: torch.Tensor, pad=(1, 1), out: torch.Tensor | None = None):
+        assert x.dtype == torch.float16 and w.dtype == torch.float16
+        assert x.is_cuda and w.is_cuda and x.is_contiguous() and w.is_contiguous()
+        self.L = load_library()
+        n, c, h, wd = x.shape
+        k, c2, r, s = w.shape
+        assert c == c2
+        self.x, self.w = x, w
+        self.shape = (n, c, h, wd, k, r, s, 1, 1, pad[0], pad[1])
+        self.oh = (h - r + 2 * pad[0]) + 1
+        self.ow = (wd - s + 2 * pad[1]) + 1
+        probe = make_problem(*self.shape)
+        self.v_bytes, self.u_bytes = fp8_workspace_size(probe)
+        self.u16_bytes = workspace_size(probe)[1]
+        dev = x.device
+        self.out = out if out is not None else torch.empty((n, k, self.oh, self.ow), dtype=torch.float16, device=dev)
+        self.V = torch.empty(max(self.v_bytes, 2), dtype=torch.uint8, device=dev)
+        self.U = torch.empty(max(self.u_bytes, 2), dtype=torch.uint8, device=dev)
+        self.pb = make_problem(*self.shape, ptrs={
+            "in_": x.data_ptr(), "weight": w.data_ptr(), "out": self.out.data_ptr(),
+            "transform_pin": self.V.data_ptr(), "transform_filter": self.U.data_ptr(), "mid_result": None})
+
+    def prepare_weights(self, stream=None) -> None:
+        check(self.L.conv2d_fp8_prepare_weights(ctypes.byref(self.pb), _stream_handle(stream)),
+              "conv2d_fp8_prepare_weights")
+
+    def input_transform(self, stream=None) -> None:
+        check(self.L.conv2d_fp8_input_transform(ctypes.byref(self.pb), _stream_handle(stream)),
+              "conv2d_fp8_input_transform")
+
+    def gemm(self, cfg: int = -1, stream=None) -> None:
+        check(self.L.conv2d_fp8_gemm(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_fp8_gemm")
+
+    def forward(self, cfg: int = -1, stream=None) -> None:
+        check(self.L.conv2d_fp8_forward(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_fp8_forward")
+
+    def selected_config(self) -> int:
+        c = ctypes.c_int()
+        check(self.L.conv2d_fp8_select_tile_config(ctypes.byref(self.pb), ctypes.byref(c)),
+              "conv2d_fp8_select_tile_config")
+        return c.value
+
+    def direct_f64(self, stream=None) -> torch.Tensor:
+        ref = torch.empty(self.out.shape, dtype=torch.float64, device=self.x.device)
+        check(self.L.conv2d_direct_f64(ctypes.byref(self.pb), ctypes.c_void_p(ref.data_ptr()), _stream_handle(stream)),
+              "conv2d_direct_f64")
+        return ref
+
+    def u8_bytes(self) -> torch.Tensor:
+        return self.U[self.u16_bytes: self.u_bytes]
+
+    def u16_halves(self) -> torch.Tensor:
+        return self.U[: self.u16_bytes].view(torch.float16)
+
+    def v8_bytes(self) -> torch.Tensor:
+        return self.V[: self.v_bytes]
 
 
 def direct_f64(x: torch.Tensor, w: torch.Tensor, stride=(1, 1), pad=(1, 1)) -> torch.Tensor:

@@ -122,8 +122,8 @@ int getkernelInfo(__in__ problem_t* problem, __out__ kernelInfo_t* kernelInfo,
 
 /* Workspace sizes in bytes for transform_pin (V), transform_filter (U) and mid_result
  * (always 0). The reference harness sized them as 4*h*h*n*c, 16*k*c and 4*k*h*w*n halves
- * (main.cpp:33-38); those sizes suffice when h == w is even, c % 32 == 0, k % 16 == 0 and
- * n*(h/2)*(w/2) % 16 == 0, and are too small otherwise (ragged or padded shapes). */
+ * (main.cpp:33-38); those sizes suffice when h == w is even, c % 32 == 0, k % 32 == 0 and
+ * n*(h/2)*(w/2) % 32 == 0, and are too small otherwise (ragged or padded shapes). */
 int conv2d_workspace_size(const problem_t* problem, size_t* pin_bytes,
                           size_t* filter_bytes, size_t* mid_bytes);
 
@@ -157,6 +157,26 @@ int conv2d_select_tile_config(const problem_t* problem, int* cfg);
  * N,K,Oh,Ow doubles to out_f64 (device memory). The only FP64 arithmetic on the GPU;
  * no production stage uses it. */
 int conv2d_direct_f64(const problem_t* problem, double* out_f64, void* stream);
+
+/* ---- FP8 (OCP e4m3fn) compensated form: no counterpart in the reference. The GEMM stage
+ * runs on the scaled FP8 MFMA (v_mfma_scale_f32_32x32x64_f8f6f4): every transformed value is
+ * split into hi + lo e4m3 parts with one E8M0 exponent pair per (position, row, 32-channel
+ * block), and U.V is recovered as U_hi.V_hi + U_hi.V_lo + U_lo.V_hi in FP32. Same problem_t,
+ * different workspace layouts (sizes from conv2d_fp8_workspace_size); mid_result unused. ---- */
+/* Bytes of transform_pin (V records) and transform_filter (the FP16 U, then the U records). */
+int conv2d_fp8_workspace_size(const problem_t* problem, size_t* pin_bytes, size_t* filter_bytes);
+/* Filter transform (FP16, as conv2d_prepare_weights) followed by its FP8 quantisation. */
+int conv2d_fp8_prepare_weights(const problem_t* problem, void* stream);
+/* Input transform in FP32, quantised straight into V records. */
+int conv2d_fp8_input_transform(const problem_t* problem, void* stream);
+/* Three-pass FP8 GEMM with the fused output transform, tile configuration cfg
+ * (-1 = shape heuristic). */
+int conv2d_fp8_gemm(const problem_t* problem, int cfg, void* stream);
+/* Input transform + GEMM (weights prepared once). */
+int conv2d_fp8_forward(const problem_t* problem, int cfg, void* stream);
+int conv2d_fp8_num_tile_configs(void);
+int conv2d_fp8_tile_config(int cfg, int* bk, int* bt, int* threads);
+int conv2d_fp8_select_tile_config(const problem_t* problem, int* cfg);
 
 /* ---- Tuned tile-configuration table (replaces the reference's hard-coded (n, c) dispatch,
  * conv2d.cpp:273-494). getkernelInfo consults it before the shape heuristic. ---- */

@@ -41,7 +41,9 @@ _TAB = e4m3_table()
 
 
 def e4m3_encode(x: np.ndarray) -> np.ndarray:
-    """Round float values (|x| <= 448) to the nearest e4m3fn code, ties to even mantissa."""
+    """Round float values (|x| <= 448) to the nearest e4m3fn code, ties to even mantissa.
+    The sign bit is kept when the magnitude rounds to zero (-0 is code 0x80), as gfx950's
+    v_cvt_pk_fp8_f32 does (probed)."""
     x = np.asarray(x, dtype=np.float64)
     a = np.abs(x)
     assert np.all(a <= 448.0), "values must be pre-scaled into the e4m3 range"
@@ -60,7 +62,7 @@ def e4m3_encode(x: np.ndarray) -> np.ndarray:
     er = np.floor(np.log2(r[ne])).astype(np.int64)
     mant = np.round((r[ne] / 2.0 ** er - 1.0) * 8).astype(np.int64)
     code[ne] = ((er + 7) << 3) | mant
-    code = code | np.where((x < 0) & (code != 0), 0x80, 0)
+    code = code | np.where(np.signbit(x), 0x80, 0)
     return code.astype(np.uint8)
 
 
@@ -77,14 +79,21 @@ def block_exponent(m: np.ndarray) -> np.ndarray:
     return np.clip(E, -127, 127).astype(np.int64)
 
 
+LO_OFFSET = 4  # E_lo = E_hi - 4: a hi residual is at most 16 * 2^E_hi, so lo codes stay <= 256
+
+
 def quantize_blocks(v32: np.ndarray):
     """v32: (..., 32) float32 blocks. Returns (hi codes, lo codes, E_hi, E_lo) with
-    v ~= dec(hi) 2^E_hi + dec(lo) 2^E_lo, computed in the kernels' FP32 order."""
+    v ~= dec(hi) 2^E_hi + dec(lo) 2^E_lo, E_lo = max(E_hi - 4, -127).
+
+    Every intermediate is exact in FP32 (power-of-two scalings; the hi residual is the
+    difference between v and v rounded to 4 significant bits), so the FP32 kernels and this
+    FP64 restatement produce identical codes."""
     v = np.asarray(v32, dtype=np.float32).astype(np.float64)
     E_hi = block_exponent(np.max(np.abs(v), axis=-1))
     hi = e4m3_encode(v * 2.0 ** -E_hi[..., None])
-    r = v - e4m3_decode(hi) * 2.0 ** E_hi[..., None]  # exact in FP32 as well
-    E_lo = block_exponent(np.max(np.abs(r), axis=-1))
+    r = v - e4m3_decode(hi) * 2.0 ** E_hi[..., None]
+    E_lo = np.maximum(E_hi - LO_OFFSET, -127)
     lo = e4m3_encode(r * 2.0 ** -E_lo[..., None])
     return hi, lo, E_hi, E_lo
 
@@ -95,23 +104,23 @@ def dequant(codes, E):
 
 # ---- operands ---------------------------------------------------------------------------
 def _blocks(a: np.ndarray, C: int):
-    """(P, R, C) -> (P, R, CQ*4, 32) with channels padded to a multiple of 128."""
+    """(P, R, C) -> (P, R, CQ*2, 32) with channels padded to a multiple of 64."""
     P, R, _ = a.shape
-    CQ = (C + 127) // 128
-    full = np.zeros((P, R, CQ * 128), dtype=np.float32)
+    CQ = (C + 63) // 64
+    full = np.zeros((P, R, CQ * 64), dtype=np.float32)
     full[:, :, :C] = a
-    return full.reshape(P, R, CQ * 4, 32)
+    return full.reshape(P, R, CQ * 2, 32)
 
 
 def quantize_U(u16: np.ndarray):
-    """u16: (16, K, C) FP16 transformed filter -> dict of hi/lo codes (16, K, CQ*4, 32) and
-    exponents (16, K, CQ*4)."""
+    """u16: (16, K, C) FP16 transformed filter -> dict of hi/lo codes (16, K, CQ*2, 32) and
+    exponents (16, K, CQ*2)."""
     hi, lo, eh, el = quantize_blocks(_blocks(u16.astype(np.float32), u16.shape[2]))
     return {"hi": hi, "lo": lo, "eh": eh, "el": el}
 
 
 def quantize_V(v32: np.ndarray):
-    """v32: (16, C, NT) FP32 transformed input -> blocks over (16, NT, CQ*4, 32)."""
+    """v32: (16, C, NT) FP32 transformed input -> blocks over (16, NT, CQ*2, 32)."""
     vt = np.ascontiguousarray(v32.astype(np.float32).transpose(0, 2, 1))
     hi, lo, eh, el = quantize_blocks(_blocks(vt, v32.shape[1]))
     return {"hi": hi, "lo": lo, "eh": eh, "el": el}
@@ -140,30 +149,48 @@ def input_transform_f32(x16: np.ndarray, geo: wg.Geometry) -> np.ndarray:
     return wg.input_transform_fp32(x16, geo, keep_fp32=True)
 
 
-# ---- packed layouts ------------------------------------------------------------------------
-# Fragment (pos, cq, blk): 64 lanes x 32 bytes, lane = (c % 128 // 32) * 16 + r % 16,
-# byte = c % 32; scales (pos, cq, blk): 64 bytes, one E8M0 byte per lane.
-def pack_codes(codes: np.ndarray, rows_blocks: int) -> np.ndarray:
-    """codes (P, R, CQ*4, 32) -> flat bytes [pos][cq][blk][lane][32]."""
-    P, R, NB, _ = codes.shape
-    CQ = NB // 4
-    full = np.zeros((P, rows_blocks * 16, NB, 32), dtype=np.uint8)
-    full[:, :R] = codes
-    x = full.reshape(P, rows_blocks, 16, CQ, 4, 32).transpose(0, 3, 1, 4, 2, 5)
-    return np.ascontiguousarray(x).reshape(-1)
-
-
-def pack_scales(E: np.ndarray, rows_blocks: int) -> np.ndarray:
-    """E (P, R, CQ*4) exponents -> flat E8M0 bytes [pos][cq][blk][lane]."""
-    P, R, NB = E.shape
-    CQ = NB // 4
-    full = np.full((P, rows_blocks * 16, NB), 0, dtype=np.uint8)  # padding rows: scale 2^-127
-    full[:, :R] = (E + 127).astype(np.uint8)
-    x = full.reshape(P, rows_blocks, 16, CQ, 4).transpose(0, 3, 1, 4, 2)
-    return np.ascontiguousarray(x).reshape(-1)
+# ---- packed layout ---------------------------------------------------------------------------
+# The v_mfma_scale_f32_32x32x64_f8f6f4 operand (32 rows x 64 channels; probed on gfx950):
+# lane l holds row l % 32; its bytes 0-15 are channels 16 * (l // 32) + 0..15 of the first
+# 32-channel block, its bytes 16-31 the same channels of the second block; the E8M0 scale
+# of (row r, 32-channel block kb) is supplied by lane r + 32 * kb (op_sel picks the byte of
+# that lane's scale register). One record per (pos, cq, rb) (cq = 64-channel block, rb =
+# 32-row block), 4224 bytes:
+#   [0, 1024)     codes of block kb = 0: lane * 16 + (c % 16), lane = (c % 32 // 16) * 32 + r % 32
+#   [1024, 2048)  codes of block kb = 1, same arrangement            (hi part)
+#   [2048, 4096)  lo codes, same arrangement
+#   [4096, 4224)  scales: lane * 2 -> E_hi + 127, lane * 2 + 1 -> E_lo + 127, lane = kb * 32 + r % 32
+# Records are ordered [pos][cq][rb]. Each 1 KiB half is one contiguous run across the wave,
+# so a lane's operand is two conflict-free 16-byte LDS reads.
+REC_BYTES = 4224
 
 
 def pack_operand(q: dict, rows_blocks: int) -> np.ndarray:
-    """Workspace image: hi codes | lo codes | hi scales | lo scales."""
-    return np.concatenate([pack_codes(q["hi"], rows_blocks), pack_codes(q["lo"], rows_blocks),
-                           pack_scales(q["eh"], rows_blocks), pack_scales(q["el"], rows_blocks)])
+    """Workspace image of a quantised operand (dict from quantize_U / quantize_V)."""
+    P, R, NB, _ = q["hi"].shape
+    CQ = NB // 2
+    RB = rows_blocks
+    out = np.zeros((P, CQ, RB, REC_BYTES), dtype=np.uint8)
+
+    def halves(codes):  # (P, R, NB, 32) -> (P, CQ, RB, kb 2, lane 64, 16 bytes)
+        full = np.zeros((P, RB * 32, NB, 32), dtype=np.uint8)
+        full[:, :R] = codes
+        # (P, rb, r32, cq, kb, hl, b16) -> (P, cq, rb, kb, hl, r32, b16)
+        x = full.reshape(P, RB, 32, CQ, 2, 2, 16).transpose(0, 3, 1, 4, 5, 2, 6)
+        return x.reshape(P, CQ, RB, 2048)
+
+    def lane_exps(E):
+        full = np.zeros((P, RB * 32, NB), dtype=np.int64)
+        full[:, :R] = E + 127
+        x = full.reshape(P, RB, 32, CQ, 2).transpose(0, 3, 1, 4, 2)  # P cq rb kb r32
+        return x.reshape(P, CQ, RB, 64).astype(np.uint8)
+
+    out[..., 0:2048] = halves(q["hi"])
+    out[..., 2048:4096] = halves(q["lo"])
+    sc = np.stack([lane_exps(q["eh"]), lane_exps(q["el"])], axis=-1)  # (P, CQ, RB, 64, 2)
+    out[..., 4096:] = sc.reshape(P, CQ, RB, 128)
+    return out.reshape(-1)
+
+
+def operand_bytes(c: int, rows_blocks: int) -> int:
+    return 16 * ((c + 63) // 64) * rows_blocks * REC_BYTES
