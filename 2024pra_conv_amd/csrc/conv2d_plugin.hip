@@ -113,7 +113,16 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
     a->KB32 = g.KB32;
     a->band = g.band;
     a->cfg = cfg;
-    a->flags = ((pb->w % 8) == 0 && (((uintptr_t)pb->in) & 15) == 0) ? 1 : 0;
+    // widest input-row load the staging may use (rows start at multiples of w halves)
+    const uintptr_t in_addr = (uintptr_t)pb->in;
+    if (pb->w % 8 == 0 && (in_addr & 15) == 0)
+        a->flags = 1;
+    else if (pb->w % 4 == 0 && (in_addr & 7) == 0)
+        a->flags = 2;
+    else if (pb->w % 2 == 0 && (in_addr & 3) == 0)
+        a->flags = 4;
+    else
+        a->flags = 0;
 }
 
 Launch slot_filter(const Geometry& g) {

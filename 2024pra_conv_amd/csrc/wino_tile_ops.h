@@ -43,6 +43,33 @@ __device__ inline void load_window(const half_t* lds, int LR, int LWp, int lr0, 
     }
 }
 
+// Row staging for widths that are not a multiple of 8 halves: the band's (channel, row,
+// VEC-half chunk) items are spread over all threads of the workgroup (consecutive threads
+// take consecutive chunks of a row, and a channel's rows are contiguous in memory), so no
+// lane idles on short rows. VEC halves per load: W % VEC == 0 and the input base is
+// 2*VEC-byte aligned.
+template <int VEC>
+__device__ inline void stage_rows_flat(const half_t* in_n, int cb, int C, int H, int W, int ih0, int LR, int LWp,
+                                       int qq, half_t* lds) {
+    typedef uint16_t chunk_t __attribute__((ext_vector_type(VEC)));
+    const int nck = W / VEC;
+    const int per_ch = LR * nck;
+    const int total = 32 * per_ch;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int ch = i / per_ch;
+        const int rem = i - ch * per_ch;
+        const int lr = rem / nck;
+        const int j = rem - lr * nck;
+        const int c = cb * 32 + ch;
+        const int ih = ih0 + lr;
+        if (c >= C || ih < 0 || ih >= H) continue;
+        const chunk_t val = *(const chunk_t*)(in_n + ((size_t)c * H + ih) * W + j * VEC);
+        uint16_t* dst = (uint16_t*)lds + (ch * LR + lr) * LWp + qq + 8 + j * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) dst[e] = val[e];
+    }
+}
+
 // Stages the input rows of tile-row band bi (image n, 32-channel block cb) in LDS: zero fill
 // (padding halo, rows outside the image, channels >= C), then coalesced row loads. Ends with
 // a barrier, so the staging is visible to every wave.
@@ -83,17 +110,12 @@ __device__ inline void stage_input_band(const mykernelParamType& prm, int n, int
                 for (int e = 0; e < 8; ++e) dst[j * 8 + e] = val[e];
             }
         }
+    } else if (prm.flags & 2) {
+        stage_rows_flat<4>(in_n, cb, C, H, W, ih0, LR, LWp, qq, lds);
+    } else if (prm.flags & 4) {
+        stage_rows_flat<2>(in_n, cb, C, H, W, ih0, LR, LWp, qq, lds);
     } else {
-        for (int job = wave; job < njobs; job += nwaves) {
-            const int ch = job / LR;
-            const int lr = job - ch * LR;
-            const int c = cb * 32 + ch;
-            const int ih = ih0 + lr;
-            if (c >= C || ih < 0 || ih >= H) continue;
-            const half_t* src = in_n + ((size_t)c * H + ih) * W;
-            half_t* dst = lds + (ch * LR + lr) * LWp + qq + 8;
-            for (int iw = lane; iw < W; iw += 64) dst[iw] = src[iw];
-        }
+        stage_rows_flat<1>(in_n, cb, C, H, W, ih0, LR, LWp, qq, lds);
     }
     __syncthreads();
 }

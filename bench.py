@@ -33,10 +33,10 @@ PEAK_HBM_GBS = 8000.0
 WORKLOAD = dict(n=32, c=256, h=56, w=56, k=256)
 
 
-def _events_time(fn, iters: int) -> float:
+def _events_time(fn, iters: int, stream=None) -> float:
     """Average ms per call of fn over iters, HIP events on the launch stream."""
     import torch
-    s = torch.cuda.current_stream()
+    s = stream if stream is not None else torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
     for _ in range(iters):
@@ -68,6 +68,107 @@ def cpu_baseline(n, c, h, w, k) -> dict:
             "seconds": dt}
 
 
+PEAK_FP8_TFLOPS = 5033.2  # dense FP8 MFMA
+
+
+def _err(y, ref) -> dict:
+    d = (y.double() - ref).abs()
+    return {"max_rel": (d.max() / ref.abs().max()).item(), "rms_rel": (d.norm() / ref.norm()).item(),
+            "finite": bool(torch_isfinite(y))}
+
+
+def torch_isfinite(y) -> bool:
+    import torch
+    return bool(torch.isfinite(y).all())
+
+
+def fp8_section(runner, dev, steps: int) -> dict:
+    """The precision form: N=64 C=512 28x28 K=512, N(0,1) input with 1% of values x50, filters
+    N(0, 2/(9C)). FP8-compensated and plain FP16 on the same data, both against FP64."""
+    import torch
+    n, c, h, k = 64, 512, 28, 512
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20241)
+    x = torch.randn((n, c, h, h), generator=gen, device=dev, dtype=torch.float32)
+    mask = torch.rand((n, c, h, h), generator=gen, device=dev) < 0.01
+    x = torch.where(mask, x * 50.0, x).half()
+    wt = (torch.randn((k, c, 3, 3), generator=gen, device=dev, dtype=torch.float32) * (2.0 / (9 * c)) ** 0.5).half()
+    c8 = runner.WinogradConvFP8(x, wt)
+    c16 = runner.WinogradConv(x, wt)
+    cfg8, cfg16 = c8.selected_config(), c16.selected_config()
+    T = ((h + 1) // 2) ** 2
+    direct = 2.0 * n * k * c * 9 * h * h
+    useful = 32.0 * n * k * c * T
+    c8.prepare_weights()
+    c16.prepare_weights()
+    t16_full = _events_time(lambda: (c16.prepare_weights(), c16.forward(cfg16)), steps)
+    t8_full = _events_time(lambda: (c8.prepare_weights(), c8.forward(cfg8)), steps)
+    t16 = _events_time(lambda: c16.forward(cfg16), steps)
+    t8 = _events_time(lambda: c8.forward(cfg8), steps)
+    t8_s2 = _events_time(c8.input_transform, steps)
+    t8_s3 = _events_time(lambda: c8.gemm(cfg8), steps)
+    t16_s3 = _events_time(lambda: c16.gemm(cfg16), steps)
+    ref = c8.direct_f64()
+    c8.forward(cfg8)
+    c16.forward(cfg16)
+    torch.cuda.synchronize()
+    return {
+        "workload": f"N={n} C={c} H=W={h} K={k}, 1% outliers x50",
+        "fp8": {"tflops_with_weights": round(direct / (t8_full * 1e-3) / 1e12, 2),
+                "tflops_steady": round(direct / (t8 * 1e-3) / 1e12, 2),
+                "us_S2": round(t8_s2 * 1e3, 2), "us_S3": round(t8_s3 * 1e3, 2),
+                "gemm_issued_fp8_frac": round(3 * useful / (t8_s3 * 1e-3) / 1e12 / PEAK_FP8_TFLOPS, 4),
+                "gemm_useful_fp16_frac": round(useful / (t8_s3 * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 4),
+                "tile_config": cfg8, "error_vs_fp64": _err(c8.out, ref)},
+        "fp16": {"tflops_with_weights": round(direct / (t16_full * 1e-3) / 1e12, 2),
+                 "tflops_steady": round(direct / (t16 * 1e-3) / 1e12, 2),
+                 "us_S3": round(t16_s3 * 1e3, 2),
+                 "gemm_fp16_frac": round(useful / (t16_s3 * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 4),
+                 "tile_config": cfg16, "error_vs_fp64": _err(c16.out, ref)},
+        "fp8_over_fp16_steady": round(t16 / t8, 3),
+    }
+
+
+def batched_section(runner, dev, steps: int) -> dict:
+    """The batching form: 24 seeded layers (N in {1,4,16}, C,K in {3..512}, H=W in {7..112},
+    uniform[-1,1]) as one captured persistent graph vs 24 separate plain launches."""
+    import torch
+    shapes = runner.batch_layer_shapes()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20242)
+    convs, flops = [], 0.0
+    for (n, c, h, k) in shapes:
+        x = (torch.rand((n, c, h, h), generator=gen, device=dev) * 2 - 1).half()
+        wt = (torch.rand((k, c, 3, 3), generator=gen, device=dev) * 2 - 1).half()
+        convs.append(runner.WinogradConv(x, wt))
+        flops += 2.0 * n * k * c * 9 * h * h
+    b = runner.Batch(convs)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b.prepare_weights(s)
+        b.autotune(reps=3, stream=s)
+        b.capture_graph(s)
+        t_graph = _events_time(lambda: b.launch_graph(s), steps, s)
+        t_persist = _events_time(lambda: b.forward(s), steps, s)
+        t_sep = _events_time(lambda: b.forward_separate(s), steps, s)
+        b.launch_graph(s)
+        s.synchronize()
+        worst = 0.0
+        for cv in convs:
+            worst = max(worst, _err(cv.out, cv.direct_f64(s))["max_rel"])
+        s.synchronize()
+    cfgs = b.layer_configs()
+    b.close()
+    return {"layers": len(shapes), "direct_gflop": round(flops / 1e9, 2),
+            "us_graph": round(t_graph * 1e3, 2), "us_persistent": round(t_persist * 1e3, 2),
+            "us_separate": round(t_sep * 1e3, 2),
+            "tflops_graph": round(flops / (t_graph * 1e-3) / 1e12, 2),
+            "tflops_separate": round(flops / (t_sep * 1e-3) / 1e12, 2),
+            "graph_over_separate": round(t_sep / t_graph, 3),
+            "layer_configs": cfgs,
+            "max_rel_worst_layer": worst}
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "wino_gemm_n32_c256_h56_k256.md")
 
 
@@ -93,6 +194,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cfg", type=int, default=-1, help="GEMM tile configuration (-1 = library choice)")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP8 and batched sections")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before warm-up, run one device-to-device copy of this many MiB (untimed): the "
                          "known-size kernel that hardware byte counters are calibrated against")
@@ -199,6 +301,11 @@ def main() -> int:
                           "definition": "max|y-y*|/max|y*| and ||y-y*||/||y*|| vs FP64 direct conv"},
         "wall_s_timed": round(wall, 4),
     }
+    if not args.no_forms:
+        del conv
+        torch.cuda.empty_cache()
+        line["fp8_form"] = fp8_section(runner, dev, max(5, args.steps // 5))
+        line["batched_form"] = batched_section(runner, dev, max(5, args.steps // 5))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(1, c, h, w, k)
         line["cpu_baseline"].pop("seconds", None)

@@ -31,7 +31,8 @@ typedef struct mykernelParamType {
     int KB32;    /* ceil(k / 32), 32-row blocks of U */
     int band;    /* tile rows per input-transform workgroup */
     int cfg;     /* GEMM tile configuration index */
-    int flags;   /* bit 0: input rows are 16-byte aligned (w % 8 == 0 and aligned base) */
+    int flags;   /* input-row load width: bit 0 16 B (w % 8 == 0, 16-byte aligned base),
+                    bit 1 8 B (w % 4 == 0), bit 2 4 B (w % 2 == 0); none: 2 B */
 } mykernelParamType;
 
 /*
