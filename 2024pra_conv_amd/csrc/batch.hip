@@ -1,6 +1,7 @@
 // Multi-layer (batched) form: many conv layers of uneven shape run as one persistent input
-// transform and one persistent GEMM over (layer, tile-block) work queues, optionally
-// captured once into a hipGraph and replayed with a single launch.
+// transform over a (layer, band, channel block, image) work queue and one grouped GEMM
+// launch per tile configuration in use over (layer, k-block, column block) items, optionally
+// captured once into a hipGraph and replayed with a single graph launch.
 //
 // No counterpart in the reference, whose harness runs one shape per process
 // (src/main.cpp:8-166) with a hand-picked kernel quadruple per (n, c) (conv2d.cpp:273-494).
@@ -15,73 +16,75 @@
 #include "wino_gemm.h"
 
 struct conv2d_batch {
+    struct Segment {
+        int cfg;    // GEMM tile configuration of every layer in the segment
+        int first;  // first descriptor of the segment
+        int count;  // layers in the segment
+        int items;  // GEMM work items of the segment's queue
+    };
     std::vector<problem_t> layers;
-    std::vector<int> bcfg;               // batched GEMM configuration per layer
-    std::vector<BatchLayer> host;        // descriptors in queue order
+    std::vector<int> cfg;                // GEMM tile configuration per layer
+    std::vector<BatchLayer> host;        // descriptors, grouped by configuration
+    std::vector<Segment> segs;
     BatchLayer* d_layers = nullptr;
-    unsigned* d_counters = nullptr;      // [0] input-transform queue, [1] GEMM queue
-    int s2_total = 0, s3_total = 0;
+    unsigned* d_counters = nullptr;      // input-transform queue
+    int s2_total = 0;
     unsigned s2_lds = 0;
+    int cus = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
-// Global GEMM configuration index of each batched configuration (same tile shapes).
-const int kBatchToGlobal[kBatchCfgs] = {1, 2, 4};
-
-int global_to_batch(int cfg) {
-    for (int b = 0; b < kBatchCfgs; ++b)
-        if (kBatchToGlobal[b] == cfg) return b;
-    return -1;
-}
-
-int batch_heuristic(const problem_t& pb) {
-    const int kp = ((int)pb.k + 15) / 16 * 16;
-    if (kp >= 128) return 1;
-    if (kp >= 64) return 0;
-    return 2;
-}
-
-// Rebuilds the device descriptors from layers + bcfg. Layers are queued by decreasing
-// per-item GEMM cost so large items start first (the order changes no result).
+// Rebuilds the device descriptors from layers + cfg. Layers are grouped by tile configuration
+// (one persistent GEMM launch per group) and, inside a group, queued by decreasing per-item
+// cost so large items start first (the order changes no result).
 int upload(conv2d_batch* b) {
     const int nl = (int)b->layers.size();
     std::vector<int> order(nl);
     for (int i = 0; i < nl; ++i) order[i] = i;
     auto cost = [&](int i) {
-        const problem_t& p = b->layers[i];
-        return (double)batch_cfg_bk(b->bcfg[i]) * batch_cfg_bt(b->bcfg[i]) * p.c;
+        const GemmConfig& c = kGemmConfigs[b->cfg[i]];
+        return (double)c.bk * c.bt * b->layers[i].c;
     };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost(a) > cost(c); });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        if (b->cfg[a] != b->cfg[c]) return b->cfg[a] < b->cfg[c];
+        return cost(a) > cost(c);
+    });
     b->host.assign(nl, BatchLayer{});
-    int s2 = 0, s3 = 0;
+    b->segs.clear();
+    int s2 = 0;
     unsigned lds = 0;
     for (int qi = 0; qi < nl; ++qi) {
         const int i = order[qi];
         const problem_t& pb = b->layers[i];
+        const int cfg = b->cfg[i];
         wino::Geometry g;
         int rc = wino::require_winograd(&pb, &g);
         if (rc) return rc;
+        if (b->segs.empty() || b->segs.back().cfg != cfg)
+            b->segs.push_back(conv2d_batch::Segment{cfg, qi, 0, 0});
+        conv2d_batch::Segment& seg = b->segs.back();
         BatchLayer& bl = b->host[qi];
-        wino::fill_param(&pb, g, kBatchToGlobal[b->bcfg[i]], &bl.prm);
+        wino::fill_param(&pb, g, cfg, &bl.prm);
         bl.nbands = (g.tiles_h + g.band - 1) / g.band;
         bl.s2_first = s2;
         bl.s2_items = bl.nbands * g.CB * (int)pb.n;
         s2 += bl.s2_items;
-        const int uf = batch_cfg_bk(b->bcfg[i]) / 32, vf = batch_cfg_bt(b->bcfg[i]) / 32;
+        const GemmConfig& c = kGemmConfigs[cfg];
+        const int uf = c.bk / 32, vf = c.bt / 32;
         bl.nk = (g.KB32 + uf - 1) / uf;
         const int ng = (int)((g.G32 + vf - 1) / vf);
-        bl.s3_first = s3;
+        bl.s3_first = seg.items;
         bl.s3_items = bl.nk * ng;
-        s3 += bl.s3_items;
-        bl.bcfg = b->bcfg[i];
+        bl.cfg = cfg;
+        seg.items += bl.s3_items;
+        seg.count += 1;
         const unsigned l = (unsigned)input_lds_bytes(g.band, g.tiles_w, (int)pb.w, (int)pb.q);
         lds = std::max(lds, l);
     }
     b->s2_total = s2;
-    b->s3_total = s3;
     b->s2_lds = lds;
     if (!b->d_layers && hipMalloc(&b->d_layers, sizeof(BatchLayer) * nl) != hipSuccess) return CONV2D_ERR_HIP;
     if (hipMemcpy(b->d_layers, b->host.data(), sizeof(BatchLayer) * nl, hipMemcpyHostToDevice) != hipSuccess)
@@ -96,16 +99,25 @@ void drop_graph(conv2d_batch* b) {
     b->graph = nullptr;
 }
 
-// Resets the queues with a kernel (not a memset node: it is captured into graphs).
+// Resets the input-transform queue with a kernel (not a memset node: it is captured into
+// graphs), then one persistent input-transform launch over every layer and one grouped GEMM
+// launch per tile configuration in use.
 int launch_forward(conv2d_batch* b, hipStream_t st) {
     const int nl = (int)b->layers.size();
-    hipLaunchKernelGGL(wino_batch_reset, dim3(1), dim3(64), 0, st, b->d_counters);
-    const int g2 = std::min(b->s2_total, 256 * 4);
+    const int nseg = (int)b->segs.size();
+    hipLaunchKernelGGL(wino_batch_reset, dim3(1), dim3(64), 0, st, b->d_counters, 1);
+    const int g2 = std::min(b->s2_total, b->cus * 4);
     hipLaunchKernelGGL(wino_batch_input_transform, dim3(g2), dim3(256), b->s2_lds, st,
                        (const BatchLayer*)b->d_layers, nl, b->s2_total, b->d_counters);
-    const int g3 = std::min(b->s3_total, 256 * 2);
-    hipLaunchKernelGGL(wino_batch_gemm, dim3(g3), dim3(256), 0, st, (const BatchLayer*)b->d_layers, nl,
-                       b->s3_total, b->d_counters + 1);
+    for (int i = 0; i < nseg; ++i) {
+        const conv2d_batch::Segment& seg = b->segs[i];
+        const BatchLayer* layers = b->d_layers + seg.first;
+        int count = seg.count, items = seg.items;
+        void* args[3] = {(void*)&layers, (void*)&count, (void*)&items};
+        if (hipLaunchKernel(kGemmConfigs[seg.cfg].grouped_kernel, dim3(seg.items), dim3(kGemmConfigs[seg.cfg].threads),
+                            args, 0, st) != hipSuccess)
+            return CONV2D_ERR_HIP;
+    }
     return hipGetLastError() == hipSuccess ? CONV2D_OK : CONV2D_ERR_HIP;
 }
 
@@ -127,13 +139,18 @@ int conv2d_batch_create(const problem_t* layers, int nlayers, conv2d_batch** out
     conv2d_batch* b = new (std::nothrow) conv2d_batch;
     if (!b) return CONV2D_ERR_INVALID;
     b->layers.assign(layers, layers + nlayers);
-    b->bcfg.resize(nlayers);
+    b->cfg.resize(nlayers);
     for (int i = 0; i < nlayers; ++i) {
-        const int t = global_to_batch(wino::tuned_config_lookup(&layers[i]));
-        b->bcfg[i] = t >= 0 ? t : batch_heuristic(layers[i]);
+        wino::Geometry g;
+        (void)wino::require_winograd(&layers[i], &g);
+        b->cfg[i] = wino::select_config(&layers[i], g);
     }
     int rc = CONV2D_OK;
-    if (hipMalloc(&b->d_counters, 2 * sizeof(unsigned)) != hipSuccess) rc = CONV2D_ERR_HIP;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        rc = CONV2D_ERR_HIP;
+    if (!rc && hipMalloc(&b->d_counters, sizeof(unsigned)) != hipSuccess) rc = CONV2D_ERR_HIP;
     if (!rc) rc = upload(b);
     if (rc) {
         if (b->d_layers) (void)hipFree(b->d_layers);
@@ -175,7 +192,7 @@ int conv2d_batch_forward(conv2d_batch* b, void* stream) {
 int conv2d_batch_forward_separate(conv2d_batch* b, void* stream) {
     if (!b) return CONV2D_ERR_NULL;
     for (size_t i = 0; i < b->layers.size(); ++i) {
-        int rc = conv2d_forward(&b->layers[i], kBatchToGlobal[b->bcfg[i]], stream);
+        int rc = conv2d_forward(&b->layers[i], b->cfg[i], stream);
         if (rc) return rc;
     }
     return CONV2D_OK;
@@ -209,25 +226,27 @@ int conv2d_batch_launch_graph(conv2d_batch* b, void* stream) {
     return hipGraphLaunch(b->exec, (hipStream_t)stream) == hipSuccess ? CONV2D_OK : CONV2D_ERR_HIP;
 }
 
-// Per-layer tile configurations from the autotuner, restricted to the batched set. Each
-// layer's U and V must be prepared; every layer's output is overwritten.
+// Per-layer tile configurations from the autotuner (every configuration). Each layer's U
+// and V must be prepared; every layer's output is overwritten.
 int conv2d_batch_autotune(conv2d_batch* b, int reps, void* stream) {
     if (!b) return CONV2D_ERR_NULL;
+    std::vector<int> all(kNumGemmConfigs);
+    for (int c = 0; c < kNumGemmConfigs; ++c) all[c] = c;
     for (size_t i = 0; i < b->layers.size(); ++i) {
         int best = -1;
-        int rc = wino::autotune_subset(&b->layers[i], reps, stream, kBatchToGlobal, kBatchCfgs, &best, nullptr);
+        int rc = wino::autotune_subset(&b->layers[i], reps, stream, all.data(), kNumGemmConfigs, &best, nullptr);
         if (rc) return rc;
-        b->bcfg[i] = global_to_batch(best);
+        b->cfg[i] = best;
     }
     drop_graph(b);
     return upload(b);
 }
 
-// Batched GEMM configuration of layer i (as a global configuration index).
+// GEMM tile configuration of layer i.
 int conv2d_batch_layer_config(const conv2d_batch* b, int i, int* cfg) {
     if (!b || !cfg) return CONV2D_ERR_NULL;
     if (i < 0 || i >= (int)b->layers.size()) return CONV2D_ERR_INVALID;
-    *cfg = kBatchToGlobal[b->bcfg[i]];
+    *cfg = b->cfg[i];
     return CONV2D_OK;
 }
 

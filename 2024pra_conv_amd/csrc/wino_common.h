@@ -40,27 +40,22 @@ __device__ void input_transform_block(const mykernelParamType& prm, int n, int c
 struct BatchLayer {
     mykernelParamType prm;
     int s2_first, s2_items, nbands;
-    int s3_first, s3_items, nk;  // nk: k-blocks of the layer's GEMM tile (column blocks = items / nk)
-    int bcfg;                    // index into the batched GEMM configurations
+    int s3_first, s3_items, nk;  // GEMM items are numbered within the queue of the layer's tile
+                                 // configuration; nk: k-blocks (column blocks = items / nk)
+    int cfg;                     // GEMM tile configuration
     int pad_;
 };
 
-// Batched GEMM configurations (all 256 threads, so one persistent kernel runs any of them).
-constexpr int kBatchCfgs = 3;
-__host__ __device__ inline int batch_cfg_bk(int b) { return b == 1 ? 128 : (b == 2 ? 32 : 64); }
-__host__ __device__ inline int batch_cfg_bt(int b) { return b == 1 ? 64 : 128; }
-
-__global__ void wino_batch_reset(unsigned* counters);
+__global__ void wino_batch_reset(unsigned* counters, int n);
 __global__ void wino_batch_input_transform(const BatchLayer* __restrict__ layers, int nlayers,
                                            int total, unsigned* counter);
-__global__ void wino_batch_gemm(const BatchLayer* __restrict__ layers, int nlayers, int total,
-                                unsigned* counter);
 
-// Work-queue helper: thread 0 takes the next item index, every thread gets it.
+// Work-queue helper: thread 0 takes the next item index, every thread gets it (as a
+// wave-uniform value, so everything derived from it stays in scalar registers).
 __device__ inline int batch_next_item(unsigned* counter, int* slot) {
     if (threadIdx.x == 0) *slot = (int)atomicAdd(counter, 1u);
     __syncthreads();
-    const int item = *slot;
+    const int item = __builtin_amdgcn_readfirstlane(*slot);
     __syncthreads();
     return item;
 }

@@ -371,7 +371,7 @@ wino_gemm_fp8_kernel(mykernelParamType prm) {
 }
 
 #define FP8_CFG(BK, BT, WK, WT, D) \
-    { BK, BT, WK * WT * 64, D, (const void*)&wino_gemm_fp8_kernel<BK, BT, WK, WT, D> }
+    { BK, BT, WK * WT * 64, D, (const void*)&wino_gemm_fp8_kernel<BK, BT, WK, WT, D>, nullptr }
 
 const GemmConfig kFp8Configs[] = {
     FP8_CFG(128, 64, 2, 2, 2),   // 0: wave 64x32, 4 waves

@@ -7,6 +7,7 @@ struct GemmConfig {
     int threads;         // threads per workgroup
     int depth;           // global-load steps in flight (register ring); 0 = 2-D grid without XCD remap
     const void* kernel;  // host stub of wino_gemm_kernel<...>
+    const void* grouped_kernel;  // host stub of wino_gemm_grouped_kernel<...> (multi-layer form)
 };
 
 extern const GemmConfig kGemmConfigs[];

@@ -182,11 +182,208 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
 
+// ---------------------------------------------------------------------------------------
+// LDS-DMA variant. Operand fragments go straight from global memory into an S-stage LDS ring
+// with global_load_lds_dwordx4 (each wave moves whole 1 KiB fragments: lane i writes 16 bytes
+// at the wave's LDS base in m0 + 16 i), so the prefetch depth costs no registers and no LDS
+// store instructions; waves then read their MFMA operands from LDS into a register double
+// buffer. Written as inline assembly because the compiler's own waits for LDS DMA drain every
+// load in flight (vmcnt(0)) before each LDS read or barrier, which would leave one step of
+// prefetch; here the wait for step s is vmcnt((S - 3) * loads per step), counted by hand:
+// this wave issues exactly FPW of these loads per step and no other vector-memory loads
+// inside the loop.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ inline void lds_dma16(const half_t* gptr, uint32_t lds_byte_addr) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(lds_byte_addr), "v"(gptr)
+                 : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+template <int N>
+__device__ inline void wait_vm() {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+__device__ inline uint32_t lds_address(const half_t* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) half_t*)p;
+}
+
+template <int BK, int BT, int WK, int WT, int S>
+struct DmaShape {
+    static constexpr int kWaves = WK * WT;
+    static constexpr int kThreads = kWaves * 64;
+    static constexpr int MI = BK / WK / 32, NI = BT / WT / 32;
+    static constexpr int UF = BK / 32, VF = BT / 32;
+    static constexpr int kFrags = KS * (UF + VF);  // 1 KiB fragments per step
+    static constexpr int kFPW = kFrags / kWaves;   // DMA loads per wave per step
+    static constexpr int kStageHalves = kFrags * kFragHalves;
+    static constexpr int kSmemHalves = S * kStageHalves;
+    static_assert(kFrags % kWaves == 0, "fragments per step must split evenly over the waves");
+    static_assert(S >= 4 && 16 % S == 0, "stage count: a power of two dividing the step count");
+    static_assert((S - 2) * kFPW <= 63, "vmcnt range");
+    static_assert(MI >= 1 && NI >= 1 && BK % (WK * 32) == 0 && BT % (WT * 32) == 0, "tile shape");
+};
+
+}  // namespace
+
+template <int BK, int BT, int WK, int WT, int S>
+__device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
+    using Sh = DmaShape<BK, BT, WK, WT, S>;
+    constexpr int MI = Sh::MI, NI = Sh::NI, UF = Sh::UF, VF = Sh::VF, FPW = Sh::kFPW;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wk = wave % WK;
+    const int wt = wave / WK;
+    const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
+    const int gb0 = gblk * VF;
+    const int kb0 = kblk * UF;
+    const int nsteps = 16 * CB;
+    const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
+    const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
+
+    // this wave's DMA slots: fragments wave*FPW .. wave*FPW+FPW-1 of the stage layout
+    // [kk][UF U fragments][VF V fragments]; beyond-the-matrix fragments read fragment 0
+    const half_t* src[FPW];
+    size_t stride[FPW];
+#pragma unroll
+    for (int j = 0; j < FPW; ++j) {
+        const int f = wave * FPW + j;
+        const int kk = f / (UF + VF), q = f % (UF + VF);
+        if (q < UF) {
+            const bool ok = kb0 + q < KB;
+            src[j] = U + (ok ? ((size_t)kk * KB + kb0 + q) * kFragHalves : 0) + lane * 8;
+            stride[j] = (size_t)2 * KB * kFragHalves;
+        } else {
+            const bool ok = gb0 + q - UF < G32;
+            src[j] = V + (ok ? ((size_t)kk * G32 + gb0 + q - UF) * kFragHalves : 0) + lane * 8;
+            stride[j] = (size_t)2 * G32 * kFragHalves;
+        }
+    }
+    const uint32_t lds0 = lds_address(smem);
+    auto dma = [&](int step, int stage) {
+        const size_t s = (size_t)min(step, nsteps - 1);
+#pragma unroll
+        for (int j = 0; j < FPW; ++j) {
+            const uint32_t dst = __builtin_amdgcn_readfirstlane(
+                lds0 + (uint32_t)((stage * Sh::kStageHalves + (wave * FPW + j) * kFragHalves) * 2));
+            lds_dma16(src[j] + s * stride[j], dst);
+        }
+    };
+
+    floatx16 acc[MI][NI];
+    floatx16 Y[4][MI][NI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Y[q][mi][ni] = acc[mi][ni];
+        }
+
+    half8 ra[2][KS][MI], rb[2][KS][NI];
+    auto lread = [&](half8 (&A)[KS][MI], half8 (&B)[KS][NI], int stage) {
+        const half_t* sb = smem + stage * Sh::kStageHalves;
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const half_t* sk = sb + kk * (UF + VF) * kFragHalves;
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi) A[kk][mi] = *(const half8*)(sk + ((wk * MI + mi) * 64 + lane) * 8);
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni)
+                B[kk][ni] = *(const half8*)(sk + UF * kFragHalves + ((wt * NI + ni) * 64 + lane) * 8);
+        }
+    };
+
+    // Pipeline (stage = step % S; registers = step & 1):
+    //   prologue: DMA steps 0..S-2; wait for step 0; barrier; regs <- stage 0
+    //   iteration s: wait for steps <= s+1 (S-3 steps stay in flight) | barrier (stage s+1
+    //   complete for every wave; every wave's reads of stage s-1 done) | DMA step s+S-1 into
+    //   stage s-1 | regs(s+1) <- stage s+1 | MFMAs on regs(s) | fold at a position's end
+#pragma unroll
+    for (int p = 0; p < S - 1; ++p) dma(p, p);
+    wait_vm<(S - 2) * FPW>();
+    __syncthreads();
+    lread(ra[0], rb[0], 0);
+
+    int pos = 0, cb = 0;
+    for (int s0 = 0; s0 < nsteps; s0 += S) {
+#pragma unroll
+        for (int d = 0; d < S; ++d) {
+            const int s = s0 + d;
+            const int cur = d & 1;  // S is even, so s & 1 == d & 1
+            wait_vm<(S - 3) * FPW>();
+            __syncthreads();
+            dma(s + S - 1, (d + S - 1) % S);
+            lread(ra[cur ^ 1], rb[cur ^ 1], (d + 1) % S);
+#pragma unroll
+            for (int kk = 0; kk < KS; ++kk)
+#pragma unroll
+                for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < NI; ++ni)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[cur][kk][mi], rb[cur][kk][ni],
+                                                                             acc[mi][ni], 0, 0, 0);
+            if (++cb == CB) {
+                fold_position(pos, acc, Y);
+#pragma unroll
+                for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+                cb = 0;
+                ++pos;
+            }
+        }
+    }
+    wait_vm<0>();  // the trailing (re-read) DMAs land before the workgroup's LDS is released
+
+    store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
+}
+
 // Waves per SIMD the register allocation must allow: two for 64x32 wave tiles (so two waves
 // of a SIMD overlap LDS traffic with MFMAs), one for 64x64 tiles (their output accumulators
 // alone take 256 registers).
 template <int BK, int BT, int WK, int WT>
 constexpr int kWavesPerEU = (BK / WK / 32) * (BT / WT / 32) >= 4 ? 1 : 2;
+
+template <int BK, int BT, int WK, int WT, int S>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
+wino_gemm_dma_kernel(mykernelParamType prm) {
+    using Sh = DmaShape<BK, BT, WK, WT, S>;
+    __shared__ __attribute__((aligned(16))) half_t smem[Sh::kSmemHalves];
+    const int nk = (prm.KB32 + Sh::UF - 1) / Sh::UF;
+    const int ng = (prm.G32 + Sh::VF - 1) / Sh::VF;
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
+    gemm_dma_body<BK, BT, WK, WT, S>(prm, kblk, gblk, smem);
+}
+
+template <int BK, int BT, int WK, int WT, int S>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
+wino_gemm_dma_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
+    using Sh = DmaShape<BK, BT, WK, WT, S>;
+    __shared__ __attribute__((aligned(16))) half_t smem[Sh::kSmemHalves];
+    int item, unused;
+    xcd_remap(blockIdx.x, total, total, &item, &unused);
+    int lo = 0, hi = nlayers - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (layers[mid].s3_first <= item) lo = mid; else hi = mid - 1;
+    }
+    const BatchLayer& bl = layers[lo];
+    const int local = item - bl.s3_first;
+    gemm_dma_body<BK, BT, WK, WT, S>(bl.prm, local % bl.nk, local / bl.nk, smem);
+}
 
 template <int BK, int BT, int WK, int WT, int D>
 __global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
@@ -201,45 +398,51 @@ wino_gemm_kernel(mykernelParamType prm) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Multi-layer persistent GEMM: workgroups pull (layer, k-block, column block) items from
-// one queue; each layer runs the tile configuration chosen for it (256-thread shapes, so
-// one kernel covers all; the LDS buffer is sized for the largest).
+// Multi-layer (grouped) GEMM: one launch covers every layer that uses this tile
+// configuration. Workgroup ids are remapped per XCD (as in the single-layer kernel) onto the
+// group's (layer, k-block, column block) items, which are numbered layer by layer in
+// decreasing cost; a binary search finds the item's layer. One item per workgroup keeps the
+// register allocation of the single-layer kernel (a persistent loop around the body spills).
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) wino_batch_gemm(const BatchLayer* __restrict__ layers, int nlayers,
-                                                       int total, unsigned* counter) {
-    constexpr int kSmem = 2 * GemmShape<64, 128, 1, 4>::kStageHalves;
-    static_assert(2 * GemmShape<128, 64, 2, 2>::kStageHalves <= kSmem, "LDS sizing");
-    static_assert(2 * GemmShape<32, 128, 1, 4>::kStageHalves <= kSmem, "LDS sizing");
-    __shared__ __attribute__((aligned(16))) half_t smem[kSmem];
-    __shared__ int slot;
-    for (;;) {
-        const int item = batch_next_item(counter, &slot);
-        if (item >= total) break;
-        const int L = batch_find_layer(layers, nlayers, item, true);
-        const BatchLayer& bl = layers[L];
-        const int local = item - bl.s3_first;
-        const int kblk = local % bl.nk;
-        const int gblk = local / bl.nk;
-        switch (bl.bcfg) {
-            case 1: gemm_body<128, 64, 2, 2, 2>(bl.prm, kblk, gblk, smem); break;
-            case 2: gemm_body<32, 128, 1, 4, 2>(bl.prm, kblk, gblk, smem); break;
-            default: gemm_body<64, 128, 1, 4, 2>(bl.prm, kblk, gblk, smem); break;
-        }
-        __syncthreads();
+template <int BK, int BT, int WK, int WT, int D>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
+wino_gemm_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
+    using S = GemmShape<BK, BT, WK, WT>;
+    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
+    int item, unused;
+    xcd_remap(blockIdx.x, total, total, &item, &unused);
+    int lo = 0, hi = nlayers - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (layers[mid].s3_first <= item) lo = mid; else hi = mid - 1;
     }
+    const BatchLayer& bl = layers[lo];
+    const int local = item - bl.s3_first;
+    gemm_body<BK, BT, WK, WT, D>(bl.prm, local % bl.nk, local / bl.nk, smem);
 }
 
 // ---- configuration table --------------------------------------------------------------
 
-#define WINO_CFG(BK, BT, WK, WT, D) \
-    { BK, BT, WK * WT * 64, D, (const void*)&wino_gemm_kernel<BK, BT, WK, WT, D> }
+#define WINO_CFG(BK, BT, WK, WT, D)                                                     \
+    {                                                                                   \
+        BK, BT, WK * WT * 64, D, (const void*)&wino_gemm_kernel<BK, BT, WK, WT, D>,      \
+            (const void*)&wino_gemm_grouped_kernel<BK, BT, WK, WT, D>                      \
+    }
+
+// depth field: register-ring depth D for the register-staged kernels, LDS stages S for the
+// LDS-DMA kernels
+#define WINO_DMA_CFG(BK, BT, WK, WT, S)                                                  \
+    {                                                                                    \
+        BK, BT, WK * WT * 64, S, (const void*)&wino_gemm_dma_kernel<BK, BT, WK, WT, S>,   \
+            (const void*)&wino_gemm_dma_grouped_kernel<BK, BT, WK, WT, S>                  \
+    }
 
 const GemmConfig kGemmConfigs[] = {
     WINO_CFG(128, 128, 2, 4, 2),  // 0: wave 64x32, 8 waves
-    WINO_CFG(64, 128, 1, 4, 2),   // 1: wave 64x32, 4 waves (batched set)
-    WINO_CFG(128, 64, 2, 2, 2),   // 2: wave 64x32, 4 waves (batched set)
+    WINO_CFG(64, 128, 1, 4, 2),   // 1: wave 64x32, 4 waves
+    WINO_CFG(128, 64, 2, 2, 2),   // 2: wave 64x32, 4 waves
     WINO_CFG(64, 64, 1, 2, 2),    // 3: wave 64x32, 2 waves
-    WINO_CFG(32, 128, 1, 4, 2),   // 4: wave 32x32, 4 waves (batched set)
+    WINO_CFG(32, 128, 1, 4, 2),   // 4: wave 32x32, 4 waves
     WINO_CFG(256, 64, 4, 2, 2),   // 5: wave 64x32, 8 waves
     WINO_CFG(32, 256, 1, 8, 2),   // 6: wave 32x32, 8 waves (small K)
     WINO_CFG(64, 256, 1, 8, 2),   // 7: wave 64x32, 8 waves
@@ -251,5 +454,8 @@ const GemmConfig kGemmConfigs[] = {
     WINO_CFG(64, 128, 1, 4, 4),   // 13
     WINO_CFG(128, 64, 2, 2, 4),   // 14
     WINO_CFG(32, 128, 1, 4, 4),   // 15
+    WINO_DMA_CFG(128, 128, 2, 2, 4),  // 16: LDS-DMA ring, wave 64x64, 4 waves
+    WINO_DMA_CFG(128, 64, 2, 2, 4),   // 17: LDS-DMA ring, wave 64x32, 4 waves
+    WINO_DMA_CFG(64, 128, 1, 2, 4),   // 18: LDS-DMA ring, wave 64x64, 2 waves
 };
 const int kNumGemmConfigs = (int)(sizeof(kGemmConfigs) / sizeof(kGemmConfigs[0]));

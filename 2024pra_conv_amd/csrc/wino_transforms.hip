@@ -147,8 +147,8 @@ __global__ void wino_noop(mykernelParamType) {}
 // Multi-layer persistent input transform: workgroups pull (layer, band, channel block,
 // image) items from one queue until it is empty. Dynamic LDS = the largest layer's staging.
 // ---------------------------------------------------------------------------------------
-__global__ void wino_batch_reset(unsigned* counters) {
-    if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+__global__ void wino_batch_reset(unsigned* counters, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0u;
 }
 
 __global__ void __launch_bounds__(256) wino_batch_input_transform(const BatchLayer* __restrict__ layers,

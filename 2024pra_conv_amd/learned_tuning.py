@@ -8,8 +8,9 @@ workload (N in {1,4,16}, C,K in {3,32,64,96,128,256,512}, H=W in {7,13,14,27,28,
   python -m 2024pra_conv_amd.learned_tuning --train 2000 --test 200   (from the repo root,
   via: python 2024pra_conv_amd/learned_tuning.py ...)
 
-Prints one JSON line: mean / median / p90 regret of the MLP's choice and of the shape
-heuristic (the method without the network) against the exhaustive best, and the tuning
+Prints one JSON line: mean / median / p90 regret of the MLP's choice, of the library's shape
+heuristic, of a static K-threshold dispatch and of a random choice against the exhaustive
+best, and the tuning
 wall-clock of exhaustive search vs. MLP inference. Saves the trained parameters (.npy) and
 the measurement table (.npz) when --out is given, so tuning and training can resume.
 """
@@ -126,6 +127,12 @@ def select(P: torch.Tensor, shapes):
     return np.array(out), time.perf_counter() - t0
 
 
+def static_k_dispatch(shapes) -> np.ndarray:
+    """A static tile choice by output-channel count alone (the style of a hand-written
+    per-shape dispatch): 128x128 for K >= 128, 64x128 for K >= 64, 32x128 for K >= 32."""
+    return np.array([0 if k >= 128 else (1 if k >= 64 else (4 if k >= 32 else 6)) for (_, _, _, k) in shapes])
+
+
 def regret(times: np.ndarray, choice: np.ndarray) -> np.ndarray:
     best = times.min(axis=1)
     return times[np.arange(len(choice)), choice] / best - 1.0
@@ -143,6 +150,7 @@ def run(n_train: int, n_test: int, steps: int, lr: float, reps: int, seed: int, 
     mlp_choice, infer_s = select(P, te_shapes)
     r_mlp = regret(te_times, mlp_choice)
     r_heur = regret(te_times, te_heur)
+    r_static = regret(te_times, static_k_dispatch(te_shapes))
     rng = np.random.default_rng(seed + 2)
     r_rand = regret(te_times, rng.integers(0, ncfg, len(te_shapes)))
     if out:
@@ -159,7 +167,8 @@ def run(n_train: int, n_test: int, steps: int, lr: float, reps: int, seed: int, 
         "workload": f"learned tuning: {n_train} train / {n_test} held-out shapes x {ncfg} tile configs",
         "mlp": "21-64-64-1 FP32, full-batch Adam on GPU (library HIP kernels)",
         "train_steps": steps, "final_train_mse_log_time": round(final_loss, 5), "train_seconds": round(train_s, 2),
-        "regret_mlp": stats(r_mlp), "regret_heuristic": stats(r_heur), "regret_random": stats(r_rand),
+        "regret_mlp": stats(r_mlp), "regret_library_heuristic": stats(r_heur),
+        "regret_static_k_dispatch": stats(r_static), "regret_random": stats(r_rand),
         "held_out_total_time_us": {"exhaustive_best": round(float(te_times.min(axis=1).sum()), 1),
                                    "mlp_choice": round(float(te_times[np.arange(n_test), mlp_choice].sum()), 1),
                                    "heuristic": round(float(te_times[np.arange(n_test), te_heur].sum()), 1)},

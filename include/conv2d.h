@@ -199,8 +199,9 @@ int conv2d_autotune(const problem_t* problem, int reps, void* stream, int* best_
 
 /* ---- Multi-layer (batched) form: no counterpart in the reference (its harness runs one
  * shape per process, main.cpp:8-166). Many layers of uneven shape run as one persistent
- * input-transform launch and one persistent GEMM launch over (layer, tile-block) work
- * queues; the pair can be captured once into a hipGraph and replayed with one launch. ---- */
+ * input-transform launch over a (layer, band, channel block, image) work queue and one
+ * grouped GEMM launch per tile configuration in use over (layer, tile-block) items; the
+ * sequence can be captured once into a hipGraph and replayed with one launch. ---- */
 typedef struct conv2d_batch conv2d_batch;
 /* Validates every layer (all pointers set, Winograd geometry) and uploads descriptors.
  * Per-layer tile configurations come from the tuned table or the shape heuristic. */
@@ -208,14 +209,14 @@ int conv2d_batch_create(const problem_t* layers, int nlayers, conv2d_batch** bat
 int conv2d_batch_destroy(conv2d_batch* batch);
 /* Filter transforms of every layer (once per weight set). */
 int conv2d_batch_prepare_weights(conv2d_batch* batch, void* stream);
-/* Input transforms + GEMMs of all layers as two persistent launches. */
+/* Input transforms + GEMMs of all layers (the launches described above). */
 int conv2d_batch_forward(conv2d_batch* batch, void* stream);
 /* The same work as separate per-layer launches with the same tile configurations. */
 int conv2d_batch_forward_separate(conv2d_batch* batch, void* stream);
 /* Captures conv2d_batch_forward into a hipGraph on a non-null stream / replays it. */
 int conv2d_batch_capture_graph(conv2d_batch* batch, void* stream);
 int conv2d_batch_launch_graph(conv2d_batch* batch, void* stream);
-/* Autotunes each layer over the batched configurations (U and V prepared; outputs
+/* Autotunes each layer over every GEMM tile configuration (U and V prepared; outputs
  * overwritten). Invalidates a captured graph. */
 int conv2d_batch_autotune(conv2d_batch* batch, int reps, void* stream);
 /* Tile configuration (global index) used for layer i. */

@@ -306,7 +306,7 @@ def test_batched_equals_separate_and_graph_replays(runner, cuda):
     b.close()
 
 
-def test_batched_autotune_keeps_results(runner, cuda):
+def test_batched_autotune_keeps_results(runner, wc, cuda):
     _log("case batched autotune")
     convs = _batch_convs(runner, cuda, BATCH_SMALL[:4], seed=700)
     b = runner.Batch(convs)
@@ -316,7 +316,7 @@ def test_batched_autotune_keeps_results(runner, cuda):
     ref = [c.out.clone() for c in convs]
     b.autotune(reps=2)
     cfgs = b.layer_configs()
-    assert all(c in (1, 2, 4) for c in cfgs)
+    assert all(0 <= c < len(wc.tile_configs()) for c in cfgs)
     b.forward()
     torch.cuda.synchronize()
     for i, c in enumerate(convs):
