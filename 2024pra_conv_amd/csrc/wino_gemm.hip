@@ -438,24 +438,21 @@ wino_gemm_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int
     }
 
 const GemmConfig kGemmConfigs[] = {
-    WINO_CFG(128, 128, 2, 4, 2),  // 0: wave 64x32, 8 waves
-    WINO_CFG(64, 128, 1, 4, 2),   // 1: wave 64x32, 4 waves
-    WINO_CFG(128, 64, 2, 2, 2),   // 2: wave 64x32, 4 waves
-    WINO_CFG(64, 64, 1, 2, 2),    // 3: wave 64x32, 2 waves
-    WINO_CFG(32, 128, 1, 4, 2),   // 4: wave 32x32, 4 waves
-    WINO_CFG(256, 64, 4, 2, 2),   // 5: wave 64x32, 8 waves
-    WINO_CFG(32, 256, 1, 8, 2),   // 6: wave 32x32, 8 waves (small K)
-    WINO_CFG(64, 256, 1, 8, 2),   // 7: wave 64x32, 8 waves
-    WINO_CFG(128, 128, 2, 4, 4),  // 8
-    WINO_CFG(128, 128, 2, 2, 2),  // 9: wave 64x64, 4 waves (one per SIMD)
-    WINO_CFG(32, 64, 1, 2, 2),    // 10: wave 32x32, 2 waves (small problems)
-    WINO_CFG(256, 64, 4, 2, 4),   // 11
-    WINO_CFG(64, 128, 1, 2, 2),   // 12: wave 64x64, 2 waves
-    WINO_CFG(64, 128, 1, 4, 4),   // 13
-    WINO_CFG(128, 64, 2, 2, 4),   // 14
-    WINO_CFG(32, 128, 1, 4, 4),   // 15
-    WINO_DMA_CFG(128, 128, 2, 2, 4),  // 16: LDS-DMA ring, wave 64x64, 4 waves
-    WINO_DMA_CFG(128, 64, 2, 2, 4),   // 17: LDS-DMA ring, wave 64x32, 4 waves
-    WINO_DMA_CFG(64, 128, 1, 2, 4),   // 18: LDS-DMA ring, wave 64x64, 2 waves
+    WINO_CFG(128, 128, 2, 4, 2),      // 0: wave 64x32, 8 waves
+    WINO_CFG(64, 128, 1, 4, 2),       // 1: wave 64x32, 4 waves
+    WINO_DMA_CFG(128, 64, 2, 2, 4),   // 2: LDS-DMA, 4-stage ring, wave 64x32, 4 waves
+    WINO_CFG(64, 64, 1, 2, 2),        // 3: wave 64x32, 2 waves
+    WINO_CFG(32, 128, 1, 4, 2),       // 4: wave 32x32, 4 waves
+    WINO_DMA_CFG(128, 128, 2, 4, 4),  // 5: LDS-DMA, 4-stage ring, wave 64x32, 8 waves
+    WINO_CFG(32, 256, 1, 8, 2),       // 6: wave 32x32, 8 waves (small K)
+    WINO_CFG(64, 256, 1, 8, 2),       // 7: wave 64x32, 8 waves
+    WINO_CFG(128, 128, 2, 4, 4),      // 8: wave 64x32, 8 waves, ring depth 4
+    WINO_DMA_CFG(128, 128, 2, 4, 8),  // 9: LDS-DMA, 8-stage ring (128 KiB), wave 64x32, 8 waves
+    WINO_CFG(32, 64, 1, 2, 2),        // 10: wave 32x32, 2 waves (small problems)
+    WINO_DMA_CFG(64, 128, 1, 4, 4),   // 11: LDS-DMA, 4-stage ring, wave 64x32, 4 waves
+    WINO_DMA_CFG(64, 64, 1, 2, 4),    // 12: LDS-DMA, 4-stage ring, wave 64x32, 2 waves
+    WINO_CFG(64, 128, 1, 4, 4),       // 13: wave 64x32, 4 waves, ring depth 4
+    WINO_CFG(128, 64, 2, 2, 4),       // 14: wave 64x32, 4 waves, ring depth 4
+    WINO_CFG(32, 128, 1, 4, 4),       // 15: wave 32x32, 4 waves, ring depth 4
 };
 const int kNumGemmConfigs = (int)(sizeof(kGemmConfigs) / sizeof(kGemmConfigs[0]));
