@@ -98,17 +98,17 @@ def test_training_fits_a_smooth_target_and_select_is_argmin(wc, cuda):
 
 
 def test_trained_model_gains_over_static_choice(wc, cuda):
-    """What the learned part gains: autotuner timings of 240 seeded shapes (drawn as in the
+    """What the learned part gains: autotuner timings of 600 seeded shapes (drawn as in the
     24-layer workload) train the MLP on the GPU; on 60 held-out shapes its chosen tile
     configuration is compared, over one measured table, with exhaustive search (time saved,
     regret), a static K-threshold dispatch, and the library's occupancy heuristic (printed)."""
     import importlib
     lt = importlib.import_module("2024pra_conv_amd.learned_tuning")
     ncfg = wc.load_library().conv2d_num_tile_configs()
-    tr, te = lt.draw_shapes(21, 240), lt.draw_shapes(22, 60)
+    tr, te = lt.draw_shapes(21, 600), lt.draw_shapes(22, 60)
     tr_t, _, _ = lt.measure(tr, reps=2, log_every=0)
     te_t, te_lib, te_wall = lt.measure(te, reps=2, log_every=0)
-    P, loss, _ = lt.train(lt.features(tr, ncfg), np.log(tr_t.reshape(-1)), steps=1500, lr=2e-3, seed=3)
+    P, loss, _ = lt.train(lt.features(tr, ncfg), np.log(tr_t.reshape(-1)), steps=3000, lr=2e-3, seed=3)
     choice, infer_s = lt.select(P, te)
     r_mlp = lt.regret(te_t, choice)
     r_static = lt.regret(te_t, lt.static_k_dispatch(te))
