@@ -38,7 +38,7 @@ struct conv2d_batch {
 namespace {
 
 // Rebuilds the device descriptors from layers + cfg. Layers are grouped by tile configuration
-// (one persistent GEMM launch per group) and, inside a group, queued by decreasing per-item
+// (one grouped GEMM launch per group) and, inside a group, queued by decreasing per-item
 // cost so large items start first (the order changes no result).
 int upload(conv2d_batch* b) {
     const int nl = (int)b->layers.size();

@@ -364,11 +364,13 @@ int conv2d_direct_f64(const problem_t* pb, double* out_f64, void* stream) {
 
 // ---- FP8 compensated form ----------------------------------------------------------------
 
+// FP8 tile heuristic: on MI355X the 32-row tiles measured fastest (more workgroups per
+// launch); 32x128 once it gives two workgroups per compute unit, else 32x64.
 static int fp8_heuristic_config(const problem_t* pb) {
-    const int kp = ((int)pb->k + 31) / 32 * 32;
-    if (kp >= 128) return 0;
-    if (kp >= 64) return 1;
-    return 3;
+    Geometry g;
+    if (derive_geometry(pb, &g)) return 3;
+    const long long items = (long long)g.KB32 * ((g.G32 + 3) / 4);
+    return items >= 2LL * compute_units() ? 3 : 6;
 }
 
 static int fp8_prepare(const problem_t* pb, int cfg, Geometry* g, mykernelParamType* a, int* out_cfg) {

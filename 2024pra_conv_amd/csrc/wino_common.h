@@ -15,7 +15,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // 16 B vector of raw bits, used for global <-> LDS copies.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Elements of one packed MFMA fragment (16x32 FP16 operand = 64 lanes x 8 halves).
+// Elements of one packed MFMA fragment (32 rows x 16 channels of FP16 = 64 lanes x 8 halves).
 constexpr int kFragHalves = 512;
 
 // Offset (in halves) of fragment (pos, cb, blk) in a packed U or V buffer whose

@@ -24,6 +24,10 @@
 // re-read the final step. Without branches the compiler's counter-based waits
 // (s_waitcnt vmcnt(N)) only wait for the oldest step, so D steps of HBM latency overlap
 // with MFMA work.
+//
+// A second body (gemm_dma_body, below) stages the fragments with LDS DMA into a deeper LDS
+// ring instead of the register ring; both bodies compute every output element in the same
+// order, so all configurations give bit-identical results.
 #include "wino_gemm.h"
 #include "wino_tile_ops.h"
 

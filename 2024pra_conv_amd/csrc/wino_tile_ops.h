@@ -200,27 +200,6 @@ __device__ inline void store_tiles32(const mykernelParamType& prm, const floatx1
     }
 }
 
-// Epilogue for 16x16 MFMA accumulators (element j of lane l is row 4*(l/16) + j, column
-// l%16); gbw / kbw count 16-column / 16-row blocks.
-template <int MI, int NI>
-__device__ inline void store_tiles16(const mykernelParamType& prm, const floatx4 (&Y)[4][MI][NI], int gbw,
-                                     int kbw, int lane) {
-#pragma unroll
-    for (int ni = 0; ni < NI; ++ni) {
-        const int g = (gbw + ni) * 16 + (lane & 15);
-        if (g >= prm.NT) continue;
-        const TileCoord tc = tile_coord(prm, g);
-#pragma unroll
-        for (int mi = 0; mi < MI; ++mi)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int k = (kbw + mi) * 16 + 4 * (lane >> 4) + j;
-                if (k >= prm.k) continue;
-                store_tile(prm, tc, k, Y[0][mi][ni][j], Y[1][mi][ni][j], Y[2][mi][ni][j], Y[3][mi][ni][j]);
-            }
-    }
-}
-
 // Hardware dispatches workgroup ids round-robin over the 8 XCDs; this remaps id L so every
 // XCD gets a contiguous run of the (k-block fastest) work list, keeping the k-blocks that
 // share one column block of V adjacent on one XCD (same L2).
