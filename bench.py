@@ -167,6 +167,8 @@ def batched_section(runner, dev, steps: int) -> dict:
         t_graph = _events_time(lambda: b.launch_graph(s), steps, s)
         t_persist = _events_time(lambda: b.forward(s), steps, s)
         t_sep = _events_time(lambda: b.forward_separate(s), steps, s)
+        # plain separate launches with the library's default (untuned) configuration choice
+        t_plain = _events_time(lambda: [cv.forward(-1, s) for cv in convs], steps, s)
         b.launch_graph(s)
         s.synchronize()
         worst = 0.0
@@ -178,9 +180,11 @@ def batched_section(runner, dev, steps: int) -> dict:
     return {"layers": len(shapes), "direct_gflop": round(flops / 1e9, 2),
             "us_graph": round(t_graph * 1e3, 2), "us_persistent": round(t_persist * 1e3, 2),
             "us_separate": round(t_sep * 1e3, 2),
+            "us_separate_default_configs": round(t_plain * 1e3, 2),
             "tflops_graph": round(flops / (t_graph * 1e-3) / 1e12, 2),
             "tflops_separate": round(flops / (t_sep * 1e-3) / 1e12, 2),
             "graph_over_separate": round(t_sep / t_graph, 3),
+            "graph_over_plain_default": round(t_plain / t_graph, 3),
             "layer_configs": cfgs,
             "max_rel_worst_layer": worst}
 
