@@ -20,6 +20,7 @@ DEMO_PATH = os.path.join(PKG_DIR, "lib", "conv2dfp16demo")
 
 # error codes (include/conv2d.h)
 OK, ERR_NULL, ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_WORKSPACE, ERR_HIP, ERR_INVALID = range(7)
+METHOD_WINOGRAD, METHOD_DIRECT = 0, 1
 ERROR_NAMES = {
     ERR_NULL: "CONV2D_ERR_NULL",
     ERR_UNSUPPORTED: "CONV2D_ERR_UNSUPPORTED",
@@ -72,7 +73,7 @@ class ParamType(ctypes.Structure):
                [(f, ctypes.c_short) for f in
                 ("n", "c", "h", "w", "k", "r", "s", "u", "v", "p", "q", "Oh", "Ow")] + \
                [(f, ctypes.c_int) for f in
-                ("tiles_h", "tiles_w", "T", "NT", "G32", "CB", "KB32", "band", "cfg", "flags")]
+                ("tiles_h", "tiles_w", "T", "NT", "G32", "CB", "KB32", "band", "cfg", "flags", "nsteps")]
 
 
 _lib = None
@@ -92,6 +93,9 @@ _SIGS = {
     "conv2d_tile_config": ([ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_select_tile_config": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_direct_f64": ([_P(Problem), ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_method": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
+    "conv2d_direct_num_tile_configs": ([], ctypes.c_int),
+    "conv2d_direct_tile_config": ([ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_fp8_workspace_size": ([_P(Problem), _P(ctypes.c_size_t), _P(ctypes.c_size_t)], ctypes.c_int),
     "conv2d_fp8_prepare_weights": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
     "conv2d_fp8_input_transform": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),

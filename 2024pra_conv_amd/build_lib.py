@@ -23,6 +23,7 @@ LIB_SOURCES = [
     "wino_transforms.hip",
     "wino_gemm.hip",
     "wino_fp8.hip",
+    "direct_pack.hip",
     "direct_f64.hip",
     "conv2d_plugin.hip",
     "tuner.hip",

@@ -57,14 +57,31 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         if (b > 1 && b > g->tiles_h) continue;
         if (input_lds_bytes(b, g->tiles_w, (int)pb->w, (int)pb->q) <= 65536) g->band = b;
     }
+    g->nsteps = 16 * g->CB;
+    g->direct = !(g->winograd_ok && g->band);
+    if (g->direct) {
+        // general-geometry path: GEMM columns are output pixels, steps run over
+        // (filter tap, 32-channel chunk) pairs, padded with zero steps to a multiple of 16
+        g->tiles_h = g->Oh;
+        g->tiles_w = g->Ow;
+        g->T = (long long)g->Oh * g->Ow;
+        g->NT = g->T * pb->n;
+        if (g->NT > (1LL << 30)) return CONV2D_ERR_OVERFLOW;
+        g->G32 = (g->NT + 31) / 32;
+        const long long steps = ((long long)pb->r * pb->s * g->CB + 15) / 16 * 16;
+        if (steps * 2 * (g->G32 + g->KB32) * kFragHalves * 2 > (1LL << 40)) return CONV2D_ERR_OVERFLOW;
+        g->nsteps = (int)steps;
+        g->band = 0;
+        g->v_bytes = (size_t)steps * 2 * (size_t)g->G32 * kFragHalves * 2;
+        g->u_bytes = (size_t)steps * 2 * (size_t)g->KB32 * kFragHalves * 2;
+    }
     return CONV2D_OK;
 }
 
 int require_winograd(const problem_t* pb, Geometry* g) {
     int rc = derive_geometry(pb, g);
     if (rc) return rc;
-    if (!g->winograd_ok) return CONV2D_ERR_UNSUPPORTED;
-    if (!g->band) return CONV2D_ERR_UNSUPPORTED;  // image row too wide for LDS staging
+    if (g->direct) return CONV2D_ERR_UNSUPPORTED;
     return CONV2D_OK;
 }
 
@@ -96,7 +113,20 @@ int heuristic_config(const problem_t* pb, const Geometry& g) {
     return 11;
 }
 
+// General-geometry path: the 128x128 LDS-DMA tile when it gives every compute unit a
+// workgroup, 64x128 when it gives one per two, else 32x64.
+static int direct_heuristic_config(const Geometry& g) {
+    const long long cus = compute_units();
+    auto items = [&](int bk, int bt) {
+        return (long long)((g.KB32 + bk / 32 - 1) / (bk / 32)) * ((g.G32 + bt / 32 - 1) / (bt / 32));
+    };
+    if (items(128, 128) >= cus) return 0;
+    if (2 * items(64, 128) >= cus) return 1;
+    return 2;
+}
+
 int select_config(const problem_t* pb, const Geometry& g) {
+    if (g.direct) return direct_heuristic_config(g);
     int cfg = tuned_config_lookup(pb);
     if (cfg >= 0 && cfg < kNumGemmConfigs) return cfg;
     return heuristic_config(pb, g);
@@ -132,6 +162,7 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
     a->KB32 = g.KB32;
     a->band = g.band;
     a->cfg = cfg;
+    a->nsteps = g.nsteps;
     // widest input-row load the staging may use (rows start at multiples of w halves)
     const uintptr_t in_addr = (uintptr_t)pb->in;
     if (pb->w % 8 == 0 && (in_addr & 15) == 0)
@@ -146,8 +177,8 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
 
 Launch slot_filter(const Geometry& g) {
     Launch L;
-    L.kernel = (const void*)&wino_filter_transform;
-    const long long threads = (long long)g.KB32 * 2 * g.CB * 64;
+    L.kernel = g.direct ? (const void*)&direct_pack_weights : (const void*)&wino_filter_transform;
+    const long long threads = (long long)g.KB32 * 2 * (g.direct ? g.nsteps : g.CB) * 64;
     L.grid = dim3((unsigned)((threads + 255) / 256), 1, 1);
     L.block = dim3(256, 1, 1);
     L.lds = 0;
@@ -156,6 +187,14 @@ Launch slot_filter(const Geometry& g) {
 
 Launch slot_input(const problem_t* pb, const Geometry& g) {
     Launch L;
+    if (g.direct) {
+        L.kernel = (const void*)&direct_pack_input;
+        const long long threads = (long long)g.G32 * 2 * g.nsteps * 64;
+        L.grid = dim3((unsigned)((threads + 255) / 256), 1, 1);
+        L.block = dim3(256, 1, 1);
+        L.lds = 0;
+        return L;
+    }
     L.kernel = (const void*)&wino_input_transform;
     L.grid = dim3((unsigned)((g.tiles_h + g.band - 1) / g.band), (unsigned)g.CB, pb->n);
     L.block = dim3(256, 1, 1);
@@ -164,7 +203,7 @@ Launch slot_input(const problem_t* pb, const Geometry& g) {
 }
 
 Launch slot_gemm(const Geometry& g, int cfg) {
-    const GemmConfig& c = kGemmConfigs[cfg];
+    const GemmConfig& c = g.direct ? kDirectConfigs[cfg] : kGemmConfigs[cfg];
     Launch L;
     L.kernel = c.kernel;
     const int vf = c.bt / 32, uf = c.bk / 32;
@@ -222,7 +261,7 @@ int getkernelInfo(problem_t* problem, kernelInfo_t* ki, void* param) {
         !problem->transform_filter)
         return CONV2D_ERR_NULL;
     Geometry g;
-    int rc = require_winograd(problem, &g);
+    int rc = derive_geometry(problem, &g);
     if (rc) return rc;
     const int cfg = select_config(problem, g);
     fill_param(problem, g, cfg, (mykernelParamType*)param);
@@ -241,7 +280,7 @@ int conv2d_workspace_size(const problem_t* problem, size_t* pin_bytes, size_t* f
                           size_t* mid_bytes) {
     if (!problem) return CONV2D_ERR_NULL;
     Geometry g;
-    int rc = require_winograd(problem, &g);
+    int rc = derive_geometry(problem, &g);
     if (rc) return rc;
     if (pin_bytes) *pin_bytes = g.v_bytes;
     if (filter_bytes) *filter_bytes = g.u_bytes;
@@ -287,7 +326,7 @@ int conv2d_prepare_weights(const problem_t* pb, void* stream) {
     int rc = check_ptrs(pb, false, true, false, false, true);
     if (rc) return rc;
     Geometry g;
-    if ((rc = require_winograd(pb, &g))) return rc;
+    if ((rc = derive_geometry(pb, &g))) return rc;
     mykernelParamType a;
     fill_param(pb, g, 0, &a);
     return launch(slot_filter(g), &a, (hipStream_t)stream);
@@ -297,7 +336,7 @@ int conv2d_input_transform(const problem_t* pb, void* stream) {
     int rc = check_ptrs(pb, true, false, false, true, false);
     if (rc) return rc;
     Geometry g;
-    if ((rc = require_winograd(pb, &g))) return rc;
+    if ((rc = derive_geometry(pb, &g))) return rc;
     mykernelParamType a;
     fill_param(pb, g, 0, &a);
     return launch(slot_input(pb, g), &a, (hipStream_t)stream);
@@ -307,9 +346,9 @@ int conv2d_gemm(const problem_t* pb, int cfg, void* stream) {
     int rc = check_ptrs(pb, false, false, true, true, true);
     if (rc) return rc;
     Geometry g;
-    if ((rc = require_winograd(pb, &g))) return rc;
+    if ((rc = derive_geometry(pb, &g))) return rc;
     if (cfg < 0) cfg = select_config(pb, g);
-    if (cfg >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
+    if (cfg >= (g.direct ? kNumDirectConfigs : kNumGemmConfigs)) return CONV2D_ERR_INVALID;
     mykernelParamType a;
     fill_param(pb, g, cfg, &a);
     return launch(slot_gemm(g, cfg), &a, (hipStream_t)stream);
@@ -319,9 +358,9 @@ int conv2d_forward(const problem_t* pb, int cfg, void* stream) {
     int rc = check_ptrs(pb, true, false, true, true, true);
     if (rc) return rc;
     Geometry g;
-    if ((rc = require_winograd(pb, &g))) return rc;
+    if ((rc = derive_geometry(pb, &g))) return rc;
     if (cfg < 0) cfg = select_config(pb, g);
-    if (cfg >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
+    if (cfg >= (g.direct ? kNumDirectConfigs : kNumGemmConfigs)) return CONV2D_ERR_INVALID;
     mykernelParamType a;
     fill_param(pb, g, cfg, &a);
     if ((rc = launch(slot_input(pb, g), &a, (hipStream_t)stream))) return rc;
@@ -341,9 +380,28 @@ int conv2d_tile_config(int cfg, int* bk, int* bt, int* threads) {
 int conv2d_select_tile_config(const problem_t* pb, int* cfg) {
     if (!pb || !cfg) return CONV2D_ERR_NULL;
     Geometry g;
-    int rc = require_winograd(pb, &g);
+    int rc = derive_geometry(pb, &g);
     if (rc) return rc;
     *cfg = select_config(pb, g);
+    return CONV2D_OK;
+}
+
+int conv2d_method(const problem_t* pb, int* method) {
+    if (!pb || !method) return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = derive_geometry(pb, &g);
+    if (rc) return rc;
+    *method = g.direct ? CONV2D_METHOD_DIRECT : CONV2D_METHOD_WINOGRAD;
+    return CONV2D_OK;
+}
+
+int conv2d_direct_num_tile_configs(void) { return kNumDirectConfigs; }
+
+int conv2d_direct_tile_config(int cfg, int* bk, int* bt, int* threads) {
+    if (cfg < 0 || cfg >= kNumDirectConfigs) return CONV2D_ERR_INVALID;
+    if (bk) *bk = kDirectConfigs[cfg].bk;
+    if (bt) *bt = kDirectConfigs[cfg].bt;
+    if (threads) *threads = kDirectConfigs[cfg].threads;
     return CONV2D_OK;
 }
 

@@ -11,11 +11,13 @@ namespace wino {
 
 struct Geometry {
     int Oh, Ow;
-    bool winograd_ok;
-    int tiles_h, tiles_w;
-    long long T, NT, G32;  // G32: 32-column blocks of V
+    bool winograd_ok;      // 3x3, stride 1, pad <= 2
+    bool direct;           // runs on the general-geometry (implicit-GEMM direct) path
+    int tiles_h, tiles_w;  // Winograd: 2x2 output tiles; direct: Oh, Ow
+    long long T, NT, G32;  // GEMM columns per image / in total, 32-column blocks of V
     int CB, KB32;          // CB: 32-channel blocks; KB32: 32-row blocks of U
     int band;
+    int nsteps;            // GEMM steps (32-channel chunks): 16 * CB, or the padded tap count
     size_t v_bytes, u_bytes;
 };
 
@@ -26,7 +28,9 @@ struct Launch {
     unsigned lds;
 };
 
+// Validates a problem and derives its geometry (Winograd or general-geometry path).
 int derive_geometry(const problem_t* pb, Geometry* g);
+// As derive_geometry, but only for problems on the Winograd path (else CONV2D_ERR_UNSUPPORTED).
 int require_winograd(const problem_t* pb, Geometry* g);
 int heuristic_config(const problem_t* pb, const Geometry& g);
 int select_config(const problem_t* pb, const Geometry& g);

@@ -29,6 +29,9 @@ __global__ void wino_filter_transform(mykernelParamType prm);
 __global__ void wino_input_transform(mykernelParamType prm);
 __global__ void wino_noop(mykernelParamType prm);
 __global__ void conv_direct_f64_kernel(mykernelParamType prm, double* out);
+// general-geometry path: packing kernels of the implicit-GEMM direct convolution
+__global__ void direct_pack_weights(mykernelParamType prm);
+__global__ void direct_pack_input(mykernelParamType prm);
 
 // Per-item bodies shared by the per-layer kernels and the multi-layer persistent kernels.
 __device__ void input_transform_block(const mykernelParamType& prm, int n, int cb, int bi, half_t* lds);

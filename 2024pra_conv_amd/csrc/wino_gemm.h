@@ -12,3 +12,7 @@ struct GemmConfig {
 
 extern const GemmConfig kGemmConfigs[];
 extern const int kNumGemmConfigs;
+
+// Implicit-GEMM direct convolution (general-geometry path): same fields, no grouped kernels.
+extern const GemmConfig kDirectConfigs[];
+extern const int kNumDirectConfigs;

@@ -56,7 +56,11 @@ struct GemmShape {
 // One GEMM work item (k-block kblk x column block gblk). smem holds 2 stages. Ends after
 // the epilogue stores; callers reusing smem for another item must __syncthreads() first.
 // Stage layout: substep kk = 0..KS-1, each [UF U fragments][VF V fragments].
-template <int BK, int BT, int WK, int WT, int D>
+// DIRECT = false: the Winograd GEMM (16 positions, fold per position, 2x2-tile epilogue).
+// DIRECT = true: the implicit-GEMM direct convolution of the general-geometry path: the
+// steps run over prm.nsteps (filter tap, channel chunk) pairs into one accumulator, whose
+// elements are the outputs themselves.
+template <int BK, int BT, int WK, int WT, int D, bool DIRECT = false>
 __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
     using S = GemmShape<BK, BT, WK, WT>;
     constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF, NL = S::kLoads;
@@ -70,7 +74,7 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
     const int gb0 = gblk * VF;  // first 32-column block of V
     const int kb0 = kblk * UF;  // first 32-row block of U
-    const int nsteps = 16 * CB;  // (position, 32-channel chunk) pairs
+    const int nsteps = DIRECT ? prm.nsteps : 16 * CB;  // (position or tap, 32-channel chunk) pairs
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
 
@@ -151,7 +155,7 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     gload(ring[1 % D], 1 + D);
     __syncthreads();
 
-    int pos = 0, cb = 0;
+    [[maybe_unused]] int pos = 0, cb = 0;
     for (int s0 = 0; s0 < nsteps; s0 += D) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -169,21 +173,26 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
             lstore(ring[(d + 2) % D], cur);
             gload(ring[(d + 2) % D], s + 2 + D);
             __syncthreads();
-            if (++cb == CB) {
-                fold_position(pos, acc, Y);
+            if constexpr (!DIRECT) {
+                if (++cb == CB) {
+                    fold_position(pos, acc, Y);
 #pragma unroll
-                for (int mi = 0; mi < MI; ++mi)
+                    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-                    for (int ni = 0; ni < NI; ++ni)
+                        for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
-                        for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
-                cb = 0;
-                ++pos;
+                            for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+                    cb = 0;
+                    ++pos;
+                }
             }
         }
     }
 
-    store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
+    if constexpr (DIRECT)
+        store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
+    else
+        store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -236,7 +245,7 @@ struct DmaShape {
 
 }  // namespace
 
-template <int BK, int BT, int WK, int WT, int S>
+template <int BK, int BT, int WK, int WT, int S, bool DIRECT = false>
 __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
     using Sh = DmaShape<BK, BT, WK, WT, S>;
     constexpr int MI = Sh::MI, NI = Sh::NI, UF = Sh::UF, VF = Sh::VF, FPW = Sh::kFPW;
@@ -248,7 +257,7 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
     const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
     const int gb0 = gblk * VF;
     const int kb0 = kblk * UF;
-    const int nsteps = 16 * CB;
+    const int nsteps = DIRECT ? prm.nsteps : 16 * CB;
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
 
@@ -318,7 +327,7 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
     __syncthreads();
     lread(ra[0], rb[0], 0);
 
-    int pos = 0, cb = 0;
+    [[maybe_unused]] int pos = 0, cb = 0;
     for (int s0 = 0; s0 < nsteps; s0 += S) {
 #pragma unroll
         for (int d = 0; d < S; ++d) {
@@ -336,22 +345,27 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
                     for (int ni = 0; ni < NI; ++ni)
                         acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[cur][kk][mi], rb[cur][kk][ni],
                                                                              acc[mi][ni], 0, 0, 0);
-            if (++cb == CB) {
-                fold_position(pos, acc, Y);
+            if constexpr (!DIRECT) {
+                if (++cb == CB) {
+                    fold_position(pos, acc, Y);
 #pragma unroll
-                for (int mi = 0; mi < MI; ++mi)
+                    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-                    for (int ni = 0; ni < NI; ++ni)
+                        for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
-                        for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
-                cb = 0;
-                ++pos;
+                            for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+                    cb = 0;
+                    ++pos;
+                }
             }
         }
     }
     wait_vm<0>();  // the trailing (re-read) DMAs land before the workgroup's LDS is released
 
-    store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
+    if constexpr (DIRECT)
+        store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
+    else
+        store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
 
 // Waves per SIMD the register allocation must allow: two for 64x32 wave tiles (so two waves
@@ -460,3 +474,44 @@ const GemmConfig kGemmConfigs[] = {
     WINO_CFG(32, 128, 1, 4, 4),       // 15: wave 32x32, 4 waves, ring depth 4
 };
 const int kNumGemmConfigs = (int)(sizeof(kGemmConfigs) / sizeof(kGemmConfigs[0]));
+
+// ---- direct (implicit-GEMM) form of the general-geometry path --------------------------------
+// The same two pipeline bodies with DIRECT = true: U holds the packed filter taps, V the
+// packed im2col input, and the steps run over (tap, channel chunk) pairs.
+
+template <int BK, int BT, int WK, int WT, int D>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
+wino_direct_kernel(mykernelParamType prm) {
+    using S = GemmShape<BK, BT, WK, WT>;
+    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
+    const int nk = (prm.KB32 + S::UF - 1) / S::UF;
+    const int ng = (prm.G32 + S::VF - 1) / S::VF;
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
+    gemm_body<BK, BT, WK, WT, D, true>(prm, kblk, gblk, smem);
+}
+
+template <int BK, int BT, int WK, int WT, int S>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
+wino_direct_dma_kernel(mykernelParamType prm) {
+    using Sh = DmaShape<BK, BT, WK, WT, S>;
+    __shared__ __attribute__((aligned(16))) half_t smem[Sh::kSmemHalves];
+    const int nk = (prm.KB32 + Sh::UF - 1) / Sh::UF;
+    const int ng = (prm.G32 + Sh::VF - 1) / Sh::VF;
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
+    gemm_dma_body<BK, BT, WK, WT, S, true>(prm, kblk, gblk, smem);
+}
+
+#define DIRECT_CFG(BK, BT, WK, WT, D) \
+    { BK, BT, WK * WT * 64, D, (const void*)&wino_direct_kernel<BK, BT, WK, WT, D>, nullptr }
+#define DIRECT_DMA_CFG(BK, BT, WK, WT, S) \
+    { BK, BT, WK * WT * 64, S, (const void*)&wino_direct_dma_kernel<BK, BT, WK, WT, S>, nullptr }
+
+const GemmConfig kDirectConfigs[] = {
+    DIRECT_DMA_CFG(128, 128, 2, 4, 8),  // 0: LDS-DMA, 8-stage ring, wave 64x32, 8 waves
+    DIRECT_CFG(64, 128, 1, 4, 2),       // 1: wave 64x32, 4 waves
+    DIRECT_CFG(32, 64, 1, 2, 2),        // 2: wave 32x32, 2 waves (small problems)
+    DIRECT_DMA_CFG(128, 64, 2, 2, 4),   // 3: LDS-DMA, 4-stage ring, wave 64x32, 4 waves
+};
+const int kNumDirectConfigs = (int)(sizeof(kDirectConfigs) / sizeof(kDirectConfigs[0]));

@@ -200,6 +200,30 @@ __device__ inline void store_tiles32(const mykernelParamType& prm, const floatx1
     }
 }
 
+// Epilogue of the direct (implicit-GEMM) form: GEMM column g is output pixel g of the batch
+// (image g / T, pixel g % T with T = Oh*Ow), so each accumulator element is one output
+// value, rounded once to FP16. Consecutive lanes hold consecutive pixels of one output
+// channel, so the 2-byte stores of a row coalesce.
+template <int MI, int NI>
+__device__ inline void store_plain32(const mykernelParamType& prm, const floatx16 (&acc)[MI][NI], int gbw, int kbw,
+                                     int lane) {
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+        const int g = (gbw + ni) * 32 + (lane & 31);
+        if (g >= prm.NT) continue;
+        const int img = g / prm.T;
+        const int pix = g - img * prm.T;
+        half_t* o = (half_t*)prm.pout + (size_t)img * prm.k * prm.T + pix;
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int k = (kbw + mi) * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                if (k < prm.k) o[(size_t)k * prm.T] = (half_t)acc[mi][ni][j];
+            }
+    }
+}
+
 // Hardware dispatches workgroup ids round-robin over the 8 XCDs; this remaps id L so every
 // XCD gets a contiguous run of the (k-block fastest) work list, keeping the k-blocks that
 // share one column block of V adjacent on one XCD (same L2).

@@ -19,11 +19,13 @@ def _stream_handle(stream: torch.cuda.Stream | None):
 class WinogradConv:
     """One convolution problem with its own output and workspaces.
 
-    x: (N, C, H, W) float16 on cuda; w: (K, C, 3, 3) float16 on cuda.
+    x: (N, C, H, W) float16 on cuda; w: (K, C, R, S) float16 on cuda. 3x3 stride-1 problems
+    with pad <= 2 run on the Winograd path, any other geometry on the direct path (the
+    library decides; `method()` reports which).
     """
 
     def __init__(self, x: torch.Tensor, w: torch.Tensor, pad=(1, 1), out: torch.Tensor | None = None,
-                 workspace: tuple[torch.Tensor, torch.Tensor] | None = None):
+                 workspace: tuple[torch.Tensor, torch.Tensor] | None = None, stride=(1, 1)):
         assert x.dtype == torch.float16 and w.dtype == torch.float16
         assert x.is_cuda and w.is_cuda and x.is_contiguous() and w.is_contiguous()
         self.L = load_library()
@@ -32,9 +34,9 @@ class WinogradConv:
         assert c == c2
         self.x, self.w = x, w
         self.pad = pad
-        self.shape = (n, c, h, wd, k, r, s, 1, 1, pad[0], pad[1])
-        self.oh = (h - r + 2 * pad[0]) + 1
-        self.ow = (wd - s + 2 * pad[1]) + 1
+        self.shape = (n, c, h, wd, k, r, s, stride[0], stride[1], pad[0], pad[1])
+        self.oh = (h - r + 2 * pad[0]) // stride[0] + 1
+        self.ow = (wd - s + 2 * pad[1]) // stride[1] + 1
         probe = make_problem(*self.shape)
         vb, ub, _ = workspace_size(probe)
         self.v_bytes, self.u_bytes = vb, ub
@@ -97,6 +99,11 @@ class WinogradConv:
         c = ctypes.c_int()
         check(self.L.conv2d_select_tile_config(ctypes.byref(self.pb), ctypes.byref(c)), "conv2d_select_tile_config")
         return c.value
+
+    def method(self) -> int:
+        m = ctypes.c_int()
+        check(self.L.conv2d_method(ctypes.byref(self.pb), ctypes.byref(m)), "conv2d_method")
+        return m.value
 
     def direct_f64(self, stream=None) -> torch.Tensor:
         ref = torch.empty(self.out.shape, dtype=torch.float64, device=self.x.device)

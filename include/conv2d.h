@@ -39,11 +39,18 @@ extern "C" {
 enum {
     CONV2D_OK = 0,
     CONV2D_ERR_NULL = 1,        /* a required pointer is NULL */
-    CONV2D_ERR_UNSUPPORTED = 2, /* geometry the Winograd path does not implement */
+    CONV2D_ERR_UNSUPPORTED = 2, /* geometry the requested form does not implement (the FP8,
+                                   multi-layer and autotuning forms are Winograd-only) */
     CONV2D_ERR_OVERFLOW = 3,    /* a dimension does not fit the kernel argument (short) */
     CONV2D_ERR_WORKSPACE = 4,   /* caller-declared workspace smaller than required */
     CONV2D_ERR_HIP = 5,         /* a HIP runtime call failed */
     CONV2D_ERR_INVALID = 6      /* bad argument (zero dimension, unknown config, ...) */
+};
+
+/* Method a problem runs on (conv2d_method). */
+enum {
+    CONV2D_METHOD_WINOGRAD = 0, /* 3x3, stride 1, pad <= 2: the four-stage Winograd path */
+    CONV2D_METHOD_DIRECT = 1    /* any other geometry: implicit-GEMM direct convolution */
 };
 
 /* Problem descriptor. Same layout as the reference's problem_t (conv2d.h:8-27). The three
@@ -149,8 +156,20 @@ int conv2d_num_tile_configs(void);
 /* Describes configuration cfg: rows of k (bk) and columns of tiles (bt) per workgroup,
  * threads per workgroup. */
 int conv2d_tile_config(int cfg, int* bk, int* bt, int* threads);
-/* The configuration getkernelInfo would select for this problem. */
+/* The configuration getkernelInfo would select for this problem (an index into the
+ * Winograd table, or into the direct table for problems on the direct path). */
 int conv2d_select_tile_config(const problem_t* problem, int* cfg);
+
+/* General-geometry path (no counterpart in the reference, which accepts r, s, u, v, p, q
+ * in problem_t, conv2d.h:21-26, but ignores them, conv2d.cpp:254-271). Every valid problem
+ * that the Winograd path does not take (filter not 3x3, stride != 1, pad > 2, rows too wide
+ * for its LDS staging) runs as an implicit-GEMM direct convolution on the same MFMA
+ * pipeline: slot 1 packs the filter taps, slot 2 the im2col input, slot 3 is the GEMM,
+ * slot 4 the empty kernel; conv2d_workspace_size, conv2d_prepare_weights,
+ * conv2d_input_transform, conv2d_gemm and conv2d_forward serve both paths. */
+int conv2d_method(const problem_t* problem, int* method);
+int conv2d_direct_num_tile_configs(void);
+int conv2d_direct_tile_config(int cfg, int* bk, int* bt, int* threads);
 
 /* Validation-only FP64 direct convolution on the GPU (any r, s, u, v, p, q): every
  * product of two FP16 values is formed exactly in FP64 and summed in FP64, written as

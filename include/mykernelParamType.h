@@ -33,6 +33,9 @@ typedef struct mykernelParamType {
     int cfg;     /* GEMM tile configuration index */
     int flags;   /* input-row load width: bit 0 16 B (w % 8 == 0, 16-byte aligned base),
                     bit 1 8 B (w % 4 == 0), bit 2 4 B (w % 2 == 0); none: 2 B */
+    int nsteps;  /* GEMM steps of 32 channels: 16 * CB (Winograd positions) or, on the
+                    general-geometry direct path, r * s * CB rounded up to a multiple of 16
+                    (the padding steps carry zeros) */
 } mykernelParamType;
 
 /*
@@ -50,6 +53,16 @@ typedef struct mykernelParamType {
  *                         Channels c >= C and columns g >= NT are zero.
  * Sizes (2 * CB 16-channel blocks): U = 16 * 2CB * KB32 * 512 halves,
  * V = 16 * 2CB * G32 * 512 halves.
+ *
+ * General-geometry (direct) path, any r, s, u, v, p, q: the same fragment layouts with the
+ * Winograd position replaced by the GEMM step st = (r_i * s + s_i) * CB + cb (filter tap
+ * outer, 32-channel chunk inner; linear 16-channel index 2 * st + c % 32 / 16), padded with
+ * zero steps to nsteps:
+ *   U (transform_filter): w[k][c][r_i][s_i], rows k, same lanes as above;
+ *   V (transform_pin):    the im2col input x[n][c][oh*u - p + r_i][ow*v - q + s_i] (zero
+ *                         outside the image), column g = n * Oh * Ow + oh * Ow + ow;
+ * with T = Oh * Ow, NT = n * T, G32 = ceil(NT / 32). Sizes: U = nsteps * 2 * KB32 * 512
+ * halves, V = nsteps * 2 * G32 * 512 halves.
  */
 
 #endif /* MYKERNELPARAMTYPE_MI355X_H_ */

@@ -97,8 +97,8 @@ def test_getkernelinfo_fills_slots_and_param(wc):
 
 
 @pytest.mark.parametrize("change,code", [
-    ({"r": 5, "s": 5}, "ERR_UNSUPPORTED"), ({"u": 2, "v": 2}, "ERR_UNSUPPORTED"), ({"p": 3}, "ERR_UNSUPPORTED"),
     ({"n": 0}, "ERR_INVALID"), ({"c": 40000}, "ERR_OVERFLOW"), ({"h": 1, "p": 0}, "ERR_INVALID"),
+    ({"r": 0}, "ERR_INVALID"), ({"u": 0}, "ERR_INVALID"), ({"h": 2, "r": 5, "p": 1}, "ERR_INVALID"),
 ])
 def test_getkernelinfo_rejects(wc, change, code):
     args = dict(n=1, c=16, h=8, w=8, k=16, r=3, s=3, u=1, v=1, p=1, q=1)
@@ -106,6 +106,52 @@ def test_getkernelinfo_rejects(wc, change, code):
     pb = wc.make_problem(**args, ptrs=FAKE)
     rc, _, _ = _ki(wc, pb)
     assert rc == getattr(wc, code)
+
+
+@pytest.mark.parametrize("geo", [
+    dict(r=5, s=5, p=2, q=2), dict(u=2, v=2), dict(p=3, q=3), dict(r=1, s=1, p=0, q=0),
+    dict(r=7, s=7, u=2, v=2, p=3, q=3), dict(r=3, s=1, p=1, q=0), dict(r=2, s=2, u=2, v=2, p=0, q=0),
+])
+def test_general_geometry_runs_on_the_direct_path(wc, geo):
+    """Geometry the Winograd path does not take (the reference ignored r, s, u, v, p, q and
+    computed garbage) is accepted and set up as an implicit-GEMM direct convolution."""
+    L = wc.load_library()
+    args = dict(n=2, c=40, h=17, w=13, k=24, r=3, s=3, u=1, v=1, p=1, q=1)
+    args.update(geo)
+    pb = wc.make_problem(**args, ptrs=FAKE)
+    m = ctypes.c_int()
+    assert L.conv2d_method(ctypes.byref(pb), ctypes.byref(m)) == wc.OK and m.value == wc.METHOD_DIRECT
+    rc, ki, prm = _ki(wc, pb)
+    assert rc == wc.OK
+    oh = (args["h"] - args["r"] + 2 * args["p"]) // args["u"] + 1
+    ow = (args["w"] - args["s"] + 2 * args["q"]) // args["v"] + 1
+    cb = -(-args["c"] // 32)
+    P = args["n"] * oh * ow
+    nsteps = -(-(args["r"] * args["s"] * cb) // 16) * 16
+    assert (prm.Oh, prm.Ow, prm.T, prm.NT, prm.G32, prm.CB, prm.nsteps) == (oh, ow, oh * ow, P, -(-P // 32), cb, nsteps)
+    assert prm.KB32 == -(-args["k"] // 32)
+    vb, ub, mb = wc.workspace_size(pb)
+    assert vb == nsteps * 2 * prm.G32 * 1024 and ub == nsteps * 2 * prm.KB32 * 1024 and mb == 0
+    assert all(ki.slot(i)["kernel"] for i in range(1, 5))
+    assert ki.slot(1)["grid"][0] * 256 >= nsteps * 2 * prm.KB32 * 64
+    assert ki.slot(2)["grid"][0] * 256 >= nsteps * 2 * prm.G32 * 64
+    ncfg = L.conv2d_direct_num_tile_configs()
+    assert 0 <= prm.cfg < ncfg
+    bk, bt, th = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert L.conv2d_direct_tile_config(prm.cfg, ctypes.byref(bk), ctypes.byref(bt), ctypes.byref(th)) == wc.OK
+    assert ki.slot(3)["grid"] == (-(-prm.G32 // (bt.value // 32)) * -(-prm.KB32 // (bk.value // 32)), 1, 1)
+    assert ki.slot(3)["block"][0] == th.value
+    # the Winograd-only forms refuse it
+    a, b = ctypes.c_size_t(), ctypes.c_size_t()
+    assert L.conv2d_fp8_workspace_size(ctypes.byref(pb), ctypes.byref(a), ctypes.byref(b)) == wc.ERR_UNSUPPORTED
+
+
+def test_winograd_geometry_reports_winograd(wc):
+    L = wc.load_library()
+    m = ctypes.c_int()
+    for p in (0, 1, 2):
+        pb = wc.make_problem(1, 16, 9, 9, 16, p=p, q=p)
+        assert L.conv2d_method(ctypes.byref(pb), ctypes.byref(m)) == wc.OK and m.value == wc.METHOD_WINOGRAD
 
 
 def test_getkernelinfo_null_pointers(wc):

@@ -241,3 +241,57 @@ def test_error_metrics_definitions():
     assert abs(m["rms_rel"] - 0.5 / np.sqrt(21.0)) < 1e-15
     # the reference's elementwise criterion (main.cpp:144) flags > 1% with the GPU value as denominator
     assert wg.reference_criterion_failures(np.array([1.0, 1.02], dtype=np.float16), np.array([1.0, 1.0])) == 1
+
+
+# ---- general-geometry (implicit-GEMM direct) path ------------------------------------------
+from oracle import direct as dg  # noqa: E402
+
+DIRECT_GEOS = [
+    dg.DirectGeometry(2, 5, 9, 8, 7, 5, 5, 1, 1, 2, 2),     # 5x5 pad 2
+    dg.DirectGeometry(1, 40, 11, 13, 33, 3, 3, 2, 2, 1, 1),  # stride 2, C and K ragged
+    dg.DirectGeometry(2, 3, 15, 15, 8, 7, 7, 2, 2, 3, 3),    # stem-like 7x7 stride 2
+    dg.DirectGeometry(1, 16, 6, 7, 16, 1, 1, 1, 1, 0, 0),    # 1x1
+    dg.DirectGeometry(1, 8, 7, 5, 4, 3, 1, 1, 1, 1, 0),      # 3x1
+    dg.DirectGeometry(1, 8, 8, 8, 8, 3, 3, 1, 1, 3, 3),      # pad 3 (> 2)
+    dg.DirectGeometry(1, 70, 6, 6, 5, 2, 2, 2, 2, 0, 0),     # 2x2 stride 2, three channel chunks
+]
+
+
+@pytest.mark.parametrize("geo", DIRECT_GEOS, ids=lambda g: f"{g.r}x{g.s}s{g.u}p{g.p}c{g.c}")
+def test_direct_packing_reproduces_the_direct_conv(geo):
+    """The implicit GEMM over the packed operands (filter taps x im2col) is the convolution:
+    exact on integer-valued data against the C direct-convolution oracle."""
+    rng = np.random.default_rng(geo.c + 7 * geo.r)
+    x = rng.integers(-3, 4, (geo.n, geo.c, geo.h, geo.w)).astype(np.float16)
+    w = rng.integers(-2, 3, (geo.k, geo.c, geo.r, geo.s)).astype(np.float16)
+    y = dg.gemm_output_f64(w, x, geo)
+    ref = cdirect.conv2d_f64(x, w, stride=(geo.u, geo.v), pad=(geo.p, geo.q))
+    assert y.shape == ref.shape and np.array_equal(y, ref)
+
+
+def test_direct_padding_is_positive_zero():
+    """Taps outside the image are +0 (the kernel's zero), also next to negative inputs."""
+    geo = DIRECT_GEOS[0]
+    x = -np.ones((geo.n, geo.c, geo.h, geo.w), dtype=np.float16)
+    px = dg.pack_input(x, geo).view(np.uint16)
+    assert not np.any(px == 0x8000)
+
+
+def test_direct_packed_offsets():
+    geo = DIRECT_GEOS[1]
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((geo.n, geo.c, geo.h, geo.w)).astype(np.float16)
+    w = rng.standard_normal((geo.k, geo.c, geo.r, geo.s)).astype(np.float16)
+    pw, px = dg.pack_weights(w, geo), dg.pack_input(x, geo)
+    assert pw.size == geo.nsteps * 2 * geo.KB32 * 512 and px.size == geo.nsteps * 2 * geo.P32 * 512
+    for (k, c, ri, si) in [(0, 0, 0, 0), (32, 39, 2, 1), (17, 33, 1, 2)]:
+        st = (ri * geo.s + si) * geo.CB + c // 32
+        L, lane = 2 * st + (c % 32) // 16, (c % 16 // 8) * 32 + k % 32
+        assert pw[(L * geo.KB32 + k // 32) * 512 + lane * 8 + c % 8] == w[k, c, ri, si]
+    for (n, c, oh, ow, ri, si) in [(0, 5, 2, 3, 1, 1), (0, 35, 5, 6, 2, 2), (0, 0, 0, 0, 0, 0)]:
+        st = (ri * geo.s + si) * geo.CB + c // 32
+        g = (n * geo.oh + oh) * geo.ow + ow
+        L, lane = 2 * st + (c % 32) // 16, (c % 16 // 8) * 32 + g % 32
+        ih, iw = oh * geo.u - geo.p + ri, ow * geo.v - geo.q + si
+        want = x[n, c, ih, iw] if (0 <= ih < geo.h and 0 <= iw < geo.w) else 0
+        assert px[(L * geo.P32 + g // 32) * 512 + lane * 8 + c % 8] == want
