@@ -60,15 +60,17 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
     g->nsteps = 16 * g->CB;
     g->direct = !(g->winograd_ok && g->band);
     if (g->direct) {
-        // general-geometry path: GEMM columns are output pixels, steps run over
-        // (filter tap, 32-channel chunk) pairs, padded with zero steps to a multiple of 16
+        // general-geometry path: GEMM columns are output pixels, steps run over 32 values
+        // of the flattened reduction index j = tap * C + c, padded with zero steps to a
+        // multiple of 8 (the deepest ring of the direct configurations)
         g->tiles_h = g->Oh;
         g->tiles_w = g->Ow;
         g->T = (long long)g->Oh * g->Ow;
         g->NT = g->T * pb->n;
         if (g->NT > (1LL << 30)) return CONV2D_ERR_OVERFLOW;
         g->G32 = (g->NT + 31) / 32;
-        const long long steps = ((long long)pb->r * pb->s * g->CB + 15) / 16 * 16;
+        const long long jn = (long long)pb->r * pb->s * pb->c;
+        const long long steps = ((jn + 31) / 32 + 7) / 8 * 8;
         if (steps * 2 * (g->G32 + g->KB32) * kFragHalves * 2 > (1LL << 40)) return CONV2D_ERR_OVERFLOW;
         g->nsteps = (int)steps;
         g->band = 0;
@@ -113,14 +115,14 @@ int heuristic_config(const problem_t* pb, const Geometry& g) {
     return 11;
 }
 
-// General-geometry path: the 128x128 LDS-DMA tile when it gives every compute unit a
-// workgroup, 64x128 when it gives one per two, else 32x64.
+// General-geometry path: the 128x128 LDS-DMA tile for K >= 128 when it gives every compute
+// unit a workgroup, 64x128 when that gives one per two, else 32x64.
 static int direct_heuristic_config(const Geometry& g) {
     const long long cus = compute_units();
     auto items = [&](int bk, int bt) {
         return (long long)((g.KB32 + bk / 32 - 1) / (bk / 32)) * ((g.G32 + bt / 32 - 1) / (bt / 32));
     };
-    if (items(128, 128) >= cus) return 0;
+    if (g.KB32 >= 4 && items(128, 128) >= cus) return 0;
     if (2 * items(64, 128) >= cus) return 1;
     return 2;
 }

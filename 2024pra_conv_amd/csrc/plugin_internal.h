@@ -17,7 +17,7 @@ struct Geometry {
     long long T, NT, G32;  // GEMM columns per image / in total, 32-column blocks of V
     int CB, KB32;          // CB: 32-channel blocks; KB32: 32-row blocks of U
     int band;
-    int nsteps;            // GEMM steps (32-channel chunks): 16 * CB, or the padded tap count
+    int nsteps;            // GEMM steps of 32 reduction values: 16 * CB, or ceil(r*s*c/32) padded to 8
     size_t v_bytes, u_bytes;
 };
 

@@ -58,13 +58,13 @@ struct GemmShape {
 // Stage layout: substep kk = 0..KS-1, each [UF U fragments][VF V fragments].
 // DIRECT = false: the Winograd GEMM (16 positions, fold per position, 2x2-tile epilogue).
 // DIRECT = true: the implicit-GEMM direct convolution of the general-geometry path: the
-// steps run over prm.nsteps (filter tap, channel chunk) pairs into one accumulator, whose
-// elements are the outputs themselves.
+// steps run over prm.nsteps chunks of 32 values of the flattened (filter tap, channel)
+// reduction index into one accumulator, whose elements are the outputs themselves.
 template <int BK, int BT, int WK, int WT, int D, bool DIRECT = false>
 __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
     using S = GemmShape<BK, BT, WK, WT>;
     constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF, NL = S::kLoads;
-    static_assert(16 % D == 0 && D % 2 == 0, "ring depth: even, dividing the step count (a multiple of 16)");
+    static_assert(8 % D == 0 && D % 2 == 0, "ring depth: even, dividing the step count (a multiple of 8)");
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -74,7 +74,7 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
     const int gb0 = gblk * VF;  // first 32-column block of V
     const int kb0 = kblk * UF;  // first 32-row block of U
-    const int nsteps = DIRECT ? prm.nsteps : 16 * CB;  // (position or tap, 32-channel chunk) pairs
+    const int nsteps = DIRECT ? prm.nsteps : 16 * CB;  // 32-value reduction chunks
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
 
@@ -238,7 +238,7 @@ struct DmaShape {
     static constexpr int kStageHalves = kFrags * kFragHalves;
     static constexpr int kSmemHalves = S * kStageHalves;
     static_assert(kFrags % kWaves == 0, "fragments per step must split evenly over the waves");
-    static_assert(S >= 4 && 16 % S == 0, "stage count: a power of two dividing the step count");
+    static_assert(S >= 4 && 8 % S == 0, "stage count: a power of two dividing the step count (a multiple of 8)");
     static_assert((S - 2) * kFPW <= 63, "vmcnt range");
     static_assert(MI >= 1 && NI >= 1 && BK % (WK * 32) == 0 && BT % (WT * 32) == 0, "tile shape");
 };

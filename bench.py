@@ -189,6 +189,44 @@ def batched_section(runner, dev, steps: int) -> dict:
             "max_rel_worst_layer": worst}
 
 
+def direct_section(runner, dev, steps: int) -> dict:
+    """The general-geometry path (implicit-GEMM direct convolution) on three layer shapes the
+    Winograd path does not take; FP16 N(0,1) input, N(0, 2/(R*S*C)) filters."""
+    import torch
+    L = importlib.import_module("2024pra_conv_amd").load_library()
+    shapes = [("stem 7x7/2", 32, 3, 224, 64, 7, 2, 3), ("3x3/2", 32, 128, 56, 256, 3, 2, 1),
+              ("1x1", 32, 256, 56, 64, 1, 1, 0)]
+    out = {}
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20243)
+    for (name, n, c, h, k, r, st, p) in shapes:
+        x = torch.randn((n, c, h, h), generator=gen, device=dev).half()
+        wt = (torch.randn((k, c, r, r), generator=gen, device=dev) * (2.0 / (r * r * c)) ** 0.5).half()
+        conv = runner.WinogradConv(x, wt, pad=(p, p), stride=(st, st))
+        oh = conv.oh
+        flops = 2.0 * n * k * c * r * r * oh * oh
+        cfg = conv.selected_config()
+        conv.prepare_weights()
+        t_full = _events_time(lambda: (conv.prepare_weights(), conv.forward(cfg)), steps)
+        t_fwd = _events_time(lambda: conv.forward(cfg), steps)
+        t_gemm = _events_time(lambda: conv.gemm(cfg), steps)
+        per_cfg = {i: round(_events_time(lambda: conv.gemm(i), steps) * 1e3, 2)
+                   for i in range(L.conv2d_direct_num_tile_configs())}
+        ref = conv.direct_f64()
+        conv.forward(cfg)
+        torch.cuda.synchronize()
+        nsteps = conv.param().nsteps
+        out[name] = {"shape": f"N={n} C={c} H=W={h} K={k} {r}x{r} stride {st} pad {p}",
+                     "tflops_with_weights": round(flops / (t_full * 1e-3) / 1e12, 2),
+                     "tflops_steady": round(flops / (t_fwd * 1e-3) / 1e12, 2),
+                     "us_gemm": round(t_gemm * 1e3, 2), "tile_config": cfg, "gemm_us_per_config": per_cfg,
+                     "gemm_mfma_frac_useful": round(flops / (t_gemm * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 4),
+                     "reduction_values": f"{nsteps * 32} computed for {c * r * r} (zero padding to 8 steps of 32)",
+                     "error_vs_fp64": _err(conv.out, ref)}
+        del conv
+    return out
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "wino_gemm_n32_c256_h56_k256.md")
 
 
@@ -214,7 +252,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cfg", type=int, default=-1, help="GEMM tile configuration (-1 = library choice)")
-    ap.add_argument("--no-forms", action="store_true", help="skip the FP8 and batched sections")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP8, batched and direct sections")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before warm-up, run one device-to-device copy of this many MiB (untimed): the "
                          "known-size kernel that hardware byte counters are calibrated against")
@@ -326,6 +364,7 @@ def main() -> int:
         torch.cuda.empty_cache()
         line["fp8_form"] = fp8_section(runner, dev, max(5, args.steps // 5))
         line["batched_form"] = batched_section(runner, dev, max(5, args.steps // 5))
+        line["direct_form"] = direct_section(runner, dev, max(5, args.steps // 5))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(1, c, h, w, k)
         line["cpu_baseline"].pop("seconds", None)

@@ -33,9 +33,10 @@ typedef struct mykernelParamType {
     int cfg;     /* GEMM tile configuration index */
     int flags;   /* input-row load width: bit 0 16 B (w % 8 == 0, 16-byte aligned base),
                     bit 1 8 B (w % 4 == 0), bit 2 4 B (w % 2 == 0); none: 2 B */
-    int nsteps;  /* GEMM steps of 32 channels: 16 * CB (Winograd positions) or, on the
-                    general-geometry direct path, r * s * CB rounded up to a multiple of 16
-                    (the padding steps carry zeros) */
+    int nsteps;  /* GEMM steps of 32 reduction values: 16 * CB (Winograd positions x
+                    channel chunks) or, on the general-geometry direct path,
+                    ceil(r * s * c / 32) rounded up to a multiple of 8 (the padding carries
+                    zeros) */
 } mykernelParamType;
 
 /*
@@ -54,15 +55,16 @@ typedef struct mykernelParamType {
  * Sizes (2 * CB 16-channel blocks): U = 16 * 2CB * KB32 * 512 halves,
  * V = 16 * 2CB * G32 * 512 halves.
  *
- * General-geometry (direct) path, any r, s, u, v, p, q: the same fragment layouts with the
- * Winograd position replaced by the GEMM step st = (r_i * s + s_i) * CB + cb (filter tap
- * outer, 32-channel chunk inner; linear 16-channel index 2 * st + c % 32 / 16), padded with
- * zero steps to nsteps:
+ * General-geometry (direct) path, any r, s, u, v, p, q: the same fragment layouts with
+ * (pos, c) replaced by the flattened reduction index j = (r_i * s + s_i) * c_total + c
+ * (filter tap outer, channel inner; 16-value block j / 16, lane (j % 16 / 8) * 32 + row,
+ * element j % 8), zero for j >= r * s * c_total up to nsteps * 32:
  *   U (transform_filter): w[k][c][r_i][s_i], rows k, same lanes as above;
  *   V (transform_pin):    the im2col input x[n][c][oh*u - p + r_i][ow*v - q + s_i] (zero
  *                         outside the image), column g = n * Oh * Ow + oh * Ow + ow;
  * with T = Oh * Ow, NT = n * T, G32 = ceil(NT / 32). Sizes: U = nsteps * 2 * KB32 * 512
- * halves, V = nsteps * 2 * G32 * 512 halves.
+ * halves, V = nsteps * 2 * G32 * 512 halves. For c a multiple of 32 this is step
+ * st = tap * CB + cb, as in the Winograd layout with the position replaced by the tap.
  */
 
 #endif /* MYKERNELPARAMTYPE_MI355X_H_ */

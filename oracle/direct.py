@@ -5,11 +5,11 @@ No reference counterpart: the reference's plugin ignores r, s, u, v, p, q (conv2
 its CPU checker (include/verfiy.h:5-43) is the convolution this path must reproduce, restated
 in oracle/direct_conv.c. Here:
 
-  steps:  st = tap * CB + cb (tap = ri * s + si, cb = 32-channel chunk), padded with zero
-          steps to nsteps = ceil(r*s*CB / 16) * 16; 16-channel block index L = 2 * st + kk.
-  W:      fragment (L, kb): lane = (c % 16 // 8) * 32 + k % 32, element c % 8, value
-          w[k][c][ri][si] (zero for k >= K, c >= C, padding steps).
-  X:      fragment (L, pb): lane = (c % 16 // 8) * 32 + g % 32, element c % 8, value
+  j:      flattened reduction index j = tap * C + c (tap = ri * s + si), zero-padded to
+          nsteps * 32 with nsteps = ceil(ceil(r*s*C / 32) / 8) * 8; 16-value block L = j // 16.
+  W:      fragment (L, kb): lane = (j % 16 // 8) * 32 + k % 32, element j % 8, value
+          w[k][c][ri][si] (zero for k >= K and j >= r*s*C).
+  X:      fragment (L, pb): lane = (j % 16 // 8) * 32 + g % 32, element j % 8, value
           x[n][c][oh*u - p + ri][ow*v - q + si] (zero outside the image), g = n*Oh*Ow + oh*Ow + ow.
 
 Pinned by tests/test_oracle.py: the GEMM over the unpacked operands equals oracle/direct_conv.c's
@@ -62,25 +62,19 @@ class DirectGeometry:
 
     @property
     def nsteps(self) -> int:
-        return -(-(self.r * self.s * self.CB) // 16) * 16
+        return -(-(-(-(self.r * self.s * self.c) // 32)) // 8) * 8
 
 
 def weight_matrix(w16: np.ndarray, geo: DirectGeometry) -> np.ndarray:
-    """(K, nsteps*32) with column j = st*32 + c%32, st = tap*CB + c//32 (zero padding)."""
+    """(K, nsteps*32) with column j = (ri*S + si)*C + c (zero padding beyond r*s*C)."""
     K, C, R, S = w16.shape
     out = np.zeros((K, geo.nsteps * 32), dtype=np.float16)
-    for ri in range(R):
-        for si in range(S):
-            tap = ri * S + si
-            for cb in range(geo.CB):
-                c0, c1 = cb * 32, min(C, cb * 32 + 32)
-                st = tap * geo.CB + cb
-                out[:, st * 32: st * 32 + (c1 - c0)] = w16[:, c0:c1, ri, si]
+    out[:, : R * S * C] = w16.transpose(0, 2, 3, 1).reshape(K, R * S * C)
     return out
 
 
 def im2col(x16: np.ndarray, geo: DirectGeometry) -> np.ndarray:
-    """(nsteps*32, P): row j = st*32 + c%32, column g = n*Oh*Ow + oh*Ow + ow."""
+    """(nsteps*32, P): row j = (ri*S + si)*C + c, column g = n*Oh*Ow + oh*Ow + ow."""
     N, C, H, W = x16.shape
     out = np.zeros((geo.nsteps * 32, geo.P), dtype=np.float16)
     oh = np.arange(geo.oh)[:, None]
@@ -93,11 +87,7 @@ def im2col(x16: np.ndarray, geo: DirectGeometry) -> np.ndarray:
             ok = (ih >= 0) & (ih < H) & (iw >= 0) & (iw < W)
             ihc, iwc = np.clip(ih, 0, H - 1), np.clip(iw, 0, W - 1)
             vals = np.where(ok, x16[:, :, ihc, iwc], np.float16(0))  # (N, C, oh, ow); +0 outside
-            vals = vals.transpose(1, 0, 2, 3).reshape(C, -1)  # (C, P)
-            for cb in range(geo.CB):
-                c0, c1 = cb * 32, min(C, cb * 32 + 32)
-                st = tap * geo.CB + cb
-                out[st * 32: st * 32 + (c1 - c0)] = vals[c0:c1]
+            out[tap * C: tap * C + C] = vals.transpose(1, 0, 2, 3).reshape(C, -1)
     return out
 
 

@@ -127,7 +127,7 @@ def test_general_geometry_runs_on_the_direct_path(wc, geo):
     ow = (args["w"] - args["s"] + 2 * args["q"]) // args["v"] + 1
     cb = -(-args["c"] // 32)
     P = args["n"] * oh * ow
-    nsteps = -(-(args["r"] * args["s"] * cb) // 16) * 16
+    nsteps = -(-(-(-(args["r"] * args["s"] * args["c"]) // 32)) // 8) * 8
     assert (prm.Oh, prm.Ow, prm.T, prm.NT, prm.G32, prm.CB, prm.nsteps) == (oh, ow, oh * ow, P, -(-P // 32), cb, nsteps)
     assert prm.KB32 == -(-args["k"] // 32)
     vb, ub, mb = wc.workspace_size(pb)

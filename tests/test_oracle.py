@@ -284,14 +284,28 @@ def test_direct_packed_offsets():
     w = rng.standard_normal((geo.k, geo.c, geo.r, geo.s)).astype(np.float16)
     pw, px = dg.pack_weights(w, geo), dg.pack_input(x, geo)
     assert pw.size == geo.nsteps * 2 * geo.KB32 * 512 and px.size == geo.nsteps * 2 * geo.P32 * 512
-    for (k, c, ri, si) in [(0, 0, 0, 0), (32, 39, 2, 1), (17, 33, 1, 2)]:
-        st = (ri * geo.s + si) * geo.CB + c // 32
-        L, lane = 2 * st + (c % 32) // 16, (c % 16 // 8) * 32 + k % 32
-        assert pw[(L * geo.KB32 + k // 32) * 512 + lane * 8 + c % 8] == w[k, c, ri, si]
-    for (n, c, oh, ow, ri, si) in [(0, 5, 2, 3, 1, 1), (0, 35, 5, 6, 2, 2), (0, 0, 0, 0, 0, 0)]:
-        st = (ri * geo.s + si) * geo.CB + c // 32
+    # C = 40: tap t covers j = 40t .. 40t+39, so taps share 16-value blocks (j = 79 | 80)
+    for (k, c, ri, si) in [(0, 0, 0, 0), (32, 39, 2, 1), (17, 33, 1, 2), (5, 39, 0, 1), (6, 0, 0, 2)]:
+        j = (ri * geo.s + si) * geo.c + c
+        L, lane = j // 16, (j % 16 // 8) * 32 + k % 32
+        assert pw[(L * geo.KB32 + k // 32) * 512 + lane * 8 + j % 8] == w[k, c, ri, si]
+    for (n, c, oh, ow, ri, si) in [(0, 5, 2, 3, 1, 1), (0, 35, 5, 6, 2, 2), (0, 0, 0, 0, 0, 0), (0, 39, 1, 1, 0, 1)]:
+        j = (ri * geo.s + si) * geo.c + c
         g = (n * geo.oh + oh) * geo.ow + ow
-        L, lane = 2 * st + (c % 32) // 16, (c % 16 // 8) * 32 + g % 32
+        L, lane = j // 16, (j % 16 // 8) * 32 + g % 32
         ih, iw = oh * geo.u - geo.p + ri, ow * geo.v - geo.q + si
         want = x[n, c, ih, iw] if (0 <= ih < geo.h and 0 <= iw < geo.w) else 0
-        assert px[(L * geo.P32 + g // 32) * 512 + lane * 8 + c % 8] == want
+        assert px[(L * geo.P32 + g // 32) * 512 + lane * 8 + j % 8] == want
+    # the reduction tail j >= r*s*C is zero in both operands
+    jn = geo.r * geo.s * geo.c
+    tail_w = pw.reshape(geo.nsteps * 2, geo.KB32, 2, 32, 8).transpose(1, 3, 0, 2, 4).reshape(geo.KB32 * 32, -1)
+    assert jn < tail_w.shape[1] and not np.any(tail_w[:, jn:])
+
+
+def test_direct_few_channel_layers_pack_densely():
+    """A 7x7 stem over 3 channels reduces over 147 values: 5 steps of 32, padded to 8, not one
+    32-channel step per tap."""
+    geo = dg.DirectGeometry(1, 3, 16, 16, 64, 7, 7, 2, 2, 3, 3)
+    assert geo.nsteps == 8
+    assert dg.DirectGeometry(1, 64, 8, 8, 8, 3, 3).nsteps == 24  # 576 values = 18 steps -> 24
+    assert dg.DirectGeometry(1, 256, 8, 8, 8, 1, 1).nsteps == 8
