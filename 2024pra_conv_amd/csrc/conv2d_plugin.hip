@@ -71,7 +71,10 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         g->G32 = (g->NT + 31) / 32;
         const long long jn = (long long)pb->r * pb->s * pb->c;
         const long long steps = ((jn + 31) / 32 + 7) / 8 * 8;
-        if (steps * 2 * (g->G32 + g->KB32) * kFragHalves * 2 > (1LL << 40)) return CONV2D_ERR_OVERFLOW;
+        // the packing kernels index j in 32 bits and launch one thread per 16 bytes of an
+        // operand (a 1-D grid of fewer than 2^32 threads): at most 64 GiB per operand
+        const long long rows = g->G32 > g->KB32 ? g->G32 : g->KB32;
+        if (jn > (1LL << 30) || steps * 2 * rows * 64 > (1LL << 32) - 256) return CONV2D_ERR_OVERFLOW;
         g->nsteps = (int)steps;
         g->band = 0;
         g->v_bytes = (size_t)steps * 2 * (size_t)g->G32 * kFragHalves * 2;

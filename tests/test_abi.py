@@ -99,6 +99,9 @@ def test_getkernelinfo_fills_slots_and_param(wc):
 @pytest.mark.parametrize("change,code", [
     ({"n": 0}, "ERR_INVALID"), ({"c": 40000}, "ERR_OVERFLOW"), ({"h": 1, "p": 0}, "ERR_INVALID"),
     ({"r": 0}, "ERR_INVALID"), ({"u": 0}, "ERR_INVALID"), ({"h": 2, "r": 5, "p": 1}, "ERR_INVALID"),
+    # direct path: reduction index beyond 2^30, packed operand beyond 64 GiB
+    ({"c": 30000, "r": 200, "s": 200, "p": 100, "q": 100}, "ERR_OVERFLOW"),
+    ({"n": 8000, "c": 1024, "h": 256, "w": 256, "p": 3, "q": 3}, "ERR_OVERFLOW"),
 ])
 def test_getkernelinfo_rejects(wc, change, code):
     args = dict(n=1, c=16, h=8, w=8, k=16, r=3, s=3, u=1, v=1, p=1, q=1)
