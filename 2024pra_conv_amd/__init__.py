@@ -73,7 +73,8 @@ class ParamType(ctypes.Structure):
                [(f, ctypes.c_short) for f in
                 ("n", "c", "h", "w", "k", "r", "s", "u", "v", "p", "q", "Oh", "Ow")] + \
                [(f, ctypes.c_int) for f in
-                ("tiles_h", "tiles_w", "T", "NT", "G32", "CB", "KB32", "band", "cfg", "flags", "nsteps")]
+                ("tiles_h", "tiles_w", "T", "NT", "G32", "CB", "KB32", "band", "cfg", "flags", "nsteps",
+                 "fbh", "fbw", "fnbh", "fnbw", "fpr", "fpc", "fsh", "fnu")]
 
 
 _lib = None
@@ -89,6 +90,7 @@ _SIGS = {
     "conv2d_input_transform": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
     "conv2d_gemm": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_forward": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_fused_gemm": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
     "conv2d_num_tile_configs": ([], ctypes.c_int),
     "conv2d_tile_config": ([ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_select_tile_config": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),

@@ -14,6 +14,7 @@
 #include "mykernelParamType.h"
 #include "wino_common.h"
 #include "wino_fp8.h"
+#include "wino_fused.h"
 #include "wino_gemm.h"
 #include "plugin_internal.h"
 
@@ -79,6 +80,33 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         g->band = 0;
         g->v_bytes = (size_t)steps * 2 * (size_t)g->G32 * kFragHalves * 2;
         g->u_bytes = (size_t)steps * 2 * (size_t)g->KB32 * kFragHalves * 2;
+    } else if (pb->w % 2 == 0) {
+        // fused form: the 64-tile block shape that pads the tile grid least (ties: the
+        // squarer block, whose input patch is smaller)
+        const int shapes[4][2] = {{8, 8}, {4, 16}, {16, 4}, {2, 32}};
+        long long best = -1;
+        for (int i = 0; i < 4; ++i) {
+            const int bh = shapes[i][0], bw = shapes[i][1];
+            const long long padded = (long long)((g->tiles_h + bh - 1) / bh) * bh * ((g->tiles_w + bw - 1) / bw) * bw;
+            if (best < 0 || padded < best) {
+                best = padded;
+                g->fbh = bh;
+                g->fbw = bw;
+            }
+        }
+        g->fnbh = (g->tiles_h + g->fbh - 1) / g->fbh;
+        g->fnbw = (g->tiles_w + g->fbw - 1) / g->fbw;
+        g->fsh = (int)(pb->q & 1);
+        g->fpr = 2 * g->fbh + 2;
+        g->fpc = (2 * g->fbw + 2 + g->fsh + 1) & ~1;
+        const int units = 4 * g->fpr * (g->fpc / 2);
+        g->fnu = (units + 255) / 256;
+        g->fitems = (long long)pb->n * g->fnbh * g->fnbw * ((pb->k + 63) / 64);
+        // buffer-load offsets are 32-bit: one image with 32 channels of slack under 2^31
+        // bytes (out-of-image offsets are placed at 2^31 and above), U under 2^31 bytes
+        const long long img_bytes = ((long long)pb->c + 32) * pb->h * pb->w * 2;
+        g->fused_ok = g->fnu <= 4 && g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
+                      (long long)g->u_bytes < (1LL << 31);
     }
     return CONV2D_OK;
 }
@@ -168,6 +196,14 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
     a->band = g.band;
     a->cfg = cfg;
     a->nsteps = g.nsteps;
+    a->fbh = g.fbh;
+    a->fbw = g.fbw;
+    a->fnbh = g.fnbh;
+    a->fnbw = g.fnbw;
+    a->fpr = g.fpr;
+    a->fpc = g.fpc;
+    a->fsh = g.fsh;
+    a->fnu = g.fnu;
     // widest input-row load the staging may use (rows start at multiples of w halves)
     const uintptr_t in_addr = (uintptr_t)pb->in;
     if (pb->w % 8 == 0 && (in_addr & 15) == 0)
@@ -182,11 +218,30 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
 
 Launch slot_filter(const Geometry& g) {
     Launch L;
-    L.kernel = g.direct ? (const void*)&direct_pack_weights : (const void*)&wino_filter_transform;
-    const long long threads = (long long)g.KB32 * 2 * (g.direct ? g.nsteps : g.CB) * 64;
-    L.grid = dim3((unsigned)((threads + 255) / 256), 1, 1);
-    L.block = dim3(256, 1, 1);
     L.lds = 0;
+    if (g.direct) {
+        L.kernel = (const void*)&direct_pack_weights;
+        const long long threads = (long long)g.KB32 * 2 * g.nsteps * 64;
+        L.grid = dim3((unsigned)((threads + 255) / 256), 1, 1);
+        L.block = dim3(256, 1, 1);
+        return L;
+    }
+    // one 64-lane workgroup per (U fragment, transform row)
+    L.kernel = (const void*)&wino_filter_transform;
+    L.grid = dim3((unsigned)(g.KB32 * 2 * g.CB), 4, 1);
+    L.block = dim3(64, 1, 1);
+    return L;
+}
+
+// Fused form: one workgroup of 4 waves per (64 output channels, 64-tile block); LDS holds two
+// input patches during the channel loop and the 128 KiB Z exchange after it.
+Launch slot_fused(const Geometry& g) {
+    Launch L;
+    L.kernel = fused_kernel_ptr(g.fnu);
+    L.grid = dim3((unsigned)g.fitems, 1, 1);
+    L.block = dim3(256, 1, 1);
+    const unsigned patches = 2u * 4u * (unsigned)g.fpr * (unsigned)g.fpc * 16u;
+    L.lds = patches > 131072u ? patches : 131072u;
     return L;
 }
 
@@ -370,6 +425,17 @@ int conv2d_forward(const problem_t* pb, int cfg, void* stream) {
     fill_param(pb, g, cfg, &a);
     if ((rc = launch(slot_input(pb, g), &a, (hipStream_t)stream))) return rc;
     return launch(slot_gemm(g, cfg), &a, (hipStream_t)stream);
+}
+
+int conv2d_fused_gemm(const problem_t* pb, void* stream) {
+    int rc = check_ptrs(pb, true, false, true, false, true);
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = require_winograd(pb, &g))) return rc;
+    if (!g.fused_ok || ((uintptr_t)pb->in & 3)) return CONV2D_ERR_UNSUPPORTED;
+    mykernelParamType a;
+    fill_param(pb, g, 0, &a);
+    return launch(slot_fused(g), &a, (hipStream_t)stream);
 }
 
 int conv2d_num_tile_configs(void) { return kNumGemmConfigs; }

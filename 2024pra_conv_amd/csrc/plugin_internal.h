@@ -19,6 +19,10 @@ struct Geometry {
     int band;
     int nsteps;            // GEMM steps of 32 reduction values: 16 * CB, or ceil(r*s*c/32) padded to 8
     size_t v_bytes, u_bytes;
+    // fused form: block of fbh x fbw tiles, fnbh x fnbw blocks per image (fused_ok: even w)
+    bool fused_ok;
+    int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
+    long long fitems;
 };
 
 struct Launch {
@@ -36,6 +40,7 @@ int heuristic_config(const problem_t* pb, const Geometry& g);
 int select_config(const problem_t* pb, const Geometry& g);
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a);
 Launch slot_filter(const Geometry& g);
+Launch slot_fused(const Geometry& g);
 Launch slot_input(const problem_t* pb, const Geometry& g);
 Launch slot_gemm(const Geometry& g, int cfg);
 Launch slot_noop();

@@ -1,0 +1,304 @@
+// Fused Winograd F(2x2,3x3) forward: input transform, GEMM and output transform in one
+// kernel, so the transformed input V never goes to HBM.
+//
+// The staged form writes V (16 FP16 values per tile and channel, 4x the input) with the
+// input transform and reads it back in the GEMM (reference: BTdB* preprocessing.cuh:273-424,
+// matrixMul* matrixmul.cuh:139-556). Here V is computed in registers, directly in the MFMA
+// B-operand layout, from an LDS-staged input patch, and U (prepared by the filter
+// transform) is read from global memory (L2) straight into A-operand registers.
+//
+// Workgroup: 4 waves (one per SIMD, up to 512 registers each), 64 output channels x one
+// block of FBH x FBW = 64 tiles of one image. Wave a owns the Winograd positions of row a of
+// the 4x4 transform domain (pos = 4a + b, b = 0..3) for the whole 64 x 64 tile: 16 FP32
+// 32x32 accumulators (4 positions x 2 channel blocks x 2 tile blocks = 256 registers).
+//
+// Per 32-channel chunk:
+//   patch:  input rows 2*th0 - p .. +2*FBH+1 and columns from iw0 = 2*tw0 - q - fsh, all 32
+//           channels, zero outside the image, in LDS as [channel group g (8 ch)][row][col][8]
+//           (one ds_read_b128 = 8 channels of one pixel); double-buffered, the next chunk's
+//           global loads in flight during this chunk's MFMAs.
+//   V:      lane (tile t = tb*32 + lane%32, channels 8*(lane/32) of 16-channel half ks) reads
+//           the 2 x 4 window pixels row a needs and computes, in FP16 arithmetic,
+//             s_j = B^T row a applied to window column j   (2 rows: d0-d2, d1+d2, d2-d1, d1-d3)
+//             V_b = (s B)_b: s0-s2, s1+s2, s2-s1, s1-s3
+//           (the reference's input transform is FP16 arithmetic too, preprocessing.cuh:57-72).
+//   MFMA:   acc[b][m][tb] += U[4a+b][k block m] . V_b over the chunk's two 16-channel halves.
+//
+// Epilogue (A^T M A, reference conv2d.cpp:18-47): each wave folds its row into
+// Z_j = sum_b A^T[j][b] M[a][b] (Z_0 = (M0 + M1) + M2, Z_1 = (M1 - M2) - M3) and writes Z to
+// LDS; then wave w takes output block w (channel block w/2, tile block w%2) and forms
+// Y_0j = (Z0_j + Z1_j) + Z2_j, Y_1j = (Z1_j - Z2_j) - Z3_j, rounded once to FP16.
+#include "wino_tile_ops.h"
+#include "wino_fused.h"
+
+namespace {
+
+// Raw buffer resource (stride 0, byte offsets); dword 3 = the gfx9 default data format.
+__device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+// Staging of one thread: up to NU units, a unit = (channel group g, patch row, pixel pair) =
+// 8 channels x 2 adjacent pixels, one 4-byte buffer load per channel. Pixel pairs start on
+// even input columns and W is even, so a pair is entirely inside or outside the image and
+// its offset is 4-byte aligned. Loads are never predicated: a pair outside the image gets
+// a byte offset of 2^31 or more, and a channel >= C lands at or beyond the image's last byte;
+// both are past the resource's range (the image, C*H*W*2 < 2^31 bytes), where buffer loads
+// return zero without accessing memory.
+template <int NU>
+struct Staging {
+    uint32_t off[NU];   // byte offset of the unit's pair in channel g*8 of chunk 0
+    int lds[NU];        // LDS offset (halves) of the unit's first pixel
+    uint32_t w[NU][8];  // loaded pairs
+};
+
+// FP16 x - y as fma(y, m1, x) with m1 = -1: y * -1 is exact, so the single rounding of the
+// fma equals the rounding of the subtraction, zero signs included. An 8-wide FP16
+// subtraction, or an fma whose -1 the compiler can see, is split into scalar instructions;
+// the caller makes m1 opaque (an empty asm statement) so the fma stays packed.
+__device__ inline half8 sub8(half8 x, half8 y, half8 m1) { return __builtin_elementwise_fma(y, m1, x); }
+
+__device__ inline uint32_t pack_lo(uint32_t x, uint32_t y) { return __builtin_amdgcn_perm(y, x, 0x05040100u); }
+__device__ inline uint32_t pack_hi(uint32_t x, uint32_t y) { return __builtin_amdgcn_perm(y, x, 0x07060302u); }
+
+template <int NU>
+__device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw0, Staging<NU>& st) {
+    const int PR = prm.fpr, PC = prm.fpc, PH = PC >> 1;
+    const int nunits = 4 * PR * PH;
+    const int H = prm.h, W = prm.w;
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        const int u = threadIdx.x + 256 * i;
+        const int g = u / (PR * PH);
+        const int rem = u - g * (PR * PH);
+        const int pr = rem / PH;
+        const int m = rem - pr * PH;
+        const int ih = ih0 + pr, iw = iw0 + 2 * m;
+        const bool pix = u < nunits && ih >= 0 && ih < H && iw >= 0 && iw < W;
+        st.off[i] = pix ? (uint32_t)(((g * 8 * H + ih) * W + iw) * 2) : 0x80000000u;
+        // a thread slot without a unit stores to the dummy pixel pair past both patch buffers
+        st.lds[i] = u < nunits ? ((g * PR + pr) * PC + 2 * m) * 8 : 2 * 4 * PR * PC * 8;
+    }
+}
+
+template <int NU>
+__device__ inline void stage_load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes,
+                                  Staging<NU>& st) {
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        const uint32_t base = st.off[i] + chunk_bytes;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            st.w[i][e] = __builtin_amdgcn_raw_buffer_load_b32(in, (int)(base + e * chan_bytes), 0, 0);
+    }
+}
+
+template <int NU>
+__device__ inline void stage_store(const Staging<NU>& st, half_t* buf) {
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        u32x4 p0, p1;  // pixel 2m and 2m+1, 8 channels each
+        p0[0] = pack_lo(st.w[i][0], st.w[i][1]);
+        p0[1] = pack_lo(st.w[i][2], st.w[i][3]);
+        p0[2] = pack_lo(st.w[i][4], st.w[i][5]);
+        p0[3] = pack_lo(st.w[i][6], st.w[i][7]);
+        p1[0] = pack_hi(st.w[i][0], st.w[i][1]);
+        p1[1] = pack_hi(st.w[i][2], st.w[i][3]);
+        p1[2] = pack_hi(st.w[i][4], st.w[i][5]);
+        p1[3] = pack_hi(st.w[i][6], st.w[i][7]);
+        u32x4* dst = (u32x4*)(buf + st.lds[i]);
+        dst[0] = p0;
+        dst[1] = p1;
+    }
+}
+
+}  // namespace
+
+template <int NU>
+__global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType prm) {
+    extern __shared__ __attribute__((aligned(16))) half_t lds[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int a = __builtin_amdgcn_readfirstlane(tid >> 6);  // position row of this wave
+
+    const int K = prm.k, C = prm.c;
+    const int KB = prm.KB32, CB = prm.CB, C16 = 2 * CB;
+    const int FBW = prm.fbw, PR = prm.fpr, PC = prm.fpc;
+    const int nk = (K + 63) / 64;
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, gridDim.x, nk, &kblk, &gblk);
+    const int nbw = prm.fnbw, nbh = prm.fnbh;
+    const int img = gblk / (nbh * nbw);
+    const int brem = gblk - img * (nbh * nbw);
+    const int th0 = (brem / nbw) * prm.fbh;
+    const int tw0 = (brem - (brem / nbw) * nbw) * FBW;
+    const int ih0 = 2 * th0 - prm.p;
+    const int iw0 = 2 * tw0 - prm.q - prm.fsh;
+    const half_t* in_img = (const half_t*)prm.pin + (size_t)img * C * prm.h * prm.w;
+    const uint32_t chan_bytes = (uint32_t)prm.h * prm.w * 2;
+    const __amdgpu_buffer_rsrc_t in_rsrc = buffer_rsrc(in_img, (uint32_t)C * chan_bytes);
+
+    // U fragments of this wave: (b, ks, m) for pos = 4a + b, 16-channel half ks of the chunk,
+    // 32-row block kblk*2 + m (a block beyond K reads block 0: its rows are never stored).
+    // Byte offsets are wave-uniform (scalar); each lane adds its 16 bytes.
+    const __amdgpu_buffer_rsrc_t u_rsrc =
+        buffer_rsrc(prm.transform_filter, (uint32_t)(16u * C16 * KB * kFragHalves * 2u));
+    const int ulane = lane * 16;
+    auto load_u = [&](half8 (&u)[16], int ch) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const int kb = kblk * 2 + m < KB ? kblk * 2 + m : 0;
+                    const uint32_t frag = ((uint32_t)(4 * a + b) * C16 + ch * 2 + ks) * KB + kb;
+                    const u32x4 raw =
+                        __builtin_amdgcn_raw_buffer_load_b128(u_rsrc, ulane, (int)(frag * (kFragHalves * 2u)), 0);
+                    u[(ks * 4 + b) * 2 + m] = __builtin_bit_cast(half8, raw);
+                }
+    };
+
+    // window origin (in pixels) inside the patch for this lane's two tiles
+    int woff[2];
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb) {
+        const int t = tb * 32 + (lane & 31);
+        const int tr = t / FBW, tc = t - (t / FBW) * FBW;
+        woff[tb] = (2 * tr) * PC + 2 * tc + prm.fsh;
+    }
+    // rows of the window that row a of B^T combines: s = d[rp] + sg * d[rm], sg = +-1 (exact,
+    // so the fma rounds once like the addition or subtraction it stands for)
+    const int rp = a == 0 ? 0 : (a == 2 ? 2 : 1);
+    const int rm = a == 0 ? 2 : (a == 2 ? 1 : (a == 1 ? 2 : 3));
+    const half_t sgh = a == 1 ? (half_t)1.0f : (half_t)-1.0f;
+    const half8 sg = {sgh, sgh, sgh, sgh, sgh, sgh, sgh, sgh};
+    half8 m1 = {-1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16};
+    asm volatile("" : "+s"(m1));
+
+    const int PB = 4 * PR * PC * 8;  // halves per patch buffer
+
+    floatx16 acc[4][2][2];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[b][m][tb][e] = 0.f;
+
+    Staging<NU> st;
+    stage_setup<NU>(prm, ih0, iw0, st);
+    stage_load<NU>(in_rsrc, 0u, chan_bytes, st);
+    stage_store<NU>(st, lds);
+    half8 ua[16], ub[16];
+    load_u(ua, 0);
+    __syncthreads();
+
+    auto chunk = [&](int ch, half8 (&ucur)[16], half8 (&unext)[16]) {
+        const bool more = ch + 1 < CB;
+        if (more) {
+            stage_load<NU>(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes, st);
+            load_u(unext, ch + 1);
+        }
+        const half_t* buf = lds + (ch & 1) * PB;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int gsel = ks * 2 + (lane >> 5);
+#pragma unroll
+            for (int tb = 0; tb < 2; ++tb) {
+                const half_t* wp = buf + ((size_t)gsel * PR * PC + woff[tb]) * 8;
+                half8 s[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const half8 dp = *(const half8*)(wp + (rp * PC + j) * 8);
+                    const half8 dm = *(const half8*)(wp + (rm * PC + j) * 8);
+                    s[j] = __builtin_elementwise_fma(dm, sg, dp);
+                }
+                half8 v[4];
+                v[0] = sub8(s[0], s[2], m1);
+                v[1] = s[1] + s[2];
+                v[2] = sub8(s[2], s[1], m1);
+                v[3] = sub8(s[1], s[3], m1);
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        acc[b][m][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ucur[(ks * 4 + b) * 2 + m], v[b],
+                                                                              acc[b][m][tb], 0, 0, 0);
+            }
+        }
+        if (more) stage_store<NU>(st, lds + ((ch + 1) & 1) * PB);
+        __syncthreads();
+    };
+    int ch = 0;
+    for (; ch + 1 < CB; ch += 2) {
+        chunk(ch, ua, ub);
+        chunk(ch + 1, ub, ua);
+    }
+    if (ch < CB) chunk(ch, ua, ub);
+
+    // ---- epilogue: Z exchange through LDS, then Y for one 32x32 block per wave ----
+    // LDS Z layout (floats): [a][j][block = m*2 + tb][e/4][lane][4]
+    float* Z = (float*)lds;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int e4 = 0; e4 < 4; ++e4) {
+                floatx4 z0, z1;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int e = e4 * 4 + q;
+                    const float M0 = acc[0][m][tb][e], M1 = acc[1][m][tb][e];
+                    const float M2 = acc[2][m][tb][e], M3 = acc[3][m][tb][e];
+                    z0[q] = (M0 + M1) + M2;
+                    z1[q] = (M1 - M2) - M3;
+                }
+                const int blk = m * 2 + tb;
+                *(floatx4*)(Z + ((((a * 2 + 0) * 4 + blk) * 4 + e4) * 64 + lane) * 4) = z0;
+                *(floatx4*)(Z + ((((a * 2 + 1) * 4 + blk) * 4 + e4) * 64 + lane) * 4) = z1;
+            }
+    __syncthreads();
+    const int blk = a;  // output block of this wave
+    const int m = blk >> 1, tb = blk & 1;
+    const int t = tb * 32 + (lane & 31);
+    const int th = th0 + t / FBW, tw = tw0 + (t - (t / FBW) * FBW);
+    if (th < prm.tiles_h && tw < prm.tiles_w) {
+        TileCoord tc;
+        tc.img = img;
+        tc.oh = 2 * th;
+        tc.ow = 2 * tw;
+        tc.r1 = tc.oh + 1 < prm.Oh;
+        tc.c1 = tc.ow + 1 < prm.Ow;
+#pragma unroll
+        for (int e4 = 0; e4 < 4; ++e4) {
+            floatx4 z[4][2];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    z[r][j] = *(const floatx4*)(Z + ((((r * 2 + j) * 4 + blk) * 4 + e4) * 64 + lane) * 4);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = e4 * 4 + q;
+                const int k = kblk * 64 + m * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                if (k >= K) continue;
+                const float y00 = (z[0][0][q] + z[1][0][q]) + z[2][0][q];
+                const float y01 = (z[0][1][q] + z[1][1][q]) + z[2][1][q];
+                const float y10 = (z[1][0][q] - z[2][0][q]) - z[3][0][q];
+                const float y11 = (z[1][1][q] - z[2][1][q]) - z[3][1][q];
+                store_tile(prm, tc, k, y00, y01, y10, y11);
+            }
+        }
+    }
+}
+
+template __global__ void wino_fused_kernel<3>(mykernelParamType);
+template __global__ void wino_fused_kernel<4>(mykernelParamType);
+
+const void* fused_kernel_ptr(int nu) {
+    return nu <= 3 ? (const void*)&wino_fused_kernel<3> : (const void*)&wino_fused_kernel<4>;
+}

@@ -11,65 +11,71 @@
 // ---------------------------------------------------------------------------------------
 // S1: filter transform.
 //   G = [[1,0,0],[1/2,1/2,1/2],[1/2,-1/2,1/2],[0,0,1]]   (reference preprocessing.cuh:77-81)
-// One lane produces one (k, 8-channel group) for all 16 positions, so each of its 16
-// stores is one full 16-byte lane slot of a fragment and a wave stores whole 1 KiB
-// fragments (32 rows x 16 channels).
+// Workgroup = one fragment (32 rows k x 16 channels) and one row a of the 4x4 transform
+// domain (blockIdx.y): each lane produces positions 4a .. 4a+3 of one (k, 8-channel group),
+// so each of its 4 stores is one full 16-byte lane slot and the wave stores whole 1 KiB
+// fragments. Splitting by row gives KB32 * 2CB * 4 workgroups (512 for K = C = 256); the
+// 9-value filters of a lane's 8 channels are 144 contiguous bytes, loaded as nine 16-byte
+// vectors when C % 8 == 0 and the weight base is 16-byte aligned.
 // ---------------------------------------------------------------------------------------
-__device__ inline void filter_tile_fp32(const float g[9], float u[16]) {
-    float t[4][3];
+// Row a of U = G g G^T for one 3x3 filter g (row-major), FP32: t = row a of G g, then
+// u[b] = (t G^T)[b]. The same expressions, in the same order, as every other producer of U.
+__device__ inline void filter_row_fp32(const float g[9], int a, float u[4]) {
+    float t[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const float g0 = g[0 * 3 + j], g1 = g[1 * 3 + j], g2 = g[2 * 3 + j];
-        t[0][j] = g0;
-        t[1][j] = ((g0 + g1) + g2) * 0.5f;
-        t[2][j] = ((g0 - g1) + g2) * 0.5f;
-        t[3][j] = g2;
+        t[j] = a == 0 ? g0 : a == 1 ? ((g0 + g1) + g2) * 0.5f : a == 2 ? ((g0 - g1) + g2) * 0.5f : g2;
     }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const float a0 = t[i][0], a1 = t[i][1], a2 = t[i][2];
-        u[i * 4 + 0] = a0;
-        u[i * 4 + 1] = ((a0 + a1) + a2) * 0.5f;
-        u[i * 4 + 2] = ((a0 - a1) + a2) * 0.5f;
-        u[i * 4 + 3] = a2;
-    }
+    u[0] = t[0];
+    u[1] = ((t[0] + t[1]) + t[2]) * 0.5f;
+    u[2] = ((t[0] - t[1]) + t[2]) * 0.5f;
+    u[3] = t[2];
 }
 
-__global__ void __launch_bounds__(256) wino_filter_transform(mykernelParamType prm) {
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int frag = gtid >> 6;
-    const int lane = gtid & 63;
+__global__ void __launch_bounds__(64) wino_filter_transform(mykernelParamType prm) {
+    const int frag = blockIdx.x;
+    const int a = blockIdx.y;
+    const int lane = threadIdx.x;
     const int KB = prm.KB32, CB16 = 2 * prm.CB;
-    if (frag >= KB * CB16) return;
     const int kb = frag % KB;
     const int cb = frag / KB;  // 16-channel block
     const int k = kb * 32 + (lane & 31);
     const int c0 = cb * 16 + (lane >> 5) * 8;
     const int K = prm.k, C = prm.c;
+    const half_t* src = (const half_t*)prm.pweight + ((size_t)k * C + c0) * 9;
 
-    half8 outv[16];
+    // the lane's 72 filter values (8 channels x 9), zero outside K and C; with C % 8 == 0 a
+    // lane's channels are all inside C or all outside it
+    half_t w[72];
+    const bool vec = (C & 7) == 0 && (((uintptr_t)prm.pweight) & 15) == 0;
+    if (k < K && c0 < C && vec) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const half8 v = *(const half8*)(src + 8 * i);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) w[8 * i + e] = v[e];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 72; ++i) w[i] = (k < K && c0 + i / 9 < C) ? src[i] : (half_t)0.0f;
+    }
+
+    half8 outv[4];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const int c = c0 + e;
-        float u[16];
-        if (k < K && c < C) {
-            const half_t* src = (const half_t*)prm.pweight + ((size_t)k * C + c) * 9;
-            float g[9];
+        float g[9], u[4];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) g[i] = (float)src[i];
-            filter_tile_fp32(g, u);
-        } else {
+        for (int i = 0; i < 9; ++i) g[i] = (float)w[9 * e + i];
+        filter_row_fp32(g, a, u);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) u[i] = 0.0f;
-        }
-#pragma unroll
-        for (int pos = 0; pos < 16; ++pos) outv[pos][e] = (half_t)u[pos];
+        for (int b = 0; b < 4; ++b) outv[b][e] = (half_t)u[b];
     }
     half_t* U = (half_t*)prm.transform_filter;
 #pragma unroll
-    for (int pos = 0; pos < 16; ++pos) {
-        half8* dst = (half8*)(U + frag_offset(pos, cb, kb, CB16, KB) + lane * 8);
-        *dst = outv[pos];
+    for (int b = 0; b < 4; ++b) {
+        half8* dst = (half8*)(U + frag_offset(4 * a + b, cb, kb, CB16, KB) + lane * 8);
+        *dst = outv[b];
     }
 }
 

@@ -86,6 +86,10 @@ class WinogradConv:
     def forward(self, cfg: int = -1, stream=None) -> None:
         check(self.L.conv2d_forward(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_forward")
 
+    def fused_gemm(self, stream=None) -> None:
+        """Fused form: input transform inside the GEMM (needs prepare_weights; V unused)."""
+        check(self.L.conv2d_fused_gemm(ctypes.byref(self.pb), _stream_handle(stream)), "conv2d_fused_gemm")
+
     def autotune(self, reps: int = 5, stream=None):
         """Runs the library's autotuner; returns (best_cfg, per-config best times in us, -1 = gated out)."""
         n = self.L.conv2d_num_tile_configs()

@@ -151,6 +151,13 @@ int conv2d_gemm(const problem_t* problem, int cfg, void* stream);
 /* S2 + S3 (steady state: U prepared once). cfg as for conv2d_gemm. */
 int conv2d_forward(const problem_t* problem, int cfg, void* stream);
 
+/* Fused form of S2 + S3 (replaces BTdB*, preprocessing.cuh:273-424, and matrixMul*,
+ * matrixmul.cuh:139-556, in one launch): the input transform runs inside the GEMM in FP16
+ * arithmetic, as the reference's does, so transform_pin is not used. Needs U prepared
+ * (conv2d_prepare_weights). Winograd geometry with even w and a 4-byte aligned input only;
+ * CONV2D_ERR_UNSUPPORTED otherwise. */
+int conv2d_fused_gemm(const problem_t* problem, void* stream);
+
 /* Tile configurations of the GEMM stage. */
 int conv2d_num_tile_configs(void);
 /* Describes configuration cfg: rows of k (bk) and columns of tiles (bt) per workgroup,

@@ -37,6 +37,11 @@ typedef struct mykernelParamType {
                     channel chunks) or, on the general-geometry direct path,
                     ceil(r * s * c / 32) rounded up to a multiple of 8 (the padding carries
                     zeros) */
+    /* fused form (input transform inside the GEMM), Winograd geometry with even w only:
+       fbh x fbw = 64 tiles per workgroup, fnbh x fnbw such blocks per image, input patch
+       of fpr rows x fpc columns starting fsh columns left of 2*tw0 - q, fnu staging units
+       per thread */
+    int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
 } mykernelParamType;
 
 /*

@@ -217,6 +217,35 @@ def input_transform_fp32(x16: np.ndarray, geo: Geometry, keep_fp32: bool = False
     return _to_g(v, geo)
 
 
+def input_transform_fp16(x16: np.ndarray, geo: Geometry) -> np.ndarray:
+    """The fused kernel's input transform (wino_fused.hip), FP16 arithmetic with one rounding
+    per operation, as the reference's own transform is FP16 arithmetic
+    (preprocessing.cuh:57-72, in another operation order): rows first,
+    s_j = d0-d2 | d1+d2 | d2-d1 | d1-d3 of window column j, then V_b = s0-s2 | s1+s2 | s2-s1 |
+    s1-s3. numpy evaluates each float16 operation in float32 and rounds once to float16,
+    which equals the correctly rounded float16 result for + and - (float32 carries more than
+    2*11+2 significand bits). Returns (16, C, NT) float16."""
+    d = input_windows(x16, geo).astype(np.float16)
+    v = np.empty((16,) + d.shape[:4], dtype=np.float16)
+    for a in range(4):
+        rp, rm, add = [(0, 2, False), (1, 2, True), (2, 1, False), (1, 3, False)][a]
+        s = [d[..., rp, j] + d[..., rm, j] if add else d[..., rp, j] - d[..., rm, j] for j in range(4)]
+        v[4 * a + 0] = s[0] - s[2]
+        v[4 * a + 1] = s[1] + s[2]
+        v[4 * a + 2] = s[2] - s[1]
+        v[4 * a + 3] = s[1] - s[3]
+    return _to_g(v, geo)
+
+
+def fused_gpu_model(x16: np.ndarray, w16: np.ndarray, geo: Geometry):
+    """The fused form's data path: FP16 U as the filter transform writes it, FP16-arithmetic
+    V, then the GEMM and output transform in FP64. Returns (y_model, bound_scale) as
+    winograd_gpu_model does."""
+    u = filter_transform_fp32(w16)
+    v = input_transform_fp16(x16, geo)
+    return gemm_output_f64(u, v, geo), gemm_output_f64(u, v, geo, absolute=True)
+
+
 # ---------------------------------------------------------------------------------------
 # Packed layouts (the v_mfma_f32_32x32x16_f16 operand layouts the kernels use)
 #   U[pos][c16][kb][lane][e], kb = k//32, lane = (c%16//8)*32 + k%32, e = c%8

@@ -93,7 +93,7 @@ def test_getkernelinfo_fills_slots_and_param(wc):
     # one-dimensional grid over (column block, k-block) pairs, remapped per XCD in the kernel
     assert ki.slot(3)["grid"] == (-(-prm.G32 // (cfg["bt"] // 32)) * -(-prm.KB32 // (cfg["bk"] // 32)), 1, 1)
     assert ki.slot(2)["grid"] == (-(-28 // prm.band), 8, 32)
-    assert ki.slot(1)["grid"][0] * 256 >= prm.KB32 * 2 * prm.CB * 64
+    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 4, 1) and ki.slot(1)["block"] == (64, 1, 1)
 
 
 @pytest.mark.parametrize("change,code", [

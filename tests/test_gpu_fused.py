@@ -1,0 +1,102 @@
+"""GPU parity tests of the fused form (input transform inside the GEMM, conv2d_fused_gemm),
+through the C-ABI, against the oracle.
+
+Bars
+  output     : within the FP32-accumulation bound of the oracle's FP64 GEMM on the same FP16
+               U and FP16-arithmetic V (oracle/winograd.py fused_gpu_model):
+               |y - y*| <= 2^-11 (|y*| + e) + e + 2^-25, e = (C + 16) 2^-24 sum_pos |coef| |U|.|V|.
+  end to end : normwise max relative error <= 1e-2 against the FP64 direct convolution.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cdirect
+from oracle import winograd as wg
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    # n, c, h, w, k, p, q                   tile grid -> block shape
+    (1, 32, 16, 16, 64, 1, 1),     # 8x8 tiles -> one 8x8 block
+    (2, 40, 14, 22, 70, 1, 1),     # 7x11 tiles, C and K ragged, two k-blocks
+    (1, 16, 9, 12, 8, 0, 0),       # pad 0, odd Oh (ragged bottom row)
+    (1, 3, 13, 10, 5, 2, 2),       # pad 2, C = 3, K = 5
+    (3, 64, 8, 64, 96, 1, 1),      # wide: 4x32 tiles -> 4x16 blocks
+    (1, 33, 70, 6, 16, 1, 0),      # tall: 35x2 tiles -> 16x4 blocks, q = 0
+    (2, 96, 28, 28, 256, 1, 1),    # four k-blocks, ragged blocks of a 14x14 tile grid
+    (1, 8, 4, 130, 32, 1, 1),      # 2x65 tiles -> 2x32 blocks (four staging units a thread)
+]
+
+
+def _ids(s):
+    return "x".join(map(str, s))
+
+
+def _to_dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=_ids)
+def test_fused_matches_oracle(shape, runner, cuda):
+    n, c, h, w, k, p, q = shape
+    print(f"case {shape}", flush=True)
+    geo = wg.Geometry(n, c, h, w, k, p, q)
+    x = wg.uniform_pm1(900 + c, (n, c, h, w))
+    wt = wg.uniform_pm1(907 + c, (k, c, 3, 3))
+    conv = runner.WinogradConv(_to_dev(x, cuda), _to_dev(wt, cuda), pad=(p, q))
+    prm = conv.param()
+    print(f"  block {prm.fbh}x{prm.fbw} units {prm.fnu}", flush=True)
+    conv.prepare_weights()
+    conv.out.fill_(float("nan"))
+    conv.fused_gemm()
+    torch.cuda.synchronize()
+    y = conv.out.cpu().numpy().astype(np.float64)
+    y_model, scale = wg.fused_gpu_model(x, wt, geo)
+    e = (c + 16) * 2.0 ** -24 * scale
+    tol = 2.0 ** -11 * (np.abs(y_model) + e) + e + 2.0 ** -25
+    assert np.all(np.abs(y - y_model) <= tol), f"max excess {np.max(np.abs(y - y_model) - tol)}"
+    ref = cdirect.conv2d_f64(x, wt, pad=(p, q))
+    m = wg.error_metrics(y, ref)
+    print(f"  max_rel={m['max_rel']:.3e} rms_rel={m['rms_rel']:.3e}", flush=True)
+    assert m["finite"] and m["max_rel"] <= 1e-2
+
+
+def test_fused_block_shapes_cover_every_variant(runner, cuda):
+    """The shapes above exercise each 64-tile block shape and both staging-unit counts."""
+    seen = set()
+    for (n, c, h, w, k, p, q) in SHAPES:
+        x = torch.zeros((n, c, h, w), dtype=torch.float16, device=cuda)
+        wt = torch.zeros((k, c, 3, 3), dtype=torch.float16, device=cuda)
+        prm = runner.WinogradConv(x, wt, pad=(p, q)).param()
+        seen.add((prm.fbh, prm.fbw, prm.fnu))
+    assert {(s[0], s[1]) for s in seen} == {(8, 8), (4, 16), (16, 4), (2, 32)}
+    assert {s[2] for s in seen} == {3, 4}
+
+
+def test_fused_full_size_against_fp64_gpu(runner, cuda):
+    """N=32 C=256 56x56 K=256: the fused form against the GPU FP64 kernel, and close to the
+    staged form on the same data."""
+    n, c, h, k = 32, 256, 56, 256
+    print("case full size", flush=True)
+    g = torch.Generator(device=cuda)
+    g.manual_seed(77)
+    x = torch.randn((n, c, h, h), generator=g, device=cuda).half()
+    w = (torch.randn((k, c, 3, 3), generator=g, device=cuda) * (2.0 / (9 * c)) ** 0.5).half()
+    conv = runner.WinogradConv(x, w)
+    conv.prepare_weights()
+    conv.fused_gemm()
+    fused = conv.out.clone()
+    conv.forward()
+    ref = conv.direct_f64()
+    torch.cuda.synchronize()
+    for name, y in (("fused", fused), ("staged", conv.out)):
+        d = (y.double() - ref).abs()
+        max_rel = (d.max() / ref.abs().max()).item()
+        rms_rel = (d.norm() / ref.norm()).item()
+        print(f"  {name}: max_rel={max_rel:.3e} rms_rel={rms_rel:.3e}", flush=True)
+        assert bool(torch.isfinite(y).all()) and max_rel <= 1e-2
+    # deterministic: a second run gives the same bits
+    conv.fused_gemm()
+    torch.cuda.synchronize()
+    assert torch.equal(conv.out.view(torch.int16), fused.view(torch.int16))

@@ -201,6 +201,47 @@ def test_fp32_transforms_within_one_ulp_of_exact():
     assert np.mean(v32 == v64) > 0.999
 
 
+def test_fp16_input_transform_exact_on_small_integers():
+    """With integer inputs every intermediate of the fused form's FP16 transform is a small
+    integer, so it must equal the exact B^T d B: a wrong row, column or sign fails."""
+    geo = wg.Geometry(2, 9, 11, 12, 4, 1, 1)
+    x = np.random.default_rng(8).integers(-6, 7, (2, 9, 11, 12)).astype(np.float16)
+    assert np.array_equal(wg.input_transform_fp16(x, geo).astype(np.float64), wg.input_transform_f64(x, geo))
+    for p in (0, 2):
+        geo = wg.Geometry(1, 5, 9, 8, 3, p, p)
+        x = np.random.default_rng(9 + p).integers(-6, 7, (1, 5, 9, 8)).astype(np.float16)
+        assert np.array_equal(wg.input_transform_fp16(x, geo).astype(np.float64), wg.input_transform_f64(x, geo))
+
+
+def test_fp16_input_transform_within_two_roundings():
+    """Two FP16 roundings (s, then V): |V - V*| <= 2^-10 (1 + 2^-12) (|B^T| |d| |B|) + 2 * 2^-24,
+    and the bound is not loose: some values sit more than one ulp away from the exact V
+    (so the transform really rounds twice, unlike the FP32-order form)."""
+    geo = wg.Geometry(2, 20, 11, 10, 4)
+    x = wg.normal(14, (2, 20, 11, 10))
+    v16 = wg.input_transform_fp16(x, geo).astype(np.float64)
+    v64 = wg.input_transform_f64(x, geo)
+    d = np.abs(wg.input_windows(x, geo).astype(np.float64))
+    scale = wg._to_g(np.einsum("ai,nctsij,bj->abncts", np.abs(wg.BT), d, np.abs(wg.BT)).reshape(16, *d.shape[:4]), geo)
+    assert np.all(np.abs(v16 - v64) <= 2.0 ** -10 * (1 + 2.0 ** -12) * scale + 2.0 ** -23)
+    one_ulp = np.spacing(np.abs(v64).astype(np.float16)).astype(np.float64)
+    assert np.any(np.abs(v16 - v64) > one_ulp)
+    assert np.mean(v16 == v64.astype(np.float16).astype(np.float64)) > 0.5
+
+
+def test_numpy_float16_add_is_correctly_rounded():
+    """The oracle's FP16 arithmetic: ties go to even, and a sum needing more than 24 bits of
+    range still rounds once (1 + 2^-11 -> 1, 1 + 3*2^-11 -> 1 + 2^-9, 2048 + 1 -> 2048,
+    2048 + 3 -> 2052, 65504 + 2^-24 -> 65504); x - x is +0."""
+    f = np.float16
+    assert f(1) + f(2.0 ** -11) == f(1)
+    assert f(1) + f(3 * 2.0 ** -11) == f(1 + 2.0 ** -9)
+    assert f(2048) + f(1) == f(2048)
+    assert f(2048) + f(3) == f(2052)
+    assert f(65504) + f(2.0 ** -24) == f(65504)
+    assert f(2.0 ** -24) - f(2.0 ** -24) == f(0) and not np.signbit(f(2.0 ** -24) - f(2.0 ** -24))
+
+
 def test_pack_roundtrip_and_known_offsets():
     geo = wg.Geometry(2, 40, 7, 9, 20)
     rng = np.random.default_rng(5)
