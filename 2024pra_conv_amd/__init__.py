@@ -21,6 +21,8 @@ DEMO_PATH = os.path.join(PKG_DIR, "lib", "conv2dfp16demo")
 # error codes (include/conv2d.h)
 OK, ERR_NULL, ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_WORKSPACE, ERR_HIP, ERR_INVALID = range(7)
 METHOD_WINOGRAD, METHOD_DIRECT = 0, 1
+ACT_NONE, ACT_RELU = 0, 1
+FORM_STAGED, FORM_FUSED = 0, 1
 ERROR_NAMES = {
     ERR_NULL: "CONV2D_ERR_NULL",
     ERR_UNSUPPORTED: "CONV2D_ERR_UNSUPPORTED",
@@ -74,7 +76,8 @@ class ParamType(ctypes.Structure):
                 ("n", "c", "h", "w", "k", "r", "s", "u", "v", "p", "q", "Oh", "Ow")] + \
                [(f, ctypes.c_int) for f in
                 ("tiles_h", "tiles_w", "T", "NT", "G32", "CB", "KB32", "band", "cfg", "flags", "nsteps",
-                 "fbh", "fbw", "fnbh", "fnbw", "fpr", "fpc", "fsh", "fnu")]
+                 "fbh", "fbw", "fnbh", "fnbw", "fpr", "fpc", "fsh", "fnu", "act", "form")] + \
+               [("bias", ctypes.c_void_p)]
 
 
 _lib = None
@@ -91,6 +94,9 @@ _SIGS = {
     "conv2d_gemm": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_forward": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_fused_gemm": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_form": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
+    "conv2d_set_epilogue": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "conv2d_forward_epilogue": ([_P(Problem), ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_num_tile_configs": ([], ctypes.c_int),
     "conv2d_tile_config": ([ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_select_tile_config": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),

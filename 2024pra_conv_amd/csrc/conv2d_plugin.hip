@@ -165,6 +165,16 @@ int select_config(const problem_t* pb, const Geometry& g) {
     return heuristic_config(pb, g);
 }
 
+// Form choice on the Winograd path: the fused kernel where it measured faster (every layer
+// with N*T*K*C below 1.2e9 in a sweep of 20 shapes, by 1.2-2.4x; at and above the benchmark
+// layer's 1.6e9 the staged GEMM ties or wins), unless the shape was tuned (the tuned table
+// holds staged configurations).
+static bool use_fused(const problem_t* pb, const Geometry& g) {
+    if (g.direct || !g.fused_ok || ((uintptr_t)pb->in & 3)) return false;
+    if (tuned_config_lookup(pb) >= 0) return false;
+    return (double)g.NT * (double)pb->k * (double)pb->c < 1.2e9;
+}
+
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {
     memset(a, 0, sizeof(*a));
     a->pin = pb->in;
@@ -324,13 +334,15 @@ int getkernelInfo(problem_t* problem, kernelInfo_t* ki, void* param) {
     int rc = derive_geometry(problem, &g);
     if (rc) return rc;
     const int cfg = select_config(problem, g);
+    const bool fused = use_fused(problem, g);
     fill_param(problem, g, cfg, (mykernelParamType*)param);
+    ((mykernelParamType*)param)->form = fused ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED;
     put_slot(slot_filter(g), &ki->blockx1, &ki->blocky1, &ki->blockz1, &ki->threadx1,
              &ki->thready1, &ki->threadz1, &ki->dynmicLdsSize1, &ki->kernelPtr1);
-    put_slot(slot_input(problem, g), &ki->blockx2, &ki->blocky2, &ki->blockz2, &ki->threadx2,
-             &ki->thready2, &ki->threadz2, &ki->dynmicLdsSize2, &ki->kernelPtr2);
-    put_slot(slot_gemm(g, cfg), &ki->blockx3, &ki->blocky3, &ki->blockz3, &ki->threadx3,
-             &ki->thready3, &ki->threadz3, &ki->dynmicLdsSize3, &ki->kernelPtr3);
+    put_slot(fused ? slot_fused(g) : slot_input(problem, g), &ki->blockx2, &ki->blocky2, &ki->blockz2,
+             &ki->threadx2, &ki->thready2, &ki->threadz2, &ki->dynmicLdsSize2, &ki->kernelPtr2);
+    put_slot(fused ? slot_noop() : slot_gemm(g, cfg), &ki->blockx3, &ki->blocky3, &ki->blockz3,
+             &ki->threadx3, &ki->thready3, &ki->threadz3, &ki->dynmicLdsSize3, &ki->kernelPtr3);
     put_slot(slot_noop(), &ki->blockx4, &ki->blocky4, &ki->blockz4, &ki->threadx4,
              &ki->thready4, &ki->threadz4, &ki->dynmicLdsSize4, &ki->kernelPtr4);
     return CONV2D_OK;
@@ -419,12 +431,60 @@ int conv2d_forward(const problem_t* pb, int cfg, void* stream) {
     if (rc) return rc;
     Geometry g;
     if ((rc = derive_geometry(pb, &g))) return rc;
+    if (cfg < 0 && use_fused(pb, g)) {
+        mykernelParamType a;
+        fill_param(pb, g, select_config(pb, g), &a);
+        a.form = CONV2D_FORM_FUSED;
+        return launch(slot_fused(g), &a, (hipStream_t)stream);
+    }
     if (cfg < 0) cfg = select_config(pb, g);
     if (cfg >= (g.direct ? kNumDirectConfigs : kNumGemmConfigs)) return CONV2D_ERR_INVALID;
     mykernelParamType a;
     fill_param(pb, g, cfg, &a);
     if ((rc = launch(slot_input(pb, g), &a, (hipStream_t)stream))) return rc;
     return launch(slot_gemm(g, cfg), &a, (hipStream_t)stream);
+}
+
+int conv2d_set_epilogue(void* param, const void* bias, int act) {
+    if (!param) return CONV2D_ERR_NULL;
+    if (act != CONV2D_ACT_NONE && act != CONV2D_ACT_RELU) return CONV2D_ERR_INVALID;
+    mykernelParamType* a = (mykernelParamType*)param;
+    a->bias = bias;
+    a->act = act;
+    return CONV2D_OK;
+}
+
+int conv2d_forward_epilogue(const problem_t* pb, int cfg, const void* bias, int act, void* stream) {
+    int rc = check_ptrs(pb, true, false, true, true, true);
+    if (rc) return rc;
+    if (act != CONV2D_ACT_NONE && act != CONV2D_ACT_RELU) return CONV2D_ERR_INVALID;
+    Geometry g;
+    if ((rc = derive_geometry(pb, &g))) return rc;
+    if (cfg < 0 && use_fused(pb, g)) {
+        mykernelParamType a;
+        fill_param(pb, g, select_config(pb, g), &a);
+        a.form = CONV2D_FORM_FUSED;
+        a.bias = bias;
+        a.act = act;
+        return launch(slot_fused(g), &a, (hipStream_t)stream);
+    }
+    if (cfg < 0) cfg = select_config(pb, g);
+    if (cfg >= (g.direct ? kNumDirectConfigs : kNumGemmConfigs)) return CONV2D_ERR_INVALID;
+    mykernelParamType a;
+    fill_param(pb, g, cfg, &a);
+    a.bias = bias;
+    a.act = act;
+    if ((rc = launch(slot_input(pb, g), &a, (hipStream_t)stream))) return rc;
+    return launch(slot_gemm(g, cfg), &a, (hipStream_t)stream);
+}
+
+int conv2d_form(const problem_t* pb, int* form) {
+    if (!pb || !form) return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = derive_geometry(pb, &g);
+    if (rc) return rc;
+    *form = use_fused(pb, g) ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED;
+    return CONV2D_OK;
 }
 
 int conv2d_fused_gemm(const problem_t* pb, void* stream) {

@@ -14,8 +14,10 @@
 //
 // Per 32-channel chunk:
 //   patch:  input rows 2*th0 - p .. +2*FBH+1 and columns from iw0 = 2*tw0 - q - fsh, all 32
-//           channels, zero outside the image, in LDS as [channel group g (8 ch)][row][col][8]
-//           (one ds_read_b128 = 8 channels of one pixel); double-buffered, the next chunk's
+//           channels, zero outside the image, in LDS as [channel group g (8 ch)][row]
+//           [column parity][column / 2][8] (one ds_read_b128 = 8 channels of one pixel; the
+//           parity split puts the same window column of consecutive tiles 16 bytes apart, so
+//           a wave's reads are free of bank conflicts); double-buffered, the next chunk's
 //           global loads in flight during this chunk's MFMAs.
 //   V:      lane (tile t = tb*32 + lane%32, channels 8*(lane/32) of 16-channel half ks) reads
 //           the 2 x 4 window pixels row a needs and computes, in FP16 arithmetic,
@@ -76,8 +78,9 @@ __device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw
         const int ih = ih0 + pr, iw = iw0 + 2 * m;
         const bool pix = u < nunits && ih >= 0 && ih < H && iw >= 0 && iw < W;
         st.off[i] = pix ? (uint32_t)(((g * 8 * H + ih) * W + iw) * 2) : 0x80000000u;
-        // a thread slot without a unit stores to the dummy pixel pair past both patch buffers
-        st.lds[i] = u < nunits ? ((g * PR + pr) * PC + 2 * m) * 8 : 2 * 4 * PR * PC * 8;
+        // pair m: pixel 2m at column parity 0, pixel 2m+1 at parity 1, both at column m; a
+        // thread slot without a unit stores to a dummy pixel pair past both patch buffers
+        st.lds[i] = u < nunits ? ((g * PR + pr) * PC + m) * 8 : 2 * 4 * PR * PC * 8;
     }
 }
 
@@ -94,7 +97,7 @@ __device__ inline void stage_load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_byte
 }
 
 template <int NU>
-__device__ inline void stage_store(const Staging<NU>& st, half_t* buf) {
+__device__ inline void stage_store(const Staging<NU>& st, half_t* buf, int half_pc) {
 #pragma unroll
     for (int i = 0; i < NU; ++i) {
         u32x4 p0, p1;  // pixel 2m and 2m+1, 8 channels each
@@ -106,9 +109,8 @@ __device__ inline void stage_store(const Staging<NU>& st, half_t* buf) {
         p1[1] = pack_hi(st.w[i][2], st.w[i][3]);
         p1[2] = pack_hi(st.w[i][4], st.w[i][5]);
         p1[3] = pack_hi(st.w[i][6], st.w[i][7]);
-        u32x4* dst = (u32x4*)(buf + st.lds[i]);
-        dst[0] = p0;
-        dst[1] = p1;
+        *(u32x4*)(buf + st.lds[i]) = p0;
+        *(u32x4*)(buf + st.lds[i] + half_pc * 8) = p1;
     }
 }
 
@@ -165,8 +167,13 @@ __global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType pr
     for (int tb = 0; tb < 2; ++tb) {
         const int t = tb * 32 + (lane & 31);
         const int tr = t / FBW, tc = t - (t / FBW) * FBW;
-        woff[tb] = (2 * tr) * PC + 2 * tc + prm.fsh;
+        woff[tb] = (2 * tr) * PC + tc;
     }
+    // window column j of tile column tc is patch column 2*tc + fsh + j: parity (fsh + j) & 1,
+    // position tc + (fsh + j) / 2
+    int colj[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) colj[j] = ((prm.fsh + j) & 1) * (PC / 2) + ((prm.fsh + j) >> 1);
     // rows of the window that row a of B^T combines: s = d[rp] + sg * d[rm], sg = +-1 (exact,
     // so the fma rounds once like the addition or subtraction it stands for)
     const int rp = a == 0 ? 0 : (a == 2 ? 2 : 1);
@@ -191,7 +198,7 @@ __global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType pr
     Staging<NU> st;
     stage_setup<NU>(prm, ih0, iw0, st);
     stage_load<NU>(in_rsrc, 0u, chan_bytes, st);
-    stage_store<NU>(st, lds);
+    stage_store<NU>(st, lds, PC / 2);
     half8 ua[16], ub[16];
     load_u(ua, 0);
     __syncthreads();
@@ -212,8 +219,8 @@ __global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType pr
                 half8 s[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const half8 dp = *(const half8*)(wp + (rp * PC + j) * 8);
-                    const half8 dm = *(const half8*)(wp + (rm * PC + j) * 8);
+                    const half8 dp = *(const half8*)(wp + (rp * PC + colj[j]) * 8);
+                    const half8 dm = *(const half8*)(wp + (rm * PC + colj[j]) * 8);
                     s[j] = __builtin_elementwise_fma(dm, sg, dp);
                 }
                 half8 v[4];
@@ -229,7 +236,7 @@ __global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType pr
                                                                               acc[b][m][tb], 0, 0, 0);
             }
         }
-        if (more) stage_store<NU>(st, lds + ((ch + 1) & 1) * PB);
+        if (more) stage_store<NU>(st, lds + ((ch + 1) & 1) * PB, PC / 2);
         __syncthreads();
     };
     int ch = 0;

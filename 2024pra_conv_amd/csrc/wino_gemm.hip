@@ -208,25 +208,6 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
 // ---------------------------------------------------------------------------------------
 namespace {
 
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Winline-asm"
-__device__ inline void lds_dma16(const half_t* gptr, uint32_t lds_byte_addr) {
-    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
-                 :
-                 : "s"(lds_byte_addr), "v"(gptr)
-                 : "memory", "m0");
-}
-#pragma clang diagnostic pop
-
-template <int N>
-__device__ inline void wait_vm() {
-    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
-}
-
-__device__ inline uint32_t lds_address(const half_t* p) {
-    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) half_t*)p;
-}
-
 template <int BK, int BT, int WK, int WT, int S>
 struct DmaShape {
     static constexpr int kWaves = WK * WT;

@@ -165,8 +165,25 @@ __device__ inline TileCoord tile_coord(const mykernelParamType& prm, int g) {
     return tc;
 }
 
+// Output epilogue in FP32: y + bias[k], then the activation (1: ReLU; a NaN stays NaN).
+// prm.bias and prm.act are kernel arguments, so the branches are wave-uniform.
+__device__ inline float epilogue(const mykernelParamType& prm, float y, float b) {
+    y += b;
+    return (prm.act == 1 && y < 0.f) ? 0.f : y;
+}
+__device__ inline float epilogue_bias(const mykernelParamType& prm, int k) {
+    return prm.bias ? (float)((const half_t*)prm.bias)[k] : 0.f;
+}
+
 __device__ inline void store_tile(const mykernelParamType& prm, const TileCoord& tc, int k, float y00, float y01,
                                   float y10, float y11) {
+    if (prm.bias || prm.act) {
+        const float b = epilogue_bias(prm, k);
+        y00 = epilogue(prm, y00, b);
+        y01 = epilogue(prm, y01, b);
+        y10 = epilogue(prm, y10, b);
+        y11 = epilogue(prm, y11, b);
+    }
     const int Oh = prm.Oh, Ow = prm.Ow;
     half_t* o = (half_t*)prm.pout + (((size_t)tc.img * prm.k + k) * Oh + tc.oh) * Ow + tc.ow;
     o[0] = (half_t)y00;
@@ -219,9 +236,37 @@ __device__ inline void store_plain32(const mykernelParamType& prm, const floatx1
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const int k = (kbw + mi) * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-                if (k < prm.k) o[(size_t)k * prm.T] = (half_t)acc[mi][ni][j];
+                if (k >= prm.k) continue;
+                float y = acc[mi][ni][j];
+                if (prm.bias || prm.act) y = epilogue(prm, y, epilogue_bias(prm, k));
+                o[(size_t)k * prm.T] = (half_t)y;
             }
     }
+}
+
+// ---- LDS DMA ------------------------------------------------------------------------------
+
+// One global_load_lds_dwordx4: lane i's 16 bytes at gptr land in LDS at lds_byte_addr + 16 i
+// (the address goes through m0, so it must be wave-uniform). No destination registers; the
+// load counts in vmcnt until its bytes are in LDS.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ inline void lds_dma16(const half_t* gptr, uint32_t lds_byte_addr) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(lds_byte_addr), "v"(gptr)
+                 : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+// Waits until at most N vector-memory operations of this wave are outstanding.
+template <int N>
+__device__ inline void wait_vm() {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+__device__ inline uint32_t lds_address(const half_t* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) half_t*)p;
 }
 
 // Hardware dispatches workgroup ids round-robin over the 8 XCDs; this remaps id L so every

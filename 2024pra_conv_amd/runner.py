@@ -86,6 +86,24 @@ class WinogradConv:
     def forward(self, cfg: int = -1, stream=None) -> None:
         check(self.L.conv2d_forward(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_forward")
 
+    def forward_epilogue(self, bias: torch.Tensor | None, act: int, cfg: int = -1, stream=None) -> None:
+        """S2 + S3 with out = act(conv + bias[k]) (bias: K float16 on the device, or None)."""
+        ptr = None if bias is None else bias.data_ptr()
+        check(self.L.conv2d_forward_epilogue(ctypes.byref(self.pb), cfg, ptr, act, _stream_handle(stream)),
+              "conv2d_forward_epilogue")
+
+    def set_slot_epilogue(self, bias: torch.Tensor | None, act: int) -> None:
+        """Makes the four slots (launch_slots) apply the epilogue."""
+        _, param = self.kernel_info()
+        ptr = None if bias is None else bias.data_ptr()
+        check(self.L.conv2d_set_epilogue(param, ptr, act), "conv2d_set_epilogue")
+
+    def form(self) -> int:
+        """FORM_STAGED or FORM_FUSED: the form the four slots and forward() use."""
+        f = ctypes.c_int()
+        check(self.L.conv2d_form(ctypes.byref(self.pb), ctypes.byref(f)), "conv2d_form")
+        return f.value
+
     def fused_gemm(self, stream=None) -> None:
         """Fused form: input transform inside the GEMM (needs prepare_weights; V unused)."""
         check(self.L.conv2d_fused_gemm(ctypes.byref(self.pb), _stream_handle(stream)), "conv2d_fused_gemm")

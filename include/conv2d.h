@@ -151,12 +151,33 @@ int conv2d_gemm(const problem_t* problem, int cfg, void* stream);
 /* S2 + S3 (steady state: U prepared once). cfg as for conv2d_gemm. */
 int conv2d_forward(const problem_t* problem, int cfg, void* stream);
 
+/* Output epilogue (no counterpart in the reference, whose stage 4 stores the plain
+ * convolution, conv2d.cpp:37-47): every output becomes act(y + bias[k]), computed in FP32
+ * before the output's one FP16 rounding. bias: K FP16 values in device memory, or NULL;
+ * act: CONV2D_ACT_NONE or CONV2D_ACT_RELU (a NaN stays NaN). */
+#define CONV2D_ACT_NONE 0
+#define CONV2D_ACT_RELU 1
+/* Sets the epilogue in a kernel-argument blob filled by getkernelInfo, so that the four
+ * slots launched with it (conv2d_launch, or the harness's own launches) apply it. */
+int conv2d_set_epilogue(void* param, const void* bias, int act);
+/* S2 + S3 with the epilogue; cfg as for conv2d_forward (-1: getkernelInfo's choice). Works on
+ * both the Winograd and the general-geometry path. */
+int conv2d_forward_epilogue(const problem_t* problem, int cfg, const void* bias, int act, void* stream);
+
 /* Fused form of S2 + S3 (replaces BTdB*, preprocessing.cuh:273-424, and matrixMul*,
  * matrixmul.cuh:139-556, in one launch): the input transform runs inside the GEMM in FP16
  * arithmetic, as the reference's does, so transform_pin is not used. Needs U prepared
  * (conv2d_prepare_weights). Winograd geometry with even w and a 4-byte aligned input only;
  * CONV2D_ERR_UNSUPPORTED otherwise. */
 int conv2d_fused_gemm(const problem_t* problem, void* stream);
+
+/* Which form getkernelInfo and conv2d_forward(-1) use for a Winograd-path problem:
+ * CONV2D_FORM_FUSED when the fused form applies, the shape has no tuned-table entry and
+ * N*T*K*C < 1.2e9 (T = 2x2 tiles per image; where the fused form measured faster on MI355X,
+ * 1.2-2.4x on layers up to that size), else CONV2D_FORM_STAGED (also for the direct path). */
+#define CONV2D_FORM_STAGED 0
+#define CONV2D_FORM_FUSED 1
+int conv2d_form(const problem_t* problem, int* form);
 
 /* Tile configurations of the GEMM stage. */
 int conv2d_num_tile_configs(void);

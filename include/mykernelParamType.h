@@ -42,6 +42,11 @@ typedef struct mykernelParamType {
        of fpr rows x fpc columns starting fsh columns left of 2*tw0 - q, fnu staging units
        per thread */
     int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
+    /* output epilogue, applied in FP32 before the one FP16 rounding of every output:
+       out = act(y + bias[k]); bias: K FP16 values or NULL, act: 0 none, 1 ReLU */
+    int act;
+    int form;  /* Winograd path: 0 staged (slots: S1, S2, S3, empty), 1 fused (S1, fused, empty x2) */
+    const void* bias;
 } mykernelParamType;
 
 /*

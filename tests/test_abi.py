@@ -40,6 +40,8 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", offsetof(problem_t, n), offsetof(problem_t, q), offsetof(kernelInfo_t, kernelPtr1), offsetof(kernelInfo_t, blockx4));
   printf("%zu %zu %zu %zu %zu\n", offsetof(mykernelParamType, n), offsetof(mykernelParamType, Ow),
          offsetof(mykernelParamType, tiles_h), offsetof(mykernelParamType, cfg), offsetof(mykernelParamType, flags));
+  printf("%zu %zu %zu %zu %zu\n", offsetof(mykernelParamType, nsteps), offsetof(mykernelParamType, fnu),
+         offsetof(mykernelParamType, act), offsetof(mykernelParamType, form), offsetof(mykernelParamType, bias));
   return 0; }
 ''')
     exe = tmp_path / "layout"
@@ -50,6 +52,7 @@ int main(void) {
     P, K, M = wc.Problem, wc.KernelInfo, wc.ParamType
     assert list(map(int, lines[1].split())) == [P.n.offset, P.q.offset, K.kernelPtr1.offset, K.blockx4.offset]
     assert list(map(int, lines[2].split())) == [M.n.offset, M.Ow.offset, M.tiles_h.offset, M.cfg.offset, M.flags.offset]
+    assert list(map(int, lines[3].split())) == [M.nsteps.offset, M.fnu.offset, M.act.offset, M.form.offset, M.bias.offset]
     # the reference layouts (probed: problem_t 96 B, kernelInfo_t 160 B; param: pointers 0-47,
     # the 13 shorts at 48-73)
     assert ctypes.sizeof(P) == 96 and ctypes.sizeof(K) == 160
@@ -94,6 +97,42 @@ def test_getkernelinfo_fills_slots_and_param(wc):
     assert ki.slot(3)["grid"] == (-(-prm.G32 // (cfg["bt"] // 32)) * -(-prm.KB32 // (cfg["bk"] // 32)), 1, 1)
     assert ki.slot(2)["grid"] == (-(-28 // prm.band), 8, 32)
     assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 4, 1) and ki.slot(1)["block"] == (64, 1, 1)
+
+
+def test_form_choice_and_fused_slots(wc):
+    """The benchmark layer (N*T*K*C = 1.6e9) stays staged, small layers take the fused form
+    (slot 2 = the fused kernel, slots 3 and 4 empty), odd widths and the direct path stay
+    staged."""
+    L = wc.load_library()
+    f = ctypes.c_int()
+    cases = [((32, 256, 56, 56, 256), {}, wc.FORM_STAGED), ((8, 256, 14, 14, 256), {}, wc.FORM_FUSED),
+             ((32, 96, 55, 55, 96), {}, wc.FORM_STAGED), ((2, 16, 9, 9, 16), dict(u=2, v=2), wc.FORM_STAGED)]
+    for (n, c, h, w, k), extra, want in cases:
+        pb = wc.make_problem(n, c, h, w, k, **extra, ptrs=FAKE)
+        assert L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK and f.value == want, (n, c, h, k)
+        rc, ki, prm = _ki(wc, pb)
+        assert rc == wc.OK and prm.form == want
+        if want == wc.FORM_FUSED:
+            assert ki.slot(2)["block"] == (256, 1, 1) and ki.slot(2)["grid"][0] == n * prm.fnbh * prm.fnbw * -(-k // 64)
+            assert ki.slot(3)["grid"] == (1, 1, 1) and ki.slot(4)["grid"] == (1, 1, 1)
+            assert ki.slot(2)["lds"] >= 131072
+    assert L.conv2d_form(None, ctypes.byref(f)) == wc.ERR_NULL
+
+
+def test_set_epilogue_fills_the_blob(wc):
+    L = wc.load_library()
+    pb = wc.make_problem(2, 16, 8, 8, 24, ptrs=FAKE)
+    ki = wc.KernelInfo()
+    buf = ctypes.create_string_buffer(ctypes.sizeof(wc.ParamType))
+    assert L.getkernelInfo(ctypes.byref(pb), ctypes.byref(ki), buf) == wc.OK
+    prm = wc.ParamType.from_buffer_copy(buf)
+    assert prm.act == wc.ACT_NONE and not prm.bias  # no epilogue unless asked for
+    assert L.conv2d_set_epilogue(buf, 4096, wc.ACT_RELU) == wc.OK
+    prm = wc.ParamType.from_buffer_copy(buf)
+    assert prm.act == wc.ACT_RELU and prm.bias == 4096 and prm.k == 24
+    assert L.conv2d_set_epilogue(buf, None, 7) == wc.ERR_INVALID
+    assert L.conv2d_set_epilogue(None, None, wc.ACT_NONE) == wc.ERR_NULL
+    assert L.conv2d_forward_epilogue(ctypes.byref(pb), -1, None, 3, None) == wc.ERR_INVALID
 
 
 @pytest.mark.parametrize("change,code", [
