@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libwinoconv.so")
 DEMO_PATH = os.path.join(PKG_DIR, "lib", "conv2dfp16demo")
 
 # error codes (include/conv2d.h)
-OK, ERR_NULL, ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_WORKSPACE, ERR_HIP, ERR_INVALID = range(7)
+OK, ERR_NULL, ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_WORKSPACE, ERR_HIP, ERR_INVALID, ERR_IO, ERR_MISMATCH = range(9)
 METHOD_WINOGRAD, METHOD_DIRECT = 0, 1
 ACT_NONE, ACT_RELU = 0, 1
 FORM_STAGED, FORM_FUSED = 0, 1
@@ -30,6 +30,8 @@ ERROR_NAMES = {
     ERR_WORKSPACE: "CONV2D_ERR_WORKSPACE",
     ERR_HIP: "CONV2D_ERR_HIP",
     ERR_INVALID: "CONV2D_ERR_INVALID",
+    ERR_IO: "CONV2D_ERR_IO",
+    ERR_MISMATCH: "CONV2D_ERR_MISMATCH",
 }
 
 
@@ -95,6 +97,9 @@ _SIGS = {
     "conv2d_forward": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_fused_gemm": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
     "conv2d_form": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
+    "conv2d_weights_save": ([_P(Problem), ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_weights_load": ([_P(Problem), ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_weight_hash": ([ctypes.c_void_p, ctypes.c_size_t, _P(ctypes.c_ulonglong)], ctypes.c_int),
     "conv2d_set_epilogue": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "conv2d_forward_epilogue": ([_P(Problem), ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_num_tile_configs": ([], ctypes.c_int),

@@ -29,6 +29,7 @@ LIB_SOURCES = [
     "conv2d_plugin.hip",
     "tuner.hip",
     "batch.hip",
+    "u_cache.hip",
     "mlp.hip",
 ]
 

@@ -98,6 +98,15 @@ class WinogradConv:
         ptr = None if bias is None else bias.data_ptr()
         check(self.L.conv2d_set_epilogue(param, ptr, act), "conv2d_set_epilogue")
 
+    def save_weights(self, path: str, stream=None) -> None:
+        """Writes the prepared U to a file keyed by the weights' hash (conv2d_weights_save)."""
+        check(self.L.conv2d_weights_save(ctypes.byref(self.pb), path.encode(), _stream_handle(stream)),
+              "conv2d_weights_save")
+
+    def load_weights(self, path: str, stream=None) -> int:
+        """Loads U from a file; returns the library's code (OK, ERR_MISMATCH, ERR_IO, ...)."""
+        return self.L.conv2d_weights_load(ctypes.byref(self.pb), path.encode(), _stream_handle(stream))
+
     def form(self) -> int:
         """FORM_STAGED or FORM_FUSED: the form the four slots and forward() use."""
         f = ctypes.c_int()

@@ -44,7 +44,10 @@ enum {
     CONV2D_ERR_OVERFLOW = 3,    /* a dimension does not fit the kernel argument (short) */
     CONV2D_ERR_WORKSPACE = 4,   /* caller-declared workspace smaller than required */
     CONV2D_ERR_HIP = 5,         /* a HIP runtime call failed */
-    CONV2D_ERR_INVALID = 6      /* bad argument (zero dimension, unknown config, ...) */
+    CONV2D_ERR_INVALID = 6,     /* bad argument (zero dimension, unknown config, ...) */
+    CONV2D_ERR_IO = 7,          /* a file could not be opened, read or written in full */
+    CONV2D_ERR_MISMATCH = 8     /* a persisted U is not for this problem, these weights or
+                                   this layout version, or its payload is corrupt */
 };
 
 /* Method a problem runs on (conv2d_method). */
@@ -170,6 +173,22 @@ int conv2d_forward_epilogue(const problem_t* problem, int cfg, const void* bias,
  * (conv2d_prepare_weights). Winograd geometry with even w and a 4-byte aligned input only;
  * CONV2D_ERR_UNSUPPORTED otherwise. */
 int conv2d_fused_gemm(const problem_t* problem, void* stream);
+
+/* Persisted transformed weights (the reference keeps U in its own transform_filter buffer,
+ * conv2d.h:14, main.cpp:38, and recomputes it every run). File: a 64-byte header (magic
+ * "WINOU16\0", layout version, method, k, c, r, s, FNV-1a 64 hash of the K*C*R*S FP16
+ * weights, U bytes, FNV-1a 64 hash of the U bytes) followed by U exactly as
+ * problem->transform_filter holds it after conv2d_prepare_weights.
+ * save: U must be prepared; writes the file (synchronises the stream).
+ * load: checks the header against the problem (CONV2D_ERR_MISMATCH before any device access
+ * for a foreign file), hashes the problem's current weights and the payload, and only when
+ * everything matches copies U into problem->transform_filter (synchronises the stream). A
+ * stale cache (the weights changed) is CONV2D_ERR_MISMATCH, never silently used. */
+#define CONV2D_U_LAYOUT_VERSION 1
+int conv2d_weights_save(const problem_t* problem, const char* path, void* stream);
+int conv2d_weights_load(const problem_t* problem, const char* path, void* stream);
+/* FNV-1a 64 of host bytes, the hash the weight cache uses. */
+int conv2d_weight_hash(const void* host_bytes, size_t nbytes, unsigned long long* hash);
 
 /* Which form getkernelInfo and conv2d_forward(-1) use for a Winograd-path problem:
  * CONV2D_FORM_FUSED when the fused form applies, the shape has no tuned-table entry and

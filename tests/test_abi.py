@@ -99,6 +99,48 @@ def test_getkernelinfo_fills_slots_and_param(wc):
     assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 4, 1) and ki.slot(1)["block"] == (64, 1, 1)
 
 
+def test_weight_hash_is_fnv1a64():
+    """The published FNV-1a 64 test vectors (offset basis, "a", "foobar")."""
+    import importlib
+    wc = importlib.import_module("2024pra_conv_amd")
+    L = wc.load_library()
+    h = ctypes.c_ulonglong()
+    for data, want in ((b"", 0xcbf29ce484222325), (b"a", 0xaf63dc4c8601ec8c), (b"foobar", 0x85944171f73967e8)):
+        buf = ctypes.create_string_buffer(data, len(data)) if data else None
+        assert L.conv2d_weight_hash(buf, len(data), ctypes.byref(h)) == wc.OK and h.value == want
+    assert L.conv2d_weight_hash(None, 4, ctypes.byref(h)) == wc.ERR_NULL
+
+
+def test_weights_load_refuses_foreign_files_before_device_access(wc, tmp_path):
+    """Header checks run on the host only (the problem's pointers here are not device memory,
+    so any device access would fail with CONV2D_ERR_HIP instead)."""
+    import struct
+    L = wc.load_library()
+    n, c, h, w, k = 2, 40, 10, 10, 24
+    pb = wc.make_problem(n, c, h, w, k, ptrs=FAKE)
+    _, ub, _ = wc.workspace_size(pb)
+
+    def header(magic=b"WINOU16\0", version=1, method=wc.METHOD_WINOGRAD, kk=k, cc=c, r=3, s_=3, ubytes=ub, zero=0):
+        return struct.pack("<8sII4IQQQQ", magic, version, method, kk, cc, r, s_, 0, ubytes, 0, zero)
+
+    def load(data):
+        f = tmp_path / "u.bin"
+        f.write_bytes(data)
+        return L.conv2d_weights_load(ctypes.byref(pb), str(f).encode(), None)
+
+    assert L.conv2d_weights_load(ctypes.byref(pb), str(tmp_path / "missing.bin").encode(), None) == wc.ERR_IO
+    assert load(header()[:40]) == wc.ERR_IO                           # truncated header
+    assert load(header(magic=b"WINOU08\0")) == wc.ERR_MISMATCH
+    assert load(header(version=2)) == wc.ERR_MISMATCH
+    assert load(header(method=wc.METHOD_DIRECT)) == wc.ERR_MISMATCH
+    assert load(header(kk=k + 1)) == wc.ERR_MISMATCH
+    assert load(header(ubytes=ub - 1024)) == wc.ERR_MISMATCH
+    assert load(header(zero=1)) == wc.ERR_MISMATCH
+    assert load(header() + b"\0" * (ub - 1)) == wc.ERR_IO               # truncated payload
+    assert L.conv2d_weights_load(None, b"x", None) == wc.ERR_NULL
+    assert wc.load_library().conv2d_weights_save(None, b"x", None) == wc.ERR_NULL
+
+
 def test_form_choice_and_fused_slots(wc):
     """The benchmark layer (N*T*K*C = 1.6e9) stays staged, small layers take the fused form
     (slot 2 = the fused kernel, slots 3 and 4 empty), odd widths and the direct path stay
