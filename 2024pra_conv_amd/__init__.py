@@ -133,6 +133,7 @@ _SIGS = {
     "conv2d_batch_launch_graph": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "conv2d_batch_autotune": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_batch_layer_config": ([ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_int)], ctypes.c_int),
+"conv2d_batch_layer_form": ([ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_mlp_param_count": ([], ctypes.c_int),
     "conv2d_mlp_features": ([_P(Problem), ctypes.c_int, _P(ctypes.c_float)], ctypes.c_int),
     "conv2d_mlp_forward": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],

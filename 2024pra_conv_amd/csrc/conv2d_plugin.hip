@@ -165,14 +165,16 @@ int select_config(const problem_t* pb, const Geometry& g) {
     return heuristic_config(pb, g);
 }
 
-// Form choice on the Winograd path: the fused kernel where it measured faster (every layer
-// with N*T*K*C below 1.2e9 in a sweep of 20 shapes, by 1.2-2.4x; at and above the benchmark
-// layer's 1.6e9 the staged GEMM ties or wins), unless the shape was tuned (the tuned table
-// holds staged configurations).
-static bool use_fused(const problem_t* pb, const Geometry& g) {
+// Form choice on the Winograd path. The fused kernel recomputes V once per 64 output
+// channels instead of writing and reading it through HBM, so it wins when K is small (up to
+// 192: 1.2-2.4x on 112x112 and 56x56 layers with C = 256-512, ties elsewhere) and on layers too
+// small to fill the chip with staged GEMM tiles (N*T*K*C below 1.2e9: 1.2-2.4x); the staged
+// GEMM ties on the 56x56 C = K = 256 layer and wins on 28x28 C = K = 512 (1.35x). A tuned
+// shape keeps its staged configuration.
+bool use_fused(const problem_t* pb, const Geometry& g) {
     if (g.direct || !g.fused_ok || ((uintptr_t)pb->in & 3)) return false;
     if (tuned_config_lookup(pb) >= 0) return false;
-    return (double)g.NT * (double)pb->k * (double)pb->c < 1.2e9;
+    return pb->k <= 192 || (double)g.NT * (double)pb->k * (double)pb->c < 1.2e9;
 }
 
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {

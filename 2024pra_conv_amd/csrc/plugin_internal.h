@@ -54,5 +54,9 @@ int tuned_config_lookup(const problem_t* pb);
 // times_us[i] = best time of cfgs[i] (us) or -1 if gated out.
 int autotune_subset(const problem_t* pb, int reps, void* stream, const int* cfgs, int ncfg, int* best_cfg,
                     float* times_us);
+// Whether getkernelInfo runs this Winograd-path problem in the fused form.
+bool use_fused(const problem_t* pb, const Geometry& g);
+// Fastest of reps launches of L with argument a on stream st, in microseconds (one warm-up).
+int time_launch_us(const Launch& L, mykernelParamType* a, int reps, hipStream_t st, float* us);
 
 }  // namespace wino

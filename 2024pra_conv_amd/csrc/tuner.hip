@@ -148,6 +148,29 @@ struct TunerBuffers {
 
 namespace wino {
 
+int time_launch_us(const Launch& L, mykernelParamType* a, int reps, hipStream_t st, float* us) {
+    if (reps < 1 || !us) return CONV2D_ERR_INVALID;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CONV2D_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = CONV2D_ERR_HIP;
+    if (!rc) rc = launch(L, a, st);
+    float best = -1.f;
+    for (int r = 0; r < reps && !rc; ++r) {
+        float ms = 0.f;
+        if (hipEventRecord(e0, st) != hipSuccess || (rc = launch(L, a, st)) != CONV2D_OK ||
+            hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = rc ? rc : CONV2D_ERR_HIP;
+            break;
+        }
+        if (best < 0.f || ms < best) best = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (!rc) *us = best * 1e3f;
+    return rc;
+}
+
 int autotune_subset(const problem_t* pb, int reps, void* stream, const int* cfgs, int ncfg, int* best_cfg,
                     float* times_us) {
     if (!pb || !pb->out || !pb->transform_pin || !pb->transform_filter) return CONV2D_ERR_NULL;

@@ -1,5 +1,7 @@
 // Fused form: input transform, GEMM and output transform in one kernel (wino_fused.hip).
 #pragma once
 
-// Kernel for fnu staging units per thread (3 or 4).
+// Kernel for fnu staging units per thread (3 or 4), and its multi-layer (grouped) form with
+// arguments (const BatchLayer* layers, int nlayers, int total items).
 const void* fused_kernel_ptr(int nu);
+const void* fused_grouped_kernel_ptr(int nu);

@@ -116,8 +116,10 @@ __device__ inline void stage_store(const Staging<NU>& st, half_t* buf, int half_
 
 }  // namespace
 
+// One work item: 64 output channels (block kblk) x one 64-tile block (gblk, image-major) of the
+// problem in prm, by a workgroup of 256 threads, in the launch's dynamic LDS.
 template <int NU>
-__global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType prm) {
+__device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gblk) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -126,9 +128,6 @@ __global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType pr
     const int K = prm.k, C = prm.c;
     const int KB = prm.KB32, CB = prm.CB, C16 = 2 * CB;
     const int FBW = prm.fbw, PR = prm.fpr, PC = prm.fpc;
-    const int nk = (K + 63) / 64;
-    int kblk, gblk;
-    xcd_remap(blockIdx.x, gridDim.x, nk, &kblk, &gblk);
     const int nbw = prm.fnbw, nbh = prm.fnbh;
     const int img = gblk / (nbh * nbw);
     const int brem = gblk - img * (nbh * nbw);
@@ -303,9 +302,39 @@ __global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType pr
     }
 }
 
+template <int NU>
+__global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType prm) {
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, gridDim.x, (prm.k + 63) / 64, &kblk, &gblk);
+    fused_item<NU>(prm, kblk, gblk);
+}
+
+// Multi-layer form: one launch over the fused items of several layers (descriptor i owns items
+// [s3_first, s3_first + s3_items), nk = 64-channel blocks), every layer with NU staging units.
+template <int NU>
+__global__ void __launch_bounds__(256, 1)
+wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
+    int item, unused;
+    xcd_remap(blockIdx.x, total, total, &item, &unused);
+    int lo = 0, hi = nlayers - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (layers[mid].s3_first <= item) lo = mid; else hi = mid - 1;
+    }
+    const BatchLayer& bl = layers[lo];
+    const int local = item - bl.s3_first;
+    fused_item<NU>(bl.prm, local % bl.nk, local / bl.nk);
+}
+
 template __global__ void wino_fused_kernel<3>(mykernelParamType);
 template __global__ void wino_fused_kernel<4>(mykernelParamType);
+template __global__ void wino_fused_grouped_kernel<3>(const BatchLayer*, int, int);
+template __global__ void wino_fused_grouped_kernel<4>(const BatchLayer*, int, int);
 
 const void* fused_kernel_ptr(int nu) {
     return nu <= 3 ? (const void*)&wino_fused_kernel<3> : (const void*)&wino_fused_kernel<4>;
+}
+
+const void* fused_grouped_kernel_ptr(int nu) {
+    return nu <= 3 ? (const void*)&wino_fused_grouped_kernel<3> : (const void*)&wino_fused_grouped_kernel<4>;
 }

@@ -171,8 +171,12 @@ __device__ inline float epilogue(const mykernelParamType& prm, float y, float b)
     y += b;
     return (prm.act == 1 && y < 0.f) ? 0.f : y;
 }
+// Output and bias live in global memory: the address-space casts keep the accesses on the
+// global (not flat) path also when prm was read from memory (multi-layer descriptors).
+typedef __attribute__((address_space(1))) half_t global_half_t;
+
 __device__ inline float epilogue_bias(const mykernelParamType& prm, int k) {
-    return prm.bias ? (float)((const half_t*)prm.bias)[k] : 0.f;
+    return prm.bias ? (float)((const global_half_t*)prm.bias)[k] : 0.f;
 }
 
 __device__ inline void store_tile(const mykernelParamType& prm, const TileCoord& tc, int k, float y00, float y01,
@@ -185,7 +189,7 @@ __device__ inline void store_tile(const mykernelParamType& prm, const TileCoord&
         y11 = epilogue(prm, y11, b);
     }
     const int Oh = prm.Oh, Ow = prm.Ow;
-    half_t* o = (half_t*)prm.pout + (((size_t)tc.img * prm.k + k) * Oh + tc.oh) * Ow + tc.ow;
+    global_half_t* o = (global_half_t*)prm.pout + (((size_t)tc.img * prm.k + k) * Oh + tc.oh) * Ow + tc.ow;
     o[0] = (half_t)y00;
     if (tc.c1) o[1] = (half_t)y01;
     if (tc.r1) {
@@ -230,7 +234,7 @@ __device__ inline void store_plain32(const mykernelParamType& prm, const floatx1
         if (g >= prm.NT) continue;
         const int img = g / prm.T;
         const int pix = g - img * prm.T;
-        half_t* o = (half_t*)prm.pout + (size_t)img * prm.k * prm.T + pix;
+        global_half_t* o = (global_half_t*)prm.pout + (size_t)img * prm.k * prm.T + pix;
 #pragma unroll
         for (int mi = 0; mi < MI; ++mi)
 #pragma unroll

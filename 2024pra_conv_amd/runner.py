@@ -278,6 +278,14 @@ class Batch:
             out.append(c.value)
         return out
 
+    def layer_forms(self) -> list[int]:
+        out = []
+        for i in range(len(self.convs)):
+            f = ctypes.c_int()
+            check(self.L.conv2d_batch_layer_form(self.h, i, ctypes.byref(f)), "conv2d_batch_layer_form")
+            out.append(f.value)
+        return out
+
 
 def batch_layer_shapes(seed: int = 4, count: int = 24):
     """The 24-layer batched workload: seeded shapes N in {1,4,16}, C,K in

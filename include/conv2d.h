@@ -192,8 +192,8 @@ int conv2d_weight_hash(const void* host_bytes, size_t nbytes, unsigned long long
 
 /* Which form getkernelInfo and conv2d_forward(-1) use for a Winograd-path problem:
  * CONV2D_FORM_FUSED when the fused form applies, the shape has no tuned-table entry and
- * N*T*K*C < 1.2e9 (T = 2x2 tiles per image; where the fused form measured faster on MI355X,
- * 1.2-2.4x on layers up to that size), else CONV2D_FORM_STAGED (also for the direct path). */
+ * K <= 192 or N*T*K*C < 1.2e9 (T = 2x2 tiles per image; where the fused form measured faster
+ * on MI355X, by up to 2.4x), else CONV2D_FORM_STAGED (also for the direct path). */
 #define CONV2D_FORM_STAGED 0
 #define CONV2D_FORM_FUSED 1
 int conv2d_form(const problem_t* problem, int* form);
@@ -287,6 +287,9 @@ int conv2d_batch_launch_graph(conv2d_batch* batch, void* stream);
 int conv2d_batch_autotune(conv2d_batch* batch, int reps, void* stream);
 /* Tile configuration (global index) used for layer i. */
 int conv2d_batch_layer_config(const conv2d_batch* batch, int i, int* cfg);
+/* Form of layer i in the batch (CONV2D_FORM_STAGED or CONV2D_FORM_FUSED): chosen as
+ * conv2d_form does, or by conv2d_batch_autotune from measured times. */
+int conv2d_batch_layer_form(const conv2d_batch* batch, int i, int* form);
 
 /* ---- Learned cost model (no counterpart in the reference): a 21 -> 64 -> 64 -> 1 FP32 MLP
  * predicting log(GEMM time, us) from log2(n, c, h, w, k) and a one-hot of the tile

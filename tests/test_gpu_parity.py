@@ -307,6 +307,9 @@ def test_batched_equals_separate_and_graph_replays(runner, cuda):
 
 
 def test_batched_autotune_keeps_results(runner, wc, cuda):
+    """After autotuning (tile configurations, and fused or staged per layer) the batched form
+    still equals separate launches in the same forms, and layers whose form did not change
+    keep their bits (every staged configuration gives the same output)."""
     _log("case batched autotune")
     convs = _batch_convs(runner, cuda, BATCH_SMALL[:4], seed=700)
     b = runner.Batch(convs)
@@ -314,11 +317,55 @@ def test_batched_autotune_keeps_results(runner, wc, cuda):
     b.forward_separate()
     torch.cuda.synchronize()
     ref = [c.out.clone() for c in convs]
+    forms0 = b.layer_forms()
     b.autotune(reps=2)
-    cfgs = b.layer_configs()
+    cfgs, forms = b.layer_configs(), b.layer_forms()
+    _log(f"  configs {cfgs} forms {forms0} -> {forms}")
     assert all(0 <= c < len(wc.tile_configs()) for c in cfgs)
+    assert all(f in (wc.FORM_STAGED, wc.FORM_FUSED) for f in forms)
+    b.forward()
+    torch.cuda.synchronize()
+    got = [c.out.clone() for c in convs]
+    b.forward_separate()
+    torch.cuda.synchronize()
+    for i, c in enumerate(convs):
+        assert torch.equal(got[i].view(torch.int16), c.out.view(torch.int16)), f"layer {i}"
+        if forms[i] == forms0[i]:
+            assert torch.equal(got[i].view(torch.int16), ref[i].view(torch.int16)), f"layer {i} changed bits"
+        y64 = c.direct_f64()
+        assert ((got[i].double() - y64).abs().max() / y64.abs().max()).item() <= 1e-2, f"layer {i}"
+    b.close()
+
+
+def test_batched_mixes_fused_and_staged_layers(runner, wc, cuda):
+    """A batch with both forms (even-width layers fused, odd-width layers staged) equals separate
+    launches, in eager and graph launches."""
+    _log("case batched mixed forms")
+    shapes = [(1, 64, 14, 64), (2, 256, 56, 256), (1, 96, 27, 128), (3, 130, 13, 140), (1, 16, 28, 16)]
+    convs = _batch_convs(runner, cuda, shapes, seed=800)
+    b = runner.Batch(convs)
+    forms = b.layer_forms()
+    _log(f"  forms {forms}")
+    assert wc.FORM_FUSED in forms and wc.FORM_STAGED in forms
+    b.prepare_weights()
+    b.forward_separate()
+    torch.cuda.synchronize()
+    ref = [c.out.clone() for c in convs]
+    for c in convs:
+        c.out.fill_(float("nan"))
     b.forward()
     torch.cuda.synchronize()
     for i, c in enumerate(convs):
         assert torch.equal(c.out.view(torch.int16), ref[i].view(torch.int16)), f"layer {i}"
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b.capture_graph(s)
+    for rep in range(2):
+        for c in convs:
+            c.out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        b.launch_graph(s)
+        s.synchronize()
+        for i, c in enumerate(convs):
+            assert torch.equal(c.out.view(torch.int16), ref[i].view(torch.int16)), f"replay {rep} layer {i}"
     b.close()
