@@ -13,6 +13,7 @@ convolution (GPU validation kernel), and the reference's CPU checker timed on th
 host on one image of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --layer N C HW K [--steps K] [--copy-mib M]   (one layer, for profiles)
 """
 from __future__ import annotations
 
@@ -245,6 +246,29 @@ def measured_traffic(cfg: int):
     return int(m.group(2))
 
 
+def layer_mode(args) -> int:
+    """One layer, in the library's chosen form, for counter profiles (not a benchmark line)."""
+    import torch
+    runner = importlib.import_module("2024pra_conv_amd.runner")
+    wc = importlib.import_module("2024pra_conv_amd")
+    dev = torch.device("cuda:0")
+    if args.copy_mib > 0:
+        src = torch.empty(args.copy_mib * 2**20, dtype=torch.uint8, device=dev)
+        torch.empty_like(src).copy_(src)
+    n, c, h, k = args.layer
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20245)
+    x = torch.randn((n, c, h, h), generator=gen, device=dev).half()
+    wt = (torch.randn((k, c, 3, 3), generator=gen, device=dev) * (2.0 / (9 * c)) ** 0.5).half()
+    conv = runner.WinogradConv(x, wt)
+    conv.prepare_weights()
+    t = _events_time(lambda: conv.forward(-1), max(1, args.steps))
+    form = conv.form()
+    print(json.dumps({"layer": f"N={n} C={c} H=W={h} K={k}", "form": "fused" if form == wc.FORM_FUSED else "staged",
+                      "us": round(t * 1e3, 2), "runs": max(1, args.steps) + 1}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,7 +280,12 @@ def main() -> int:
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before warm-up, run one device-to-device copy of this many MiB (untimed): the "
                          "known-size kernel that hardware byte counters are calibrated against")
+    ap.add_argument("--layer", type=int, nargs=4, metavar=("N", "C", "HW", "K"), default=None,
+                    help="profile mode: run one 3x3 pad-1 layer (seeded N(0,1) input, N(0, 2/(9C)) filters) "
+                         "--steps times in the form the library chooses for it, print one JSON line")
     args = ap.parse_args()
+    if args.layer is not None:
+        return layer_mode(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(the natural multi-GPU split, by images, needs no exchange).", file=sys.stderr)
