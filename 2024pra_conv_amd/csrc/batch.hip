@@ -22,6 +22,7 @@ struct conv2d_batch {
     struct Segment {
         int cfg;      // staged: GEMM tile configuration of every layer in the segment
         int nu;       // fused: staging units per thread (0 for a staged segment)
+        bool d16;     // fused: 2-byte staging loads
         int first;    // first descriptor of the segment
         int count;    // layers in the segment
         int items;    // work items of the segment's queue
@@ -55,7 +56,9 @@ int upload(conv2d_batch* b) {
         int rc = wino::require_winograd(&b->layers[i], &geo[i]);
         if (rc) return rc;
     }
-    auto key = [&](int i) { return b->form[i] == CONV2D_FORM_FUSED ? 1000 + geo[i].fnu : b->cfg[i]; };
+    auto key = [&](int i) {
+        return b->form[i] == CONV2D_FORM_FUSED ? 1000 + 2 * geo[i].fnu + (geo[i].fd16 ? 1 : 0) : b->cfg[i];
+    };
     auto cost = [&](int i) {
         if (b->form[i] == CONV2D_FORM_FUSED) return (double)b->layers[i].c;
         const GemmConfig& c = kGemmConfigs[b->cfg[i]];
@@ -79,8 +82,10 @@ int upload(conv2d_batch* b) {
         const int cfg = b->cfg[i];
         const bool fused = b->form[i] == CONV2D_FORM_FUSED;
         const int nu = fused ? g.fnu : 0;
-        if (b->segs.empty() || b->segs.back().nu != nu || (!fused && b->segs.back().cfg != cfg))
-            b->segs.push_back(conv2d_batch::Segment{cfg, nu, qi, 0, 0, 0u});
+        const bool d16 = fused && g.fd16;
+        if (b->segs.empty() || b->segs.back().nu != nu || b->segs.back().d16 != d16 ||
+            (!fused && b->segs.back().cfg != cfg))
+            b->segs.push_back(conv2d_batch::Segment{cfg, nu, d16, qi, 0, 0, 0u});
         conv2d_batch::Segment& seg = b->segs.back();
         BatchLayer& bl = b->host[qi];
         wino::fill_param(&pb, g, cfg, &bl.prm);
@@ -145,7 +150,7 @@ int launch_forward(conv2d_batch* b, hipStream_t st) {
         int count = seg.count, items = seg.items;
         void* args[3] = {(void*)&layers, (void*)&count, (void*)&items};
         const hipError_t e =
-            seg.nu ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.nu), dim3(seg.items), dim3(256), args, seg.lds, st)
+            seg.nu ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.nu, seg.d16), dim3(seg.items), dim3(256), args, seg.lds, st)
                    : hipLaunchKernel(kGemmConfigs[seg.cfg].grouped_kernel, dim3(seg.items),
                                      dim3(kGemmConfigs[seg.cfg].threads), args, 0, st);
         if (e != hipSuccess) return CONV2D_ERR_HIP;
@@ -281,7 +286,7 @@ int conv2d_batch_autotune(conv2d_batch* b, int reps, void* stream) {
         b->form[i] = CONV2D_FORM_STAGED;
         wino::Geometry g;
         if ((rc = wino::require_winograd(pb, &g))) return rc;
-        if (!g.fused_ok || ((uintptr_t)pb->in & 3)) continue;
+        if (!g.fused_ok) continue;
         mykernelParamType a;
         wino::fill_param(pb, g, best, &a);
         float t_s2 = 0.f, t_fused = 0.f;

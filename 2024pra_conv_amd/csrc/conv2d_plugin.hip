@@ -80,7 +80,7 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         g->band = 0;
         g->v_bytes = (size_t)steps * 2 * (size_t)g->G32 * kFragHalves * 2;
         g->u_bytes = (size_t)steps * 2 * (size_t)g->KB32 * kFragHalves * 2;
-    } else if (pb->w % 2 == 0) {
+    } else {
         // fused form: the 64-tile block shape that pads the tile grid least (ties: the
         // squarer block, whose input patch is smaller)
         const int shapes[4][2] = {{8, 8}, {4, 16}, {16, 4}, {2, 32}};
@@ -97,6 +97,7 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         g->fnbh = (g->tiles_h + g->fbh - 1) / g->fbh;
         g->fnbw = (g->tiles_w + g->fbw - 1) / g->fbw;
         g->fsh = (int)(pb->q & 1);
+        g->fd16 = (pb->w % 2) != 0 || ((uintptr_t)pb->in & 3) != 0;
         g->fpr = 2 * g->fbh + 2;
         g->fpc = (2 * g->fbw + 2 + g->fsh + 1) & ~1;
         const int units = 4 * g->fpr * (g->fpc / 2);
@@ -169,12 +170,17 @@ int select_config(const problem_t* pb, const Geometry& g) {
 // channels instead of writing and reading it through HBM, so it wins when K is small (up to
 // 192: 1.2-2.4x on 112x112 and 56x56 layers with C = 256-512, ties elsewhere) and on layers too
 // small to fill the chip with staged GEMM tiles (N*T*K*C below 1.2e9: 1.2-2.4x); the staged
-// GEMM ties on the 56x56 C = K = 256 layer and wins on 28x28 C = K = 512 (1.35x). A tuned
-// shape keeps its staged configuration.
+// GEMM ties on the 56x56 C = K = 256 layer and wins on 28x28 C = K = 512 (1.35x). Odd widths
+// need 2-byte staging loads and are taken only when small. A tuned shape keeps its staged
+// configuration.
 bool use_fused(const problem_t* pb, const Geometry& g) {
-    if (g.direct || !g.fused_ok || ((uintptr_t)pb->in & 3)) return false;
+    if (g.direct || !g.fused_ok) return false;
     if (tuned_config_lookup(pb) >= 0) return false;
-    return pb->k <= 192 || (double)g.NT * (double)pb->k * (double)pb->c < 1.2e9;
+    const double work = (double)g.NT * (double)pb->k * (double)pb->c;
+    // 2-byte staging (odd width) costs the kernel registers: it measured faster only on small
+    // layers with C <= 256 (1.3-2.7x; 0.7-0.95x on 27x27 and 55x55 layers above 1.5e8 or C = 512)
+    if (g.fd16 && (pb->c > 256 || work >= 1.5e8)) return false;
+    return pb->k <= 192 || work < 1.2e9;
 }
 
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {
@@ -249,7 +255,7 @@ Launch slot_filter(const Geometry& g) {
 // input patches during the channel loop and the 128 KiB Z exchange after it.
 Launch slot_fused(const Geometry& g) {
     Launch L;
-    L.kernel = fused_kernel_ptr(g.fnu);
+    L.kernel = fused_kernel_ptr(g.fnu, g.fd16);
     L.grid = dim3((unsigned)g.fitems, 1, 1);
     L.block = dim3(256, 1, 1);
     const unsigned patches = 2u * 4u * (unsigned)g.fpr * (unsigned)g.fpc * 16u;
@@ -494,7 +500,7 @@ int conv2d_fused_gemm(const problem_t* pb, void* stream) {
     if (rc) return rc;
     Geometry g;
     if ((rc = require_winograd(pb, &g))) return rc;
-    if (!g.fused_ok || ((uintptr_t)pb->in & 3)) return CONV2D_ERR_UNSUPPORTED;
+    if (!g.fused_ok) return CONV2D_ERR_UNSUPPORTED;
     mykernelParamType a;
     fill_param(pb, g, 0, &a);
     return launch(slot_fused(g), &a, (hipStream_t)stream);

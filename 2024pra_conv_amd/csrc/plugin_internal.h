@@ -21,6 +21,7 @@ struct Geometry {
     size_t v_bytes, u_bytes;
     // fused form: block of fbh x fbw tiles, fnbh x fnbw blocks per image (fused_ok: even w)
     bool fused_ok;
+    bool fd16;  // 2-byte staging loads (odd width, or an input base not 4-byte aligned)
     int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
     long long fitems;
 };

@@ -41,17 +41,19 @@ __device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t 
 }
 
 // Staging of one thread: up to NU units, a unit = (channel group g, patch row, pixel pair) =
-// 8 channels x 2 adjacent pixels, one 4-byte buffer load per channel. Pixel pairs start on
-// even input columns and W is even, so a pair is entirely inside or outside the image and
-// its offset is 4-byte aligned. Loads are never predicated: a pair outside the image gets
-// a byte offset of 2^31 or more, and a channel >= C lands at or beyond the image's last byte;
-// both are past the resource's range (the image, C*H*W*2 < 2^31 bytes), where buffer loads
-// return zero without accessing memory.
+// 8 channels x 2 adjacent pixels. With W even (D16 = false) a pair starts on an even input
+// column, so it is entirely inside or outside the image and its offset is 4-byte aligned: one
+// 4-byte buffer load per channel. With W odd or a base that is not 4-byte aligned (D16) each
+// pixel is a 2-byte load with its own offset. Loads are never predicated: a pixel outside the
+// image gets a byte offset of 2^31 or more, and a channel >= C lands at or beyond the image's
+// last byte; both are past the resource's range (the image, C*H*W*2 < 2^31 bytes), where
+// buffer loads return zero without accessing memory.
 template <int NU>
 struct Staging {
-    uint32_t off[NU];   // byte offset of the unit's pair in channel g*8 of chunk 0
+    uint32_t off[NU];   // byte offset of the unit's first pixel in channel g*8 of chunk 0
+    uint32_t off1[NU];  // D16: byte offset of its second pixel
     int lds[NU];        // LDS offset (halves) of the unit's first pixel
-    uint32_t w[NU][8];  // loaded pairs
+    uint32_t w[NU][8];  // loaded pairs (first pixel in the low half)
 };
 
 // FP16 x - y as fma(y, m1, x) with m1 = -1: y * -1 is exact, so the single rounding of the
@@ -63,7 +65,7 @@ __device__ inline half8 sub8(half8 x, half8 y, half8 m1) { return __builtin_elem
 __device__ inline uint32_t pack_lo(uint32_t x, uint32_t y) { return __builtin_amdgcn_perm(y, x, 0x05040100u); }
 __device__ inline uint32_t pack_hi(uint32_t x, uint32_t y) { return __builtin_amdgcn_perm(y, x, 0x07060302u); }
 
-template <int NU>
+template <int NU, bool D16>
 __device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw0, Staging<NU>& st) {
     const int PR = prm.fpr, PC = prm.fpc, PH = PC >> 1;
     const int nunits = 4 * PR * PH;
@@ -76,23 +78,38 @@ __device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw
         const int pr = rem / PH;
         const int m = rem - pr * PH;
         const int ih = ih0 + pr, iw = iw0 + 2 * m;
-        const bool pix = u < nunits && ih >= 0 && ih < H && iw >= 0 && iw < W;
+        const bool row = u < nunits && ih >= 0 && ih < H;
+        const bool pix = row && iw >= 0 && iw < W;
         st.off[i] = pix ? (uint32_t)(((g * 8 * H + ih) * W + iw) * 2) : 0x80000000u;
+        if (D16) {
+            const bool pix1 = row && iw + 1 >= 0 && iw + 1 < W;
+            st.off1[i] = pix1 ? (uint32_t)(((g * 8 * H + ih) * W + iw + 1) * 2) : 0x80000000u;
+        }
         // pair m: pixel 2m at column parity 0, pixel 2m+1 at parity 1, both at column m; a
         // thread slot without a unit stores to a dummy pixel pair past both patch buffers
         st.lds[i] = u < nunits ? ((g * PR + pr) * PC + m) * 8 : 2 * 4 * PR * PC * 8;
     }
 }
 
-template <int NU>
+template <int NU, bool D16>
 __device__ inline void stage_load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes,
                                   Staging<NU>& st) {
 #pragma unroll
     for (int i = 0; i < NU; ++i) {
         const uint32_t base = st.off[i] + chunk_bytes;
+        if (D16) {
+            const uint32_t base1 = st.off1[i] + chunk_bytes;
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
-            st.w[i][e] = __builtin_amdgcn_raw_buffer_load_b32(in, (int)(base + e * chan_bytes), 0, 0);
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b16(in, (int)(base + e * chan_bytes), 0, 0);
+                const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b16(in, (int)(base1 + e * chan_bytes), 0, 0);
+                st.w[i][e] = lo | (hi << 16);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                st.w[i][e] = __builtin_amdgcn_raw_buffer_load_b32(in, (int)(base + e * chan_bytes), 0, 0);
+        }
     }
 }
 
@@ -118,7 +135,7 @@ __device__ inline void stage_store(const Staging<NU>& st, half_t* buf, int half_
 
 // One work item: 64 output channels (block kblk) x one 64-tile block (gblk, image-major) of the
 // problem in prm, by a workgroup of 256 threads, in the launch's dynamic LDS.
-template <int NU>
+template <int NU, bool D16>
 __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gblk) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
     const int tid = threadIdx.x;
@@ -195,8 +212,8 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
                 for (int e = 0; e < 16; ++e) acc[b][m][tb][e] = 0.f;
 
     Staging<NU> st;
-    stage_setup<NU>(prm, ih0, iw0, st);
-    stage_load<NU>(in_rsrc, 0u, chan_bytes, st);
+    stage_setup<NU, D16>(prm, ih0, iw0, st);
+    stage_load<NU, D16>(in_rsrc, 0u, chan_bytes, st);
     stage_store<NU>(st, lds, PC / 2);
     half8 ua[16], ub[16];
     load_u(ua, 0);
@@ -205,7 +222,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     auto chunk = [&](int ch, half8 (&ucur)[16], half8 (&unext)[16]) {
         const bool more = ch + 1 < CB;
         if (more) {
-            stage_load<NU>(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes, st);
+            stage_load<NU, D16>(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes, st);
             load_u(unext, ch + 1);
         }
         const half_t* buf = lds + (ch & 1) * PB;
@@ -302,16 +319,17 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     }
 }
 
-template <int NU>
+template <int NU, bool D16>
 __global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType prm) {
     int kblk, gblk;
     xcd_remap(blockIdx.x, gridDim.x, (prm.k + 63) / 64, &kblk, &gblk);
-    fused_item<NU>(prm, kblk, gblk);
+    fused_item<NU, D16>(prm, kblk, gblk);
 }
 
 // Multi-layer form: one launch over the fused items of several layers (descriptor i owns items
-// [s3_first, s3_first + s3_items), nk = 64-channel blocks), every layer with NU staging units.
-template <int NU>
+// [s3_first, s3_first + s3_items), nk = 64-channel blocks), every layer with NU staging units
+// and the same staging load width.
+template <int NU, bool D16>
 __global__ void __launch_bounds__(256, 1)
 wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
     int item, unused;
@@ -323,18 +341,24 @@ wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, in
     }
     const BatchLayer& bl = layers[lo];
     const int local = item - bl.s3_first;
-    fused_item<NU>(bl.prm, local % bl.nk, local / bl.nk);
+    fused_item<NU, D16>(bl.prm, local % bl.nk, local / bl.nk);
 }
 
-template __global__ void wino_fused_kernel<3>(mykernelParamType);
-template __global__ void wino_fused_kernel<4>(mykernelParamType);
-template __global__ void wino_fused_grouped_kernel<3>(const BatchLayer*, int, int);
-template __global__ void wino_fused_grouped_kernel<4>(const BatchLayer*, int, int);
+#define FUSED_INSTANCES(NU, D16)                                                            \
+    template __global__ void wino_fused_kernel<NU, D16>(mykernelParamType);                 \
+    template __global__ void wino_fused_grouped_kernel<NU, D16>(const BatchLayer*, int, int);
+FUSED_INSTANCES(3, false)
+FUSED_INSTANCES(4, false)
+FUSED_INSTANCES(3, true)
+FUSED_INSTANCES(4, true)
 
-const void* fused_kernel_ptr(int nu) {
-    return nu <= 3 ? (const void*)&wino_fused_kernel<3> : (const void*)&wino_fused_kernel<4>;
+const void* fused_kernel_ptr(int nu, bool d16) {
+    if (d16) return nu <= 3 ? (const void*)&wino_fused_kernel<3, true> : (const void*)&wino_fused_kernel<4, true>;
+    return nu <= 3 ? (const void*)&wino_fused_kernel<3, false> : (const void*)&wino_fused_kernel<4, false>;
 }
 
-const void* fused_grouped_kernel_ptr(int nu) {
-    return nu <= 3 ? (const void*)&wino_fused_grouped_kernel<3> : (const void*)&wino_fused_grouped_kernel<4>;
+const void* fused_grouped_kernel_ptr(int nu, bool d16) {
+    if (d16)
+        return nu <= 3 ? (const void*)&wino_fused_grouped_kernel<3, true> : (const void*)&wino_fused_grouped_kernel<4, true>;
+    return nu <= 3 ? (const void*)&wino_fused_grouped_kernel<3, false> : (const void*)&wino_fused_grouped_kernel<4, false>;
 }

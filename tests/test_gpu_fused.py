@@ -26,6 +26,11 @@ SHAPES = [
     (1, 33, 70, 6, 16, 1, 0),      # tall: 35x2 tiles -> 16x4 blocks, q = 0
     (2, 96, 28, 28, 256, 1, 1),    # four k-blocks, ragged blocks of a 14x14 tile grid
     (1, 8, 4, 130, 32, 1, 1),      # 2x65 tiles -> 2x32 blocks (four staging units a thread)
+    # odd widths: 2-byte staging loads, pixel pairs straddling the right edge
+    (1, 16, 9, 11, 8, 1, 1),
+    (2, 96, 27, 27, 96, 1, 1),
+    (1, 40, 13, 13, 20, 0, 0),
+    (1, 8, 4, 131, 32, 2, 2),      # odd width with 2x32 blocks and pad 2
 ]
 
 
