@@ -47,6 +47,17 @@ def _events_time(fn, iters: int, stream=None) -> float:
     return e0.elapsed_time(e1) / iters
 
 
+def clock_warmup(fn, min_ms: float = 60.0, chunk: int = 20, max_calls: int = 4000) -> int:
+    """Runs fn in chunks until min_ms of GPU time has passed, so the timed steps see the
+    device's sustained clocks (from idle, the step time settles over about 20 ms: 175-180 us
+    for the first steps, 161 us after). Returns the number of calls made."""
+    calls, spent = 0, 0.0
+    while spent < min_ms and calls < max_calls:
+        spent += _events_time(fn, chunk) * chunk
+        calls += chunk
+    return calls
+
+
 def cpu_baseline(n, c, h, w, k) -> dict:
     """The reference's CPU checker conv2dcpu (include/verfiy.h:5-43), compiled from its source
     into oracle/_ref when available, else the oracle's faithful C restatement, timed on one
@@ -118,6 +129,7 @@ def fp8_section(runner, dev, steps: int) -> dict:
     useful = 32.0 * n * k * c * T
     c8.prepare_weights()
     c16.prepare_weights()
+    clock_warmup(lambda: c16.forward(cfg16), min_ms=30.0)
     t16_full = _events_time(lambda: (c16.prepare_weights(), c16.forward(cfg16)), steps)
     t8_full = _events_time(lambda: (c8.prepare_weights(), c8.forward(cfg8)), steps)
     t16 = _events_time(lambda: c16.forward(cfg16), steps)
@@ -165,6 +177,7 @@ def batched_section(runner, dev, steps: int) -> dict:
         b.prepare_weights(s)
         b.autotune(reps=3, stream=s)
         b.capture_graph(s)
+        clock_warmup(lambda: b.launch_graph(s), min_ms=30.0, chunk=5)
         t_graph = _events_time(lambda: b.launch_graph(s), steps, s)
         t_persist = _events_time(lambda: b.forward(s), steps, s)
         t_sep = _events_time(lambda: b.forward_separate(s), steps, s)
@@ -208,6 +221,7 @@ def direct_section(runner, dev, steps: int) -> dict:
         flops = 2.0 * n * k * c * r * r * oh * oh
         cfg = conv.selected_config()
         conv.prepare_weights()
+        clock_warmup(lambda: conv.forward(cfg), min_ms=30.0)
         t_full = _events_time(lambda: (conv.prepare_weights(), conv.forward(cfg)), steps)
         t_fwd = _events_time(lambda: conv.forward(cfg), steps)
         t_gemm = _events_time(lambda: conv.gemm(cfg), steps)
@@ -275,6 +289,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-clock-warmup", dest="clock_warmup", action="store_false",
+                    help="skip the extra untimed steps (>= 60 ms of GPU time) run after the W warm-up steps "
+                         "so that timing starts at sustained clocks")
     ap.add_argument("--cfg", type=int, default=-1, help="GEMM tile configuration (-1 = library choice)")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP8, batched and direct sections")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -323,6 +340,7 @@ def main() -> int:
         del src, dst
     for _ in range(args.warmup):
         step()
+    extra_warmup = clock_warmup(step) if args.clock_warmup else 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ms = _events_time(step, args.steps)
@@ -370,6 +388,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "clock_warmup_steps": extra_warmup,
         "ms_per_step": round(ms, 5),
         "higher_is_better": True,
         "scaling": "weak",
