@@ -21,7 +21,7 @@
 struct conv2d_batch {
     struct Segment {
         int cfg;      // staged: GEMM tile configuration of every layer in the segment
-        int nu;       // fused: staging units per thread (0 for a staged segment)
+        int fused;    // 1 for a fused segment, 0 for a staged one
         bool d16;     // fused: 2-byte staging loads
         int first;    // first descriptor of the segment
         int count;    // layers in the segment
@@ -57,7 +57,7 @@ int upload(conv2d_batch* b) {
         if (rc) return rc;
     }
     auto key = [&](int i) {
-        return b->form[i] == CONV2D_FORM_FUSED ? 1000 + 2 * geo[i].fnu + (geo[i].fd16 ? 1 : 0) : b->cfg[i];
+        return b->form[i] == CONV2D_FORM_FUSED ? 1000 + (geo[i].fd16 ? 1 : 0) : b->cfg[i];
     };
     auto cost = [&](int i) {
         if (b->form[i] == CONV2D_FORM_FUSED) return (double)b->layers[i].c;
@@ -81,11 +81,11 @@ int upload(conv2d_batch* b) {
         const wino::Geometry& g = geo[i];
         const int cfg = b->cfg[i];
         const bool fused = b->form[i] == CONV2D_FORM_FUSED;
-        const int nu = fused ? g.fnu : 0;
+        const int fz = fused ? 1 : 0;
         const bool d16 = fused && g.fd16;
-        if (b->segs.empty() || b->segs.back().nu != nu || b->segs.back().d16 != d16 ||
+        if (b->segs.empty() || b->segs.back().fused != fz || b->segs.back().d16 != d16 ||
             (!fused && b->segs.back().cfg != cfg))
-            b->segs.push_back(conv2d_batch::Segment{cfg, nu, d16, qi, 0, 0, 0u});
+            b->segs.push_back(conv2d_batch::Segment{cfg, fz, d16, qi, 0, 0, 0u});
         conv2d_batch::Segment& seg = b->segs.back();
         BatchLayer& bl = b->host[qi];
         wino::fill_param(&pb, g, cfg, &bl.prm);
@@ -150,7 +150,7 @@ int launch_forward(conv2d_batch* b, hipStream_t st) {
         int count = seg.count, items = seg.items;
         void* args[3] = {(void*)&layers, (void*)&count, (void*)&items};
         const hipError_t e =
-            seg.nu ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.nu, seg.d16), dim3(seg.items), dim3(256), args, seg.lds, st)
+            seg.fused ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.d16), dim3(seg.items), dim3(fused_threads()), args, seg.lds, st)
                    : hipLaunchKernel(kGemmConfigs[seg.cfg].grouped_kernel, dim3(seg.items),
                                      dim3(kGemmConfigs[seg.cfg].threads), args, 0, st);
         if (e != hipSuccess) return CONV2D_ERR_HIP;

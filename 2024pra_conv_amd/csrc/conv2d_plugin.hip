@@ -101,12 +101,12 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         g->fpr = 2 * g->fbh + 2;
         g->fpc = (2 * g->fbw + 2 + g->fsh + 1) & ~1;
         const int units = 4 * g->fpr * (g->fpc / 2);
-        g->fnu = (units + 255) / 256;
+        g->fnu = (units + kFusedThreads - 1) / kFusedThreads;
         g->fitems = (long long)pb->n * g->fnbh * g->fnbw * ((pb->k + 63) / 64);
         // buffer-load offsets are 32-bit: one image with 32 channels of slack under 2^31
         // bytes (out-of-image offsets are placed at 2^31 and above), U under 2^31 bytes
         const long long img_bytes = ((long long)pb->c + 32) * pb->h * pb->w * 2;
-        g->fused_ok = g->fnu <= 4 && g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
+        g->fused_ok = g->fnu <= 2 && g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
                       (long long)g->u_bytes < (1LL << 31);
     }
     return CONV2D_OK;
@@ -176,11 +176,11 @@ int select_config(const problem_t* pb, const Geometry& g) {
 bool use_fused(const problem_t* pb, const Geometry& g) {
     if (g.direct || !g.fused_ok) return false;
     if (tuned_config_lookup(pb) >= 0) return false;
+    // the staged GEMM gains on the fused kernel as K and N*T*K*C grow: measured crossover
+    // between 8.2e8 and 9.2e8 for K >= 256 (both staging widths); for K <= 192 the fused form
+    // was faster on every measured shape (up to 1.6e9)
     const double work = (double)g.NT * (double)pb->k * (double)pb->c;
-    // 2-byte staging (odd width) costs the kernel registers: it measured faster only on small
-    // layers with C <= 256 (1.3-2.7x; 0.7-0.95x on 27x27 and 55x55 layers above 1.5e8 or C = 512)
-    if (g.fd16 && (pb->c > 256 || work >= 1.5e8)) return false;
-    return pb->k <= 192 || work < 1.2e9;
+    return pb->k <= 192 || work < 8e8;
 }
 
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {
@@ -255,9 +255,9 @@ Launch slot_filter(const Geometry& g) {
 // input patches during the channel loop and the 128 KiB Z exchange after it.
 Launch slot_fused(const Geometry& g) {
     Launch L;
-    L.kernel = fused_kernel_ptr(g.fnu, g.fd16);
+    L.kernel = fused_kernel_ptr(g.fd16);
     L.grid = dim3((unsigned)g.fitems, 1, 1);
-    L.block = dim3(256, 1, 1);
+    L.block = dim3((unsigned)fused_threads(), 1, 1);
     const unsigned patches = 2u * 4u * (unsigned)g.fpr * (unsigned)g.fpc * 16u;
     L.lds = patches > 131072u ? patches : 131072u;
     return L;

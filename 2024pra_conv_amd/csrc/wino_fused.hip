@@ -7,10 +7,12 @@
 // B-operand layout, from an LDS-staged input patch, and U (prepared by the filter
 // transform) is read from global memory (L2) straight into A-operand registers.
 //
-// Workgroup: 4 waves (one per SIMD, up to 512 registers each), 64 output channels x one
-// block of FBH x FBW = 64 tiles of one image. Wave a owns the Winograd positions of row a of
-// the 4x4 transform domain (pos = 4a + b, b = 0..3) for the whole 64 x 64 tile: 16 FP32
-// 32x32 accumulators (4 positions x 2 channel blocks x 2 tile blocks = 256 registers).
+// Workgroup: 8 waves (two per SIMD, up to 256 registers each), 64 output channels x one
+// block of FBH x FBW = 64 tiles of one image. Wave w = 4*BH + a owns the Winograd positions
+// pos = 4a + 2*BH + bb (bb = 0, 1) of row a of the 4x4 transform domain for the whole
+// 64 x 64 tile: 8 FP32 32x32 accumulators (2 positions x 2 channel blocks x 2 tile blocks =
+// 128 registers). Two waves per SIMD let one wave's loads, LDS reads and transform overlap
+// the other's MFMAs.
 //
 // Per 32-channel chunk:
 //   patch:  input rows 2*th0 - p .. +2*FBH+1 and columns from iw0 = 2*tw0 - q - fsh, all 32
@@ -20,16 +22,18 @@
 //           a wave's reads are free of bank conflicts); double-buffered, the next chunk's
 //           global loads in flight during this chunk's MFMAs.
 //   V:      lane (tile t = tb*32 + lane%32, channels 8*(lane/32) of 16-channel half ks) reads
-//           the 2 x 4 window pixels row a needs and computes, in FP16 arithmetic,
+//           the 2 x 3 window pixels its positions need and computes, in FP16 arithmetic,
 //             s_j = B^T row a applied to window column j   (2 rows: d0-d2, d1+d2, d2-d1, d1-d3)
-//             V_b = (s B)_b: s0-s2, s1+s2, s2-s1, s1-s3
+//             V_b = (s B)_b: s0-s2, s1+s2 (BH = 0, columns 0-2); s2-s1, s1-s3 (BH = 1, 1-3)
 //           (the reference's input transform is FP16 arithmetic too, preprocessing.cuh:57-72).
-//   MFMA:   acc[b][m][tb] += U[4a+b][k block m] . V_b over the chunk's two 16-channel halves.
+//   MFMA:   acc[bb][m][tb] += U[4a+2BH+bb][k block m] . V_{2BH+bb} over the chunk's two
+//           16-channel halves.
 //
-// Epilogue (A^T M A, reference conv2d.cpp:18-47): each wave folds its row into
-// Z_j = sum_b A^T[j][b] M[a][b] (Z_0 = (M0 + M1) + M2, Z_1 = (M1 - M2) - M3) and writes Z to
-// LDS; then wave w takes output block w (channel block w/2, tile block w%2) and forms
-// Y_0j = (Z0_j + Z1_j) + Z2_j, Y_1j = (Z1_j - Z2_j) - Z3_j, rounded once to FP16.
+// Epilogue (A^T M A, reference conv2d.cpp:18-47): row a folds into Z_j = sum_b A^T[j][b] M[a][b]
+// in the order Z_0 = (M0 + M1) + M2, Z_1 = (M1 - M2) - M3: the BH = 0 wave leaves P = M0 + M1
+// and M1 in LDS, the BH = 1 wave adds its M2, M3 and writes Z; then each wave takes half an
+// output block (channel block m, tile block tb) and forms Y_0j = (Z0_j + Z1_j) + Z2_j,
+// Y_1j = (Z1_j - Z2_j) - Z3_j, rounded once to FP16.
 #include "wino_tile_ops.h"
 #include "wino_fused.h"
 
@@ -65,14 +69,14 @@ __device__ inline half8 sub8(half8 x, half8 y, half8 m1) { return __builtin_elem
 __device__ inline uint32_t pack_lo(uint32_t x, uint32_t y) { return __builtin_amdgcn_perm(y, x, 0x05040100u); }
 __device__ inline uint32_t pack_hi(uint32_t x, uint32_t y) { return __builtin_amdgcn_perm(y, x, 0x07060302u); }
 
-template <int NU, bool D16>
+template <int NU, bool D16, int NT>
 __device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw0, Staging<NU>& st) {
     const int PR = prm.fpr, PC = prm.fpc, PH = PC >> 1;
     const int nunits = 4 * PR * PH;
     const int H = prm.h, W = prm.w;
 #pragma unroll
     for (int i = 0; i < NU; ++i) {
-        const int u = threadIdx.x + 256 * i;
+        const int u = threadIdx.x + NT * i;
         const int g = u / (PR * PH);
         const int rem = u - g * (PR * PH);
         const int pr = rem / PH;
@@ -134,13 +138,16 @@ __device__ inline void stage_store(const Staging<NU>& st, half_t* buf, int half_
 }  // namespace
 
 // One work item: 64 output channels (block kblk) x one 64-tile block (gblk, image-major) of the
-// problem in prm, by a workgroup of 256 threads, in the launch's dynamic LDS.
-template <int NU, bool D16>
+// problem in prm, by the waves of one half (BH) of a 512-thread workgroup, in the launch's
+// dynamic LDS.
+template <bool D16, int BH>
 __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gblk) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
+    constexpr int NU = 2;  // 512 threads x 2 units cover the largest patch (1024 units)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int a = __builtin_amdgcn_readfirstlane(tid >> 6);  // position row of this wave
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int a = wave & 3;
 
     const int K = prm.k, C = prm.c;
     const int KB = prm.KB32, CB = prm.CB, C16 = 2 * CB;
@@ -156,28 +163,25 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     const uint32_t chan_bytes = (uint32_t)prm.h * prm.w * 2;
     const __amdgpu_buffer_rsrc_t in_rsrc = buffer_rsrc(in_img, (uint32_t)C * chan_bytes);
 
-    // U fragments of this wave: (b, ks, m) for pos = 4a + b, 16-channel half ks of the chunk,
-    // 32-row block kblk*2 + m (a block beyond K reads block 0: its rows are never stored).
-    // Byte offsets are wave-uniform (scalar); each lane adds its 16 bytes.
+    // U fragments (ks, bb, m) for pos = 4a + 2BH + bb; a block beyond K reads block 0
     const __amdgpu_buffer_rsrc_t u_rsrc =
         buffer_rsrc(prm.transform_filter, (uint32_t)(16u * C16 * KB * kFragHalves * 2u));
     const int ulane = lane * 16;
-    auto load_u = [&](half8 (&u)[16], int ch) {
+    auto load_u = [&](half8 (&u)[8], int ch) {
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+            for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
                 for (int m = 0; m < 2; ++m) {
                     const int kb = kblk * 2 + m < KB ? kblk * 2 + m : 0;
-                    const uint32_t frag = ((uint32_t)(4 * a + b) * C16 + ch * 2 + ks) * KB + kb;
+                    const uint32_t frag = ((uint32_t)(4 * a + 2 * BH + bb) * C16 + ch * 2 + ks) * KB + kb;
                     const u32x4 raw =
                         __builtin_amdgcn_raw_buffer_load_b128(u_rsrc, ulane, (int)(frag * (kFragHalves * 2u)), 0);
-                    u[(ks * 4 + b) * 2 + m] = __builtin_bit_cast(half8, raw);
+                    u[(ks * 2 + bb) * 2 + m] = __builtin_bit_cast(half8, raw);
                 }
     };
 
-    // window origin (in pixels) inside the patch for this lane's two tiles
     int woff[2];
 #pragma unroll
     for (int tb = 0; tb < 2; ++tb) {
@@ -185,13 +189,13 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         const int tr = t / FBW, tc = t - (t / FBW) * FBW;
         woff[tb] = (2 * tr) * PC + tc;
     }
-    // window column j of tile column tc is patch column 2*tc + fsh + j: parity (fsh + j) & 1,
-    // position tc + (fsh + j) / 2
-    int colj[4];
+    // window columns BH + jj (jj = 0..2): parity (fsh + j) & 1, position tc + (fsh + j) / 2
+    int colj[3];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) colj[j] = ((prm.fsh + j) & 1) * (PC / 2) + ((prm.fsh + j) >> 1);
-    // rows of the window that row a of B^T combines: s = d[rp] + sg * d[rm], sg = +-1 (exact,
-    // so the fma rounds once like the addition or subtraction it stands for)
+    for (int jj = 0; jj < 3; ++jj) {
+        const int j = BH + jj;
+        colj[jj] = ((prm.fsh + j) & 1) * (PC / 2) + ((prm.fsh + j) >> 1);
+    }
     const int rp = a == 0 ? 0 : (a == 2 ? 2 : 1);
     const int rm = a == 0 ? 2 : (a == 2 ? 1 : (a == 1 ? 2 : 3));
     const half_t sgh = a == 1 ? (half_t)1.0f : (half_t)-1.0f;
@@ -199,32 +203,31 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     half8 m1 = {-1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16};
     asm volatile("" : "+s"(m1));
 
-    const int PB = 4 * PR * PC * 8;  // halves per patch buffer
+    const int PB = 4 * PR * PC * 8;
 
-    floatx16 acc[4][2][2];
+    floatx16 acc[2][2][2];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
         for (int m = 0; m < 2; ++m)
 #pragma unroll
             for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-                for (int e = 0; e < 16; ++e) acc[b][m][tb][e] = 0.f;
+                for (int e = 0; e < 16; ++e) acc[bb][m][tb][e] = 0.f;
 
+    // the next chunk's patch loads are in flight during this chunk's MFMAs; U has one
+    // register set (two waves per SIMD hide its L2 latency), reloaded after its last use
     Staging<NU> st;
-    stage_setup<NU, D16>(prm, ih0, iw0, st);
+    stage_setup<NU, D16, 512>(prm, ih0, iw0, st);
     stage_load<NU, D16>(in_rsrc, 0u, chan_bytes, st);
     stage_store<NU>(st, lds, PC / 2);
-    half8 ua[16], ub[16];
-    load_u(ua, 0);
+    half8 ucur[8];
+    load_u(ucur, 0);
     __syncthreads();
 
-    auto chunk = [&](int ch, half8 (&ucur)[16], half8 (&unext)[16]) {
+    auto chunk = [&](int ch) {
         const bool more = ch + 1 < CB;
-        if (more) {
-            stage_load<NU, D16>(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes, st);
-            load_u(unext, ch + 1);
-        }
+        if (more) stage_load<NU, D16>(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes, st);
         const half_t* buf = lds + (ch & 1) * PB;
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
@@ -232,60 +235,84 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
 #pragma unroll
             for (int tb = 0; tb < 2; ++tb) {
                 const half_t* wp = buf + ((size_t)gsel * PR * PC + woff[tb]) * 8;
-                half8 s[4];
+                half8 s3[3];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const half8 dp = *(const half8*)(wp + (rp * PC + colj[j]) * 8);
-                    const half8 dm = *(const half8*)(wp + (rm * PC + colj[j]) * 8);
-                    s[j] = __builtin_elementwise_fma(dm, sg, dp);
+                for (int jj = 0; jj < 3; ++jj) {
+                    const half8 dp = *(const half8*)(wp + (rp * PC + colj[jj]) * 8);
+                    const half8 dm = *(const half8*)(wp + (rm * PC + colj[jj]) * 8);
+                    s3[jj] = __builtin_elementwise_fma(dm, sg, dp);
                 }
-                half8 v[4];
-                v[0] = sub8(s[0], s[2], m1);
-                v[1] = s[1] + s[2];
-                v[2] = sub8(s[2], s[1], m1);
-                v[3] = sub8(s[1], s[3], m1);
+                half8 v[2];
+                if (BH == 0) {
+                    v[0] = sub8(s3[0], s3[2], m1);  // s0 - s2
+                    v[1] = s3[1] + s3[2];           // s1 + s2
+                } else {
+                    v[0] = sub8(s3[1], s3[0], m1);  // s2 - s1
+                    v[1] = sub8(s3[0], s3[2], m1);  // s1 - s3
+                }
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
+                for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
                     for (int m = 0; m < 2; ++m)
-                        acc[b][m][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ucur[(ks * 4 + b) * 2 + m], v[b],
-                                                                              acc[b][m][tb], 0, 0, 0);
+                        acc[bb][m][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ucur[(ks * 2 + bb) * 2 + m], v[bb],
+                                                                               acc[bb][m][tb], 0, 0, 0);
             }
         }
-        if (more) stage_store<NU>(st, lds + ((ch + 1) & 1) * PB, PC / 2);
+        if (more) {
+            load_u(ucur, ch + 1);
+            stage_store<NU>(st, lds + ((ch + 1) & 1) * PB, PC / 2);
+        }
         __syncthreads();
     };
-    int ch = 0;
-    for (; ch + 1 < CB; ch += 2) {
-        chunk(ch, ua, ub);
-        chunk(ch + 1, ub, ua);
-    }
-    if (ch < CB) chunk(ch, ua, ub);
+    for (int ch = 0; ch < CB; ++ch) chunk(ch);
 
-    // ---- epilogue: Z exchange through LDS, then Y for one 32x32 block per wave ----
-    // LDS Z layout (floats): [a][j][block = m*2 + tb][e/4][lane][4]
+    // ---- epilogue; LDS Z layout (floats): [a][j][block = m*2 + tb][e/4][lane][4] ----
     float* Z = (float*)lds;
+    if (BH == 0) {
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int tb = 0; tb < 2; ++tb)
+            for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-            for (int e4 = 0; e4 < 4; ++e4) {
-                floatx4 z0, z1;
+                for (int e4 = 0; e4 < 4; ++e4) {
+                    floatx4 p, m1v;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int e = e4 * 4 + q;
-                    const float M0 = acc[0][m][tb][e], M1 = acc[1][m][tb][e];
-                    const float M2 = acc[2][m][tb][e], M3 = acc[3][m][tb][e];
-                    z0[q] = (M0 + M1) + M2;
-                    z1[q] = (M1 - M2) - M3;
+                    for (int q = 0; q < 4; ++q) {
+                        const int e = e4 * 4 + q;
+                        p[q] = acc[0][m][tb][e] + acc[1][m][tb][e];  // M0 + M1
+                        m1v[q] = acc[1][m][tb][e];                   // M1
+                    }
+                    const int blk = m * 2 + tb;
+                    *(floatx4*)(Z + ((((a * 2 + 0) * 4 + blk) * 4 + e4) * 64 + lane) * 4) = p;
+                    *(floatx4*)(Z + ((((a * 2 + 1) * 4 + blk) * 4 + e4) * 64 + lane) * 4) = m1v;
                 }
-                const int blk = m * 2 + tb;
-                *(floatx4*)(Z + ((((a * 2 + 0) * 4 + blk) * 4 + e4) * 64 + lane) * 4) = z0;
-                *(floatx4*)(Z + ((((a * 2 + 1) * 4 + blk) * 4 + e4) * 64 + lane) * 4) = z1;
-            }
+    }
     __syncthreads();
-    const int blk = a;  // output block of this wave
+    if (BH == 1) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+                for (int e4 = 0; e4 < 4; ++e4) {
+                    const int blk = m * 2 + tb;
+                    float* z0p = Z + ((((a * 2 + 0) * 4 + blk) * 4 + e4) * 64 + lane) * 4;
+                    float* z1p = Z + ((((a * 2 + 1) * 4 + blk) * 4 + e4) * 64 + lane) * 4;
+                    const floatx4 p = *(const floatx4*)z0p, m1v = *(const floatx4*)z1p;
+                    floatx4 z0, z1;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int e = e4 * 4 + q;
+                        const float M2 = acc[0][m][tb][e], M3 = acc[1][m][tb][e];
+                        z0[q] = p[q] + M2;              // (M0 + M1) + M2
+                        z1[q] = (m1v[q] - M2) - M3;     // (M1 - M2) - M3
+                    }
+                    *(floatx4*)z0p = z0;
+                    *(floatx4*)z1p = z1;
+                }
+    }
+    __syncthreads();
+    const int blk = wave >> 1;  // output block of this wave, e/4 groups 2*(wave&1) .. +1
     const int m = blk >> 1, tb = blk & 1;
     const int t = tb * 32 + (lane & 31);
     const int th = th0 + t / FBW, tw = tw0 + (t - (t / FBW) * FBW);
@@ -297,7 +324,8 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         tc.r1 = tc.oh + 1 < prm.Oh;
         tc.c1 = tc.ow + 1 < prm.Ow;
 #pragma unroll
-        for (int e4 = 0; e4 < 4; ++e4) {
+        for (int eh = 0; eh < 2; ++eh) {
+            const int e4 = 2 * (wave & 1) + eh;
             floatx4 z[4][2];
 #pragma unroll
             for (int r = 0; r < 4; ++r)
@@ -319,18 +347,27 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     }
 }
 
-template <int NU, bool D16>
-__global__ void __launch_bounds__(256, 1) wino_fused_kernel(mykernelParamType prm) {
+template <bool D16>
+__device__ inline void fused_item_dispatch(const mykernelParamType& prm, int kblk, int gblk) {
+    // waves 0-3 take BH = 0, waves 4-7 BH = 1 (wave-uniform; both paths pass the same barriers)
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 8) == 0)
+        fused_item<D16, 0>(prm, kblk, gblk);
+    else
+        fused_item<D16, 1>(prm, kblk, gblk);
+}
+
+template <bool D16>
+__global__ void __launch_bounds__(512, 1) wino_fused_kernel(mykernelParamType prm) {
     int kblk, gblk;
     xcd_remap(blockIdx.x, gridDim.x, (prm.k + 63) / 64, &kblk, &gblk);
-    fused_item<NU, D16>(prm, kblk, gblk);
+    fused_item_dispatch<D16>(prm, kblk, gblk);
 }
 
 // Multi-layer form: one launch over the fused items of several layers (descriptor i owns items
-// [s3_first, s3_first + s3_items), nk = 64-channel blocks), every layer with NU staging units
-// and the same staging load width.
-template <int NU, bool D16>
-__global__ void __launch_bounds__(256, 1)
+// [s3_first, s3_first + s3_items), nk = 64-channel blocks), every layer with the same staging
+// load width.
+template <bool D16>
+__global__ void __launch_bounds__(512, 1)
 wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
     int item, unused;
     xcd_remap(blockIdx.x, total, total, &item, &unused);
@@ -341,24 +378,20 @@ wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, in
     }
     const BatchLayer& bl = layers[lo];
     const int local = item - bl.s3_first;
-    fused_item<NU, D16>(bl.prm, local % bl.nk, local / bl.nk);
+    fused_item_dispatch<D16>(bl.prm, local % bl.nk, local / bl.nk);
 }
 
-#define FUSED_INSTANCES(NU, D16)                                                            \
-    template __global__ void wino_fused_kernel<NU, D16>(mykernelParamType);                 \
-    template __global__ void wino_fused_grouped_kernel<NU, D16>(const BatchLayer*, int, int);
-FUSED_INSTANCES(3, false)
-FUSED_INSTANCES(4, false)
-FUSED_INSTANCES(3, true)
-FUSED_INSTANCES(4, true)
+template __global__ void wino_fused_kernel<false>(mykernelParamType);
+template __global__ void wino_fused_kernel<true>(mykernelParamType);
+template __global__ void wino_fused_grouped_kernel<false>(const BatchLayer*, int, int);
+template __global__ void wino_fused_grouped_kernel<true>(const BatchLayer*, int, int);
 
-const void* fused_kernel_ptr(int nu, bool d16) {
-    if (d16) return nu <= 3 ? (const void*)&wino_fused_kernel<3, true> : (const void*)&wino_fused_kernel<4, true>;
-    return nu <= 3 ? (const void*)&wino_fused_kernel<3, false> : (const void*)&wino_fused_kernel<4, false>;
+int fused_threads() { return kFusedThreads; }
+
+const void* fused_kernel_ptr(bool d16) {
+    return d16 ? (const void*)&wino_fused_kernel<true> : (const void*)&wino_fused_kernel<false>;
 }
 
-const void* fused_grouped_kernel_ptr(int nu, bool d16) {
-    if (d16)
-        return nu <= 3 ? (const void*)&wino_fused_grouped_kernel<3, true> : (const void*)&wino_fused_grouped_kernel<4, true>;
-    return nu <= 3 ? (const void*)&wino_fused_grouped_kernel<3, false> : (const void*)&wino_fused_grouped_kernel<4, false>;
+const void* fused_grouped_kernel_ptr(bool d16) {
+    return d16 ? (const void*)&wino_fused_grouped_kernel<true> : (const void*)&wino_fused_grouped_kernel<false>;
 }

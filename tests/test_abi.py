@@ -142,16 +142,18 @@ def test_weights_load_refuses_foreign_files_before_device_access(wc, tmp_path):
 
 
 def test_form_choice_and_fused_slots(wc):
-    """The benchmark layer (K = 256, N*T*K*C = 1.6e9) and 28x28 C = K = 512 stay staged; small
-    layers and K <= 192 take the fused form (slot 2 = the fused kernel, slots 3 and 4 empty);
-    odd widths only when small with C <= 256; the direct path stays staged."""
+    """The benchmark layer (K = 256, N*T*K*C = 1.6e9), 28x28 C = K = 512 and N*T*K*C >= 8e8
+    with K > 192 stay staged; smaller layers and K <= 192 take the fused form (slot 2 = the
+    fused kernel, slots 3 and 4 empty), odd widths included; the direct path stays staged."""
     L = wc.load_library()
     f = ctypes.c_int()
     cases = [((32, 256, 56, 56, 256), {}, wc.FORM_STAGED), ((8, 256, 14, 14, 256), {}, wc.FORM_FUSED),
              ((64, 512, 28, 28, 512), {}, wc.FORM_STAGED), ((16, 512, 112, 112, 96), {}, wc.FORM_FUSED),
              ((32, 256, 56, 56, 192), {}, wc.FORM_FUSED), ((32, 256, 56, 56, 193), {}, wc.FORM_STAGED),
-             ((32, 96, 55, 55, 96), {}, wc.FORM_STAGED), ((1, 64, 55, 55, 64), {}, wc.FORM_FUSED),
-             ((4, 512, 13, 13, 96), {}, wc.FORM_STAGED), ((32, 256, 55, 55, 256), {}, wc.FORM_STAGED),
+             ((32, 96, 55, 55, 96), {}, wc.FORM_FUSED), ((1, 64, 55, 55, 64), {}, wc.FORM_FUSED),
+             ((4, 512, 13, 13, 96), {}, wc.FORM_FUSED), ((32, 256, 55, 55, 256), {}, wc.FORM_STAGED),
+             ((16, 512, 28, 28, 512), {}, wc.FORM_STAGED), ((8, 512, 28, 28, 512), {}, wc.FORM_FUSED),
+             ((16, 128, 55, 55, 256), {}, wc.FORM_FUSED), ((64, 256, 27, 27, 256), {}, wc.FORM_STAGED),
              ((2, 16, 9, 9, 16), dict(u=2, v=2), wc.FORM_STAGED)]
     for (n, c, h, w, k), extra, want in cases:
         pb = wc.make_problem(n, c, h, w, k, **extra, ptrs=FAKE)
@@ -159,7 +161,7 @@ def test_form_choice_and_fused_slots(wc):
         rc, ki, prm = _ki(wc, pb)
         assert rc == wc.OK and prm.form == want
         if want == wc.FORM_FUSED:
-            assert ki.slot(2)["block"] == (256, 1, 1) and ki.slot(2)["grid"][0] == n * prm.fnbh * prm.fnbw * -(-k // 64)
+            assert ki.slot(2)["block"] == (512, 1, 1) and ki.slot(2)["grid"][0] == n * prm.fnbh * prm.fnbw * -(-k // 64)
             assert ki.slot(3)["grid"] == (1, 1, 1) and ki.slot(4)["grid"] == (1, 1, 1)
             assert ki.slot(2)["lds"] >= 131072
     assert L.conv2d_form(None, ctypes.byref(f)) == wc.ERR_NULL
