@@ -227,6 +227,11 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
 
     auto chunk = [&](int ch) {
         const bool more = ch + 1 < CB;
+        // Wait for this chunk's U here, before the next chunk's patch loads are issued: the
+        // memory counter drains in issue order, so a wait for U placed after those loads (at the
+        // first MFMA) would also wait for them and expose their whole latency every chunk.
+        asm volatile("" ::"v"(ucur[0]), "v"(ucur[1]), "v"(ucur[2]), "v"(ucur[3]), "v"(ucur[4]), "v"(ucur[5]),
+                     "v"(ucur[6]), "v"(ucur[7]));
         if (more) stage_load<NU, D16>(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes, st);
         const half_t* buf = lds + (ch & 1) * PB;
 #pragma unroll
