@@ -176,11 +176,12 @@ int select_config(const problem_t* pb, const Geometry& g) {
 bool use_fused(const problem_t* pb, const Geometry& g) {
     if (g.direct || !g.fused_ok) return false;
     if (tuned_config_lookup(pb) >= 0) return false;
-    // the staged GEMM gains on the fused kernel as K and N*T*K*C grow: measured crossover
-    // between 8.2e8 and 9.2e8 for K >= 256 (both staging widths); for K <= 192 the fused form
-    // was faster on every measured shape (up to 1.6e9)
+    // Measured on 45 shapes (both forms timed after each had run 50 ms, so at sustained
+    // clocks): with K <= 192 the fused form was faster on every shape (up to 2.6x); with
+    // C*K <= 256^2 it was faster on average (0.44-1.17x of the staged time, mean 0.91); with a
+    // larger C*K the staged GEMM's long reduction wins from N*T*K*C ~ 8e8 (up to 1.39x).
     const double work = (double)g.NT * (double)pb->k * (double)pb->c;
-    return pb->k <= 192 || work < 8e8;
+    return pb->k <= 192 || work < 8e8 || (double)pb->c * (double)pb->k <= 65536.0;
 }
 
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {

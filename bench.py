@@ -2,13 +2,15 @@
 
 Workload (one "step" = one forward pass over one batch): N=32, C=256, H=W=56, K=256,
 3x3 stride 1 pad 1, FP16 NCHW input ~N(0,1) and KCRS filters ~N(0, 2/(9C)), seeded.
-A step runs the reference's four-slot protocol (filter transform, input transform,
-MFMA GEMM with fused output transform, empty fourth slot), as the reference harness
-times it (main.cpp:112-131). Inputs are resident in HBM before timing starts.
+A step runs the reference's four-slot protocol as the reference harness times it
+(main.cpp:112-131), in the form getkernelInfo chooses for the layer: fused (filter
+transform, fused input transform + GEMM + output transform, two empty slots) or staged
+(filter transform, input transform, MFMA GEMM with fused output transform, empty slot).
+Inputs are resident in HBM before timing starts.
 
 value = effective TFLOP/s = 2*N*K*C*9*Oh*Ow / time per step (direct-conv FLOPs).
-Also reported: per-stage times, the steady state (weights prepared once), the
-roofline of the dominant kernel (the GEMM), the error against the FP64 direct
+Also reported: per-stage times of both forms, the steady state (weights prepared once),
+the roofline of the chosen form's dominant kernel, the error against the FP64 direct
 convolution (GPU validation kernel), and the reference's CPU checker timed on the
 host on one image of the same workload.
 
@@ -35,9 +37,12 @@ WORKLOAD = dict(n=32, c=256, h=56, w=56, k=256)
 
 
 def _events_time(fn, iters: int, stream=None) -> float:
-    """Average ms per call of fn over iters, HIP events on the launch stream."""
+    """Average ms per call of fn over iters, HIP events on the launch stream, after one untimed
+    call (a kernel's first launch in a process loads its code object on the host)."""
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
+    fn()
+    s.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
     for _ in range(iters):
@@ -242,20 +247,22 @@ def direct_section(runner, dev, steps: int) -> dict:
     return out
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "wino_gemm_n32_c256_h56_k256.md")
+TRAFFIC_PROFILES = {"staged": os.path.join(ROOT, "profiles", "wino_gemm_n32_c256_h56_k256.md"),
+                    "fused": os.path.join(ROOT, "profiles", "wino_fused_n32_c256_h56_k256.md")}
 
 
-def measured_traffic(cfg: int):
-    """HBM bytes per GEMM launch on this workload, from the committed counter profile
-    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by its copy-kernel calibration).
-    None when the profile is absent or was taken for another tile configuration."""
+def measured_traffic(form: str, key: str):
+    """HBM bytes per launch of the dominant kernel on this workload, from the committed counter
+    profile (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by its copy-kernel calibration).
+    None when the profile is absent or was taken for another kernel variant (key: "cfg=9" for
+    a GEMM tile configuration, "form=fused" for the fused kernel)."""
     import re
     try:
-        text = open(TRAFFIC_PROFILE).read()
+        text = open(TRAFFIC_PROFILES[form]).read()
     except OSError:
         return None
-    m = re.search(r"^traffic_per_launch: cfg=(\d+) bytes=(\d+)", text, flags=re.M)
-    if m is None or int(m.group(1)) != cfg:
+    m = re.search(r"^traffic_per_launch: (\S+) bytes=(\d+)", text, flags=re.M)
+    if m is None or m.group(1) != key:
         return None
     return int(m.group(2))
 
@@ -320,6 +327,8 @@ def main() -> int:
     if args.cfg >= 0:
         conv.L.conv2d_tuned_clear()
     cfg = conv.selected_config() if args.cfg < 0 else args.cfg
+    wc = importlib.import_module("2024pra_conv_amd")
+    fused = args.cfg < 0 and conv.form() == wc.FORM_FUSED
 
     oh, ow = conv.oh, conv.ow
     T = ((oh + 1) // 2) * ((ow + 1) // 2)
@@ -347,12 +356,15 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
-    # per-stage and steady-state timings (same stream, HIP events)
+    # per-stage and steady-state timings (same stream, HIP events); the staged stages are
+    # reported for either form
     conv.prepare_weights()
     ms_s1 = _events_time(conv.prepare_weights, args.steps)
+    ms_fused = _events_time(conv.fused_gemm, args.steps) if fused else None
     ms_s2 = _events_time(conv.input_transform, args.steps)
     ms_s3 = _events_time(lambda: conv.gemm(cfg), args.steps)
-    ms_steady = _events_time(lambda: conv.forward(cfg), args.steps)
+    ms_staged = _events_time(lambda: conv.forward(cfg), args.steps)
+    ms_steady = _events_time(lambda: conv.forward(-1), args.steps)
 
     # accuracy of the benchmarked output against the FP64 direct convolution
     step()
@@ -364,21 +376,28 @@ def main() -> int:
     rms_rel = (d.norm() / ref.norm()).item()
     finite = bool(torch.isfinite(y).all())
 
-    s3_bytes = v_bytes + u_bytes + out_bytes
-    s3_tflops = gemm_flops / (ms_s3 * 1e-3) / 1e12
-    s3_gbs = s3_bytes / (ms_s3 * 1e-3) / 1e9
-    mfma_floor = gemm_flops / (PEAK_FP16_TFLOPS * 1e12)
-    hbm_floor = s3_bytes / (PEAK_HBM_GBS * 1e9)
-    if hbm_floor > mfma_floor:
-        roof = {"bound": "hbm", "achieved": round(s3_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(s3_gbs / PEAK_HBM_GBS, 4)}
+    # roofline of the dominant kernel of the form the step runs
+    if fused:
+        k_ms, k_bytes = ms_fused, in_bytes + u_bytes + out_bytes
+        k_name, k_traffic = ("wino_fused_kernel (input transform + GEMM + output transform)",
+                             measured_traffic("fused", "form=fused"))
     else:
-        roof = {"bound": "mfma", "achieved": round(s3_tflops, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(s3_tflops / PEAK_FP16_TFLOPS, 4)}
-    roof.update({"kernel": "wino_gemm (S3, fused output transform)", "traffic": measured_traffic(cfg),
-                 "mfma_frac": round(s3_tflops / PEAK_FP16_TFLOPS, 4),
-                 "hbm_frac": round(s3_gbs / PEAK_HBM_GBS, 4),
-                 "algorithmic_bytes": s3_bytes, "algorithmic_flops": gemm_flops})
+        k_ms, k_bytes = ms_s3, v_bytes + u_bytes + out_bytes
+        k_name, k_traffic = "wino_gemm (S3, fused output transform)", measured_traffic("staged", f"cfg={cfg}")
+    k_tflops = gemm_flops / (k_ms * 1e-3) / 1e12
+    k_gbs = k_bytes / (k_ms * 1e-3) / 1e9
+    mfma_floor = gemm_flops / (PEAK_FP16_TFLOPS * 1e12)
+    hbm_floor = k_bytes / (PEAK_HBM_GBS * 1e9)
+    if hbm_floor > mfma_floor:
+        roof = {"bound": "hbm", "achieved": round(k_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(k_gbs / PEAK_HBM_GBS, 4)}
+    else:
+        roof = {"bound": "mfma", "achieved": round(k_tflops, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(k_tflops / PEAK_FP16_TFLOPS, 4)}
+    roof.update({"kernel": k_name, "traffic": k_traffic, "us_per_launch": round(k_ms * 1e3, 2),
+                 "mfma_frac": round(k_tflops / PEAK_FP16_TFLOPS, 4),
+                 "hbm_frac": round(k_gbs / PEAK_HBM_GBS, 4),
+                 "algorithmic_bytes": k_bytes, "algorithmic_flops": gemm_flops})
 
     value = direct_flops / (ms * 1e-3) / 1e12
     line = {
@@ -397,10 +416,13 @@ def main() -> int:
         "data": "synthetic: seeded N(0,1) FP16 input, N(0,2/(9C)) FP16 filters",
         "config": {"workload": f"plain FP16 Winograd F(2x2,3x3) conv2d, N={n} C={c} H=W={h} K={k}, 3x3 stride 1 pad 1, "
                                "four-slot reference protocol per step (filter transform included)",
-                   "tile_config": cfg},
+                   "form": "fused" if fused else "staged", "tile_config": None if fused else cfg},
         "roofline": roof,
-        "stages_us": {"S1_filter": round(ms_s1 * 1e3, 2), "S2_input": round(ms_s2 * 1e3, 2),
-                      "S3_gemm_fused": round(ms_s3 * 1e3, 2), "steady_S2_S3": round(ms_steady * 1e3, 2)},
+        "stages_us": {"S1_filter": round(ms_s1 * 1e3, 2),
+                      "fused_kernel": round(ms_fused * 1e3, 2) if fused else None,
+                      "staged_S2_input": round(ms_s2 * 1e3, 2), "staged_S3_gemm": round(ms_s3 * 1e3, 2),
+                      "staged_S2_S3": round(ms_staged * 1e3, 2), "staged_tile_config": cfg,
+                      "steady": round(ms_steady * 1e3, 2)},
         "steady_state_tflops": round(direct_flops / (ms_steady * 1e-3) / 1e12, 2),
         "s2_hbm_frac": round((in_bytes + v_bytes) / (ms_s2 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
         "error_vs_fp64": {"max_rel": max_rel, "rms_rel": rms_rel, "finite": finite,

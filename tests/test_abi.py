@@ -90,12 +90,25 @@ def test_getkernelinfo_fills_slots_and_param(wc):
     for i in range(1, 5):
         s = ki.slot(i)
         assert s["kernel"] and all(s["grid"]) and all(s["block"])
+    # this layer (C*K = 256^2) takes the fused form: filter transform, fused kernel, two empty
+    assert prm.form == wc.FORM_FUSED
+    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 4, 1) and ki.slot(1)["block"] == (64, 1, 1)
+    assert ki.slot(2)["grid"] == (32 * prm.fnbh * prm.fnbw * 4, 1, 1) and ki.slot(2)["block"] == (512, 1, 1)
+    assert ki.slot(3)["grid"] == (1, 1, 1) and ki.slot(4)["grid"] == (1, 1, 1)
+
+
+def test_getkernelinfo_staged_slots(wc):
+    """A staged layer (C = K = 512, N*T*K*C = 3.3e9): input transform, GEMM, empty."""
+    pb = wc.make_problem(64, 512, 28, 28, 512, ptrs=FAKE)
+    rc, ki, prm = _ki(wc, pb)
+    assert rc == wc.OK and prm.form == wc.FORM_STAGED
+    assert (prm.tiles_h, prm.tiles_w, prm.T, prm.NT, prm.G32, prm.CB, prm.KB32) == (14, 14, 196, 12544, 392, 16, 16)
     assert ki.slot(4)["grid"] == (1, 1, 1)
     cfg = wc.tile_configs()[prm.cfg]
     assert ki.slot(3)["block"][0] == cfg["threads"]
     # one-dimensional grid over (column block, k-block) pairs, remapped per XCD in the kernel
     assert ki.slot(3)["grid"] == (-(-prm.G32 // (cfg["bt"] // 32)) * -(-prm.KB32 // (cfg["bk"] // 32)), 1, 1)
-    assert ki.slot(2)["grid"] == (-(-28 // prm.band), 8, 32)
+    assert ki.slot(2)["grid"] == (-(-14 // prm.band), 16, 64)
     assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 4, 1) and ki.slot(1)["block"] == (64, 1, 1)
 
 
@@ -142,18 +155,18 @@ def test_weights_load_refuses_foreign_files_before_device_access(wc, tmp_path):
 
 
 def test_form_choice_and_fused_slots(wc):
-    """The benchmark layer (K = 256, N*T*K*C = 1.6e9), 28x28 C = K = 512 and N*T*K*C >= 8e8
-    with K > 192 stay staged; smaller layers and K <= 192 take the fused form (slot 2 = the
-    fused kernel, slots 3 and 4 empty), odd widths included; the direct path stays staged."""
+    """Layers with K <= 192, C*K <= 256^2 or N*T*K*C < 8e8 take the fused form (slot 2 = the
+    fused kernel, slots 3 and 4 empty), odd widths included; larger C*K from N*T*K*C = 8e8 on
+    (28x28 C = K = 512, 56x56 C = K = 384) and the direct path stay staged."""
     L = wc.load_library()
     f = ctypes.c_int()
-    cases = [((32, 256, 56, 56, 256), {}, wc.FORM_STAGED), ((8, 256, 14, 14, 256), {}, wc.FORM_FUSED),
+    cases = [((32, 256, 56, 56, 256), {}, wc.FORM_FUSED), ((8, 256, 14, 14, 256), {}, wc.FORM_FUSED),
              ((64, 512, 28, 28, 512), {}, wc.FORM_STAGED), ((16, 512, 112, 112, 96), {}, wc.FORM_FUSED),
-             ((32, 256, 56, 56, 192), {}, wc.FORM_FUSED), ((32, 256, 56, 56, 193), {}, wc.FORM_STAGED),
+             ((32, 512, 56, 56, 192), {}, wc.FORM_FUSED), ((32, 512, 56, 56, 193), {}, wc.FORM_STAGED),
              ((32, 96, 55, 55, 96), {}, wc.FORM_FUSED), ((1, 64, 55, 55, 64), {}, wc.FORM_FUSED),
-             ((4, 512, 13, 13, 96), {}, wc.FORM_FUSED), ((32, 256, 55, 55, 256), {}, wc.FORM_STAGED),
+             ((4, 512, 13, 13, 96), {}, wc.FORM_FUSED), ((32, 256, 55, 55, 256), {}, wc.FORM_FUSED),
              ((16, 512, 28, 28, 512), {}, wc.FORM_STAGED), ((8, 512, 28, 28, 512), {}, wc.FORM_FUSED),
-             ((16, 128, 55, 55, 256), {}, wc.FORM_FUSED), ((64, 256, 27, 27, 256), {}, wc.FORM_STAGED),
+             ((32, 384, 56, 56, 384), {}, wc.FORM_STAGED), ((64, 256, 56, 56, 257), {}, wc.FORM_STAGED),
              ((2, 16, 9, 9, 16), dict(u=2, v=2), wc.FORM_STAGED)]
     for (n, c, h, w, k), extra, want in cases:
         pb = wc.make_problem(n, c, h, w, k, **extra, ptrs=FAKE)
