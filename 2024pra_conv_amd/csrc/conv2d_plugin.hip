@@ -252,7 +252,7 @@ Launch slot_filter(const Geometry& g) {
     return L;
 }
 
-// Fused form: one workgroup of 4 waves per (64 output channels, 64-tile block); LDS holds two
+// Fused form: one workgroup of 8 waves per (64 output channels, 64-tile block); LDS holds two
 // input patches during the channel loop and the 128 KiB Z exchange after it.
 Launch slot_fused(const Geometry& g) {
     Launch L;

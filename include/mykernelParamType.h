@@ -37,7 +37,7 @@ typedef struct mykernelParamType {
                     channel chunks) or, on the general-geometry direct path,
                     ceil(r * s * c / 32) rounded up to a multiple of 8 (the padding carries
                     zeros) */
-    /* fused form (input transform inside the GEMM), Winograd geometry with even w only:
+    /* fused form (input transform inside the GEMM), Winograd geometry:
        fbh x fbw = 64 tiles per workgroup, fnbh x fnbw such blocks per image, input patch
        of fpr rows x fpc columns starting fsh columns left of 2*tw0 - q, fnu staging units
        (8 channels x 2 pixels) per thread of the 512-thread workgroup */
