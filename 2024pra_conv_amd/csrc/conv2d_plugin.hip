@@ -103,11 +103,13 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         const int units = 4 * g->fpr * (g->fpc / 2);
         g->fnu = (units + kFusedThreads - 1) / kFusedThreads;
         g->fitems = (long long)pb->n * g->fnbh * g->fnbw * ((pb->k + 63) / 64);
-        // buffer-load offsets are 32-bit: one image with 32 channels of slack under 2^31
-        // bytes (out-of-image offsets are placed at 2^31 and above), U under 2^31 bytes
+        // buffer offsets are 32-bit: one input image with 32 channels of slack under 2^31
+        // bytes (out-of-image offsets are placed at 2^31 and above), one output image and U
+        // under 2^31 bytes
         const long long img_bytes = ((long long)pb->c + 32) * pb->h * pb->w * 2;
+        const long long out_img_bytes = (long long)pb->k * g->Oh * g->Ow * 2;
         g->fused_ok = g->fnu <= 2 && g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
-                      (long long)g->u_bytes < (1LL << 31);
+                      out_img_bytes < (1LL << 31) && (long long)g->u_bytes < (1LL << 31);
     }
     return CONV2D_OK;
 }

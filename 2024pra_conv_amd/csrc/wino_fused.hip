@@ -321,13 +321,28 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     const int m = blk >> 1, tb = blk & 1;
     const int t = tb * 32 + (lane & 31);
     const int th = th0 + t / FBW, tw = tw0 + (t - (t / FBW) * FBW);
+    // Output of this image through a buffer resource: 32-bit byte offsets (the host checks
+    // K*Oh*Ow*2 < 2^31), two FP16 values of a row in one 4-byte store when every row starts
+    // 4-byte aligned (even Ow, aligned base; then a tile's second column is always inside).
+    const int Oh = prm.Oh, Ow = prm.Ow;
+    const uint32_t plane = (uint32_t)Oh * (uint32_t)Ow;
+    const __amdgpu_buffer_rsrc_t out_rsrc =
+        buffer_rsrc((const half_t*)prm.pout + (size_t)img * K * plane, (uint32_t)K * plane * 2u);
+    const bool pairs = (Ow & 1) == 0 && ((uintptr_t)prm.pout & 3) == 0;
     if (th < prm.tiles_h && tw < prm.tiles_w) {
-        TileCoord tc;
-        tc.img = img;
-        tc.oh = 2 * th;
-        tc.ow = 2 * tw;
-        tc.r1 = tc.oh + 1 < prm.Oh;
-        tc.c1 = tc.ow + 1 < prm.Ow;
+        const bool r1 = 2 * th + 1 < Oh, c1 = 2 * tw + 1 < Ow;
+        const uint32_t pix = ((uint32_t)(2 * th) * (uint32_t)Ow + (uint32_t)(2 * tw)) * 2u;
+        const uint32_t row_bytes = (uint32_t)Ow * 2u;
+        auto put_row = [&](uint32_t off, float ya, float yb) {
+            const uint16_t ha = __builtin_bit_cast(uint16_t, (half_t)ya);
+            const uint16_t hb = __builtin_bit_cast(uint16_t, (half_t)yb);
+            if (pairs) {
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)ha | ((uint32_t)hb << 16), out_rsrc, (int)off, 0, 0);
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b16(ha, out_rsrc, (int)off, 0, 0);
+                if (c1) __builtin_amdgcn_raw_buffer_store_b16(hb, out_rsrc, (int)(off + 2u), 0, 0);
+            }
+        };
 #pragma unroll
         for (int eh = 0; eh < 2; ++eh) {
             const int e4 = 2 * (wave & 1) + eh;
@@ -342,11 +357,20 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
                 const int e = e4 * 4 + q;
                 const int k = kblk * 64 + m * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
                 if (k >= K) continue;
-                const float y00 = (z[0][0][q] + z[1][0][q]) + z[2][0][q];
-                const float y01 = (z[0][1][q] + z[1][1][q]) + z[2][1][q];
-                const float y10 = (z[1][0][q] - z[2][0][q]) - z[3][0][q];
-                const float y11 = (z[1][1][q] - z[2][1][q]) - z[3][1][q];
-                store_tile(prm, tc, k, y00, y01, y10, y11);
+                float y00 = (z[0][0][q] + z[1][0][q]) + z[2][0][q];
+                float y01 = (z[0][1][q] + z[1][1][q]) + z[2][1][q];
+                float y10 = (z[1][0][q] - z[2][0][q]) - z[3][0][q];
+                float y11 = (z[1][1][q] - z[2][1][q]) - z[3][1][q];
+                if (prm.bias || prm.act) {
+                    const float bk = epilogue_bias(prm, k);
+                    y00 = epilogue(prm, y00, bk);
+                    y01 = epilogue(prm, y01, bk);
+                    y10 = epilogue(prm, y10, bk);
+                    y11 = epilogue(prm, y11, bk);
+                }
+                const uint32_t off = (uint32_t)k * plane * 2u + pix;
+                put_row(off, y00, y01);
+                if (r1) put_row(off + row_bytes, y10, y11);
             }
         }
     }

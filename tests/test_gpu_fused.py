@@ -25,7 +25,8 @@ SHAPES = [
     (3, 64, 8, 64, 96, 1, 1),      # wide: 4x32 tiles -> 4x16 blocks
     (1, 33, 70, 6, 16, 1, 0),      # tall: 35x2 tiles -> 16x4 blocks, q = 0
     (2, 96, 28, 28, 256, 1, 1),    # four k-blocks, ragged blocks of a 14x14 tile grid
-    (1, 8, 4, 130, 32, 1, 1),      # 2x65 tiles -> 2x32 blocks (four staging units a thread)
+    (1, 8, 4, 130, 32, 1, 1),      # 2x65 tiles -> 2x32 blocks (the widest patch: 816 units)
+    (1, 512, 12, 12, 64, 1, 1),    # 16 channel chunks (the double-buffered patch alternates 16x)
     # odd widths: 2-byte staging loads, pixel pairs straddling the right edge
     (1, 16, 9, 11, 8, 1, 1),
     (2, 96, 27, 27, 96, 1, 1),
@@ -65,6 +66,32 @@ def test_fused_matches_oracle(shape, runner, cuda):
     m = wg.error_metrics(y, ref)
     print(f"  max_rel={m['max_rel']:.3e} rms_rel={m['rms_rel']:.3e}", flush=True)
     assert m["finite"] and m["max_rel"] <= 1e-2
+
+
+def test_fused_unaligned_input_base_takes_2byte_loads(runner, wc, cuda):
+    """An even-width input whose base address is 2 bytes past a 4-byte boundary cannot use
+    4-byte pixel-pair loads: the fused kernel takes its 2-byte staging path and still meets
+    the oracle bound."""
+    n, c, h, w, k, p, q = 2, 40, 14, 22, 70, 1, 1
+    print("case unaligned base", flush=True)
+    geo = wg.Geometry(n, c, h, w, k, p, q)
+    x = wg.uniform_pm1(950, (n, c, h, w))
+    wt = wg.uniform_pm1(951, (k, c, 3, 3))
+    flat = torch.empty(n * c * h * w + 1, dtype=torch.float16, device=cuda)
+    X = flat[1:].view(n, c, h, w)
+    X.copy_(_to_dev(x, cuda))
+    assert X.data_ptr() % 4 == 2
+    conv = runner.WinogradConv(X, _to_dev(wt, cuda), pad=(p, q))
+    assert conv.form() == wc.FORM_FUSED
+    conv.prepare_weights()
+    conv.out.fill_(float("nan"))
+    conv.launch_slots()
+    torch.cuda.synchronize()
+    y = conv.out.cpu().numpy().astype(np.float64)
+    y_model, scale = wg.fused_gpu_model(x, wt, geo)
+    e = (c + 16) * 2.0 ** -24 * scale
+    tol = 2.0 ** -11 * (np.abs(y_model) + e) + e + 2.0 ** -25
+    assert np.all(np.abs(y - y_model) <= tol), f"max excess {np.max(np.abs(y - y_model) - tol)}"
 
 
 def test_fused_block_shapes_cover_every_variant(runner, cuda):
