@@ -190,6 +190,18 @@ __device__ inline void store_tile(const mykernelParamType& prm, const TileCoord&
     }
     const int Oh = prm.Oh, Ow = prm.Ow;
     global_half_t* o = (global_half_t*)prm.pout + (((size_t)tc.img * prm.k + k) * Oh + tc.oh) * Ow + tc.ow;
+    if ((Ow & 1) == 0 && ((uintptr_t)prm.pout & 3) == 0) {
+        // every row starts 4-byte aligned and a tile's second column is inside: one 4-byte
+        // store per row
+        typedef __attribute__((address_space(1))) uint32_t global_u32_t;
+        auto pack = [](float a, float b) {
+            return (uint32_t)__builtin_bit_cast(uint16_t, (half_t)a) |
+                   ((uint32_t)__builtin_bit_cast(uint16_t, (half_t)b) << 16);
+        };
+        *(global_u32_t*)o = pack(y00, y01);
+        if (tc.r1) *(global_u32_t*)(o + Ow) = pack(y10, y11);
+        return;
+    }
     o[0] = (half_t)y00;
     if (tc.c1) o[1] = (half_t)y01;
     if (tc.r1) {
