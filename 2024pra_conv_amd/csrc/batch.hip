@@ -1,8 +1,8 @@
 // Multi-layer (batched) form: many conv layers of uneven shape. Staged-form layers run as one
 // persistent input transform over a (layer, band, channel block, image) work queue and one
 // grouped GEMM launch per tile configuration in use over (layer, k-block, column block)
-// items; fused-form layers as one grouped fused launch per staging-unit count over (layer,
-// 64-channel block, tile block) items. The whole forward pass can be captured once into a
+// items; fused-form layers as one grouped fused launch per (staging width, tile-block shape)
+// over (layer, 64-channel block, tile block) items. The whole forward pass can be captured once into a
 // hipGraph and replayed with a single graph launch.
 //
 // No counterpart in the reference, whose harness runs one shape per process
@@ -23,6 +23,7 @@ struct conv2d_batch {
         int cfg;      // staged: GEMM tile configuration of every layer in the segment
         int fused;    // 1 for a fused segment, 0 for a staged one
         bool d16;     // fused: 2-byte staging loads
+        int shape;    // fused: 2 * tile-block shape index (fused_shape_index) + column shift
         int first;    // first descriptor of the segment
         int count;    // layers in the segment
         int items;    // work items of the segment's queue
@@ -47,7 +48,7 @@ namespace {
 
 // Rebuilds the device descriptors from layers + cfg + form. Staged layers come first, grouped
 // by tile configuration (one grouped GEMM launch per group); fused layers follow, grouped by
-// staging-unit count (one grouped fused launch per group). Inside a group layers are queued
+// staging width and tile-block shape (one grouped fused launch per group). Inside a group layers are queued
 // by decreasing per-item cost so large items start first (the order changes no result).
 int upload(conv2d_batch* b) {
     const int nl = (int)b->layers.size();
@@ -57,7 +58,7 @@ int upload(conv2d_batch* b) {
         if (rc) return rc;
     }
     auto key = [&](int i) {
-        return b->form[i] == CONV2D_FORM_FUSED ? 1000 + (geo[i].fd16 ? 1 : 0) : b->cfg[i];
+        return b->form[i] == CONV2D_FORM_FUSED ? 1000 + 4 * fused_shape_index(geo[i].fbh, geo[i].fbw) + 2 * geo[i].fsh + (geo[i].fd16 ? 1 : 0) : b->cfg[i];
     };
     auto cost = [&](int i) {
         if (b->form[i] == CONV2D_FORM_FUSED) return (double)b->layers[i].c;
@@ -83,9 +84,10 @@ int upload(conv2d_batch* b) {
         const bool fused = b->form[i] == CONV2D_FORM_FUSED;
         const int fz = fused ? 1 : 0;
         const bool d16 = fused && g.fd16;
+        const int shape = fused ? 2 * fused_shape_index(g.fbh, g.fbw) + g.fsh : 0;
         if (b->segs.empty() || b->segs.back().fused != fz || b->segs.back().d16 != d16 ||
-            (!fused && b->segs.back().cfg != cfg))
-            b->segs.push_back(conv2d_batch::Segment{cfg, fz, d16, qi, 0, 0, 0u});
+            b->segs.back().shape != shape || (!fused && b->segs.back().cfg != cfg))
+            b->segs.push_back(conv2d_batch::Segment{cfg, fz, d16, shape, qi, 0, 0, 0u});
         conv2d_batch::Segment& seg = b->segs.back();
         BatchLayer& bl = b->host[qi];
         wino::fill_param(&pb, g, cfg, &bl.prm);
@@ -135,7 +137,7 @@ void drop_graph(conv2d_batch* b) {
 // Staged layers: resets the input-transform queue with a kernel (not a memset node: it is
 // captured into graphs), one persistent input-transform launch over the staged layers and one
 // grouped GEMM launch per tile configuration in use. Fused layers: one grouped fused launch per
-// staging-unit count.
+// (staging width, tile-block shape).
 int launch_forward(conv2d_batch* b, hipStream_t st) {
     const int nseg = (int)b->segs.size();
     if (b->s2_total > 0) {
@@ -150,7 +152,7 @@ int launch_forward(conv2d_batch* b, hipStream_t st) {
         int count = seg.count, items = seg.items;
         void* args[3] = {(void*)&layers, (void*)&count, (void*)&items};
         const hipError_t e =
-            seg.fused ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.d16), dim3(seg.items), dim3(fused_threads()), args, seg.lds, st)
+            seg.fused ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.d16, seg.shape >> 1, seg.shape & 1), dim3(seg.items), dim3(fused_threads()), args, seg.lds, st)
                    : hipLaunchKernel(kGemmConfigs[seg.cfg].grouped_kernel, dim3(seg.items),
                                      dim3(kGemmConfigs[seg.cfg].threads), args, 0, st);
         if (e != hipSuccess) return CONV2D_ERR_HIP;

@@ -83,10 +83,9 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
     } else {
         // fused form: the 64-tile block shape that pads the tile grid least (ties: the
         // squarer block, whose input patch is smaller)
-        const int shapes[4][2] = {{8, 8}, {4, 16}, {16, 4}, {2, 32}};
         long long best = -1;
         for (int i = 0; i < 4; ++i) {
-            const int bh = shapes[i][0], bw = shapes[i][1];
+            const int bh = kFusedShapes[i][0], bw = kFusedShapes[i][1];
             const long long padded = (long long)((g->tiles_h + bh - 1) / bh) * bh * ((g->tiles_w + bw - 1) / bw) * bw;
             if (best < 0 || padded < best) {
                 best = padded;
@@ -99,7 +98,7 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         g->fsh = (int)(pb->q & 1);
         g->fd16 = (pb->w % 2) != 0 || ((uintptr_t)pb->in & 3) != 0;
         g->fpr = 2 * g->fbh + 2;
-        g->fpc = (2 * g->fbw + 2 + g->fsh + 1) & ~1;
+        g->fpc = fused_patch_cols(g->fbw);
         const int units = 4 * g->fpr * (g->fpc / 2);
         g->fnu = (units + kFusedThreads - 1) / kFusedThreads;
         g->fitems = (long long)pb->n * g->fnbh * g->fnbw * ((pb->k + 63) / 64);
@@ -258,7 +257,7 @@ Launch slot_filter(const Geometry& g) {
 // input patches during the channel loop and the 128 KiB Z exchange after it.
 Launch slot_fused(const Geometry& g) {
     Launch L;
-    L.kernel = fused_kernel_ptr(g.fd16);
+    L.kernel = fused_kernel_ptr(g.fd16, fused_shape_index(g.fbh, g.fbw), g.fsh);
     L.grid = dim3((unsigned)g.fitems, 1, 1);
     L.block = dim3((unsigned)fused_threads(), 1, 1);
     const unsigned patches = 2u * 4u * (unsigned)g.fpr * (unsigned)g.fpc * 16u;

@@ -69,10 +69,10 @@ __device__ inline half8 sub8(half8 x, half8 y, half8 m1) { return __builtin_elem
 __device__ inline uint32_t pack_lo(uint32_t x, uint32_t y) { return __builtin_amdgcn_perm(y, x, 0x05040100u); }
 __device__ inline uint32_t pack_hi(uint32_t x, uint32_t y) { return __builtin_amdgcn_perm(y, x, 0x07060302u); }
 
-template <int NU, bool D16, int NT>
+template <int NU, bool D16, int NT, int PR, int PC>
 __device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw0, Staging<NU>& st) {
-    const int PR = prm.fpr, PC = prm.fpc, PH = PC >> 1;
-    const int nunits = 4 * PR * PH;
+    constexpr int PH = PC >> 1;
+    constexpr int nunits = 4 * PR * PH;
     const int H = prm.h, W = prm.w;
 #pragma unroll
     for (int i = 0; i < NU; ++i) {
@@ -137,11 +137,17 @@ __device__ inline void stage_store(const Staging<NU>& st, half_t* buf, int half_
 
 }  // namespace
 
-// One work item: 64 output channels (block kblk) x one 64-tile block (gblk, image-major) of the
-// problem in prm, by the waves of one half (BH) of a 512-thread workgroup, in the launch's
-// dynamic LDS.
-template <bool D16, int BH>
+// One work item: 64 output channels (block kblk) x one FBH x FBW = 64-tile block (gblk,
+// image-major) of the problem in prm, by the waves of one half (BH) of a 512-thread
+// workgroup, in the launch's dynamic LDS. The patch is PR = 2*FBH + 2 rows by
+// PC = 2*FBW + 4 columns (room for the column shift FSH = q & 1), so every LDS offset
+// except the lane's two bases (window rows rp, rm) and the chunk's buffer is a
+// compile-time constant.
+template <bool D16, int BH, int FBH, int FBW, int FSH>
 __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gblk) {
+    constexpr int PR = 2 * FBH + 2, PC = 2 * FBW + 4;
+    constexpr int PB = 4 * PR * PC * 8;  // halves per patch buffer
+    static_assert(FBH * FBW == 64, "64 tiles per block");
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
     constexpr int NU = 2;  // 512 threads x 2 units cover the largest patch (1024 units)
     const int tid = threadIdx.x;
@@ -151,14 +157,13 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
 
     const int K = prm.k, C = prm.c;
     const int KB = prm.KB32, CB = prm.CB, C16 = 2 * CB;
-    const int FBW = prm.fbw, PR = prm.fpr, PC = prm.fpc;
     const int nbw = prm.fnbw, nbh = prm.fnbh;
     const int img = gblk / (nbh * nbw);
     const int brem = gblk - img * (nbh * nbw);
-    const int th0 = (brem / nbw) * prm.fbh;
+    const int th0 = (brem / nbw) * FBH;
     const int tw0 = (brem - (brem / nbw) * nbw) * FBW;
     const int ih0 = 2 * th0 - prm.p;
-    const int iw0 = 2 * tw0 - prm.q - prm.fsh;
+    const int iw0 = 2 * tw0 - prm.q - FSH;
     const half_t* in_img = (const half_t*)prm.pin + (size_t)img * C * prm.h * prm.w;
     const uint32_t chan_bytes = (uint32_t)prm.h * prm.w * 2;
     const __amdgpu_buffer_rsrc_t in_rsrc = buffer_rsrc(in_img, (uint32_t)C * chan_bytes);
@@ -182,28 +187,25 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
                 }
     };
 
-    int woff[2];
+    // rows of the window that row a of B^T combines: s = d[rp] + sg * d[rm], sg = +-1 (exact,
+    // so the fma rounds once like the addition or subtraction it stands for)
+    const int rp = a == 0 ? 0 : (a == 2 ? 2 : 1);
+    const int rm = a == 0 ? 2 : (a == 2 ? 1 : (a == 1 ? 2 : 3));
+    // LDS offsets (halves) of this lane's window rows rp and rm in channel group lane/32 of
+    // buffer 0: tile t = tb*32 + lane%32 at patch row 2*(t / FBW), column pair t % FBW; window
+    // column j then sits at parity (FSH + j) & 1, position (FSH + j) / 2 past the pair
+    int rbase[2][2];
 #pragma unroll
     for (int tb = 0; tb < 2; ++tb) {
         const int t = tb * 32 + (lane & 31);
-        const int tr = t / FBW, tc = t - (t / FBW) * FBW;
-        woff[tb] = (2 * tr) * PC + tc;
+        const int lb = ((lane >> 5) * PR * PC + 2 * (t / FBW) * PC + t % FBW) * 8;
+        rbase[tb][0] = lb + rp * PC * 8;
+        rbase[tb][1] = lb + rm * PC * 8;
     }
-    // window columns BH + jj (jj = 0..2): parity (fsh + j) & 1, position tc + (fsh + j) / 2
-    int colj[3];
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj) {
-        const int j = BH + jj;
-        colj[jj] = ((prm.fsh + j) & 1) * (PC / 2) + ((prm.fsh + j) >> 1);
-    }
-    const int rp = a == 0 ? 0 : (a == 2 ? 2 : 1);
-    const int rm = a == 0 ? 2 : (a == 2 ? 1 : (a == 1 ? 2 : 3));
     const half_t sgh = a == 1 ? (half_t)1.0f : (half_t)-1.0f;
     const half8 sg = {sgh, sgh, sgh, sgh, sgh, sgh, sgh, sgh};
     half8 m1 = {-1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16};
     asm volatile("" : "+s"(m1));
-
-    const int PB = 4 * PR * PC * 8;
 
     floatx16 acc[2][2][2];
 #pragma unroll
@@ -218,7 +220,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     // the next chunk's patch loads are in flight during this chunk's MFMAs; U has one
     // register set (two waves per SIMD hide its L2 latency), reloaded after its last use
     Staging<NU> st;
-    stage_setup<NU, D16, 512>(prm, ih0, iw0, st);
+    stage_setup<NU, D16, 512, PR, PC>(prm, ih0, iw0, st);
     stage_load<NU, D16>(in_rsrc, 0u, chan_bytes, st);
     stage_store<NU>(st, lds, PC / 2);
     half8 ucur[8];
@@ -226,6 +228,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     __syncthreads();
 
     auto chunk = [&](int ch) {
+        const int buf_ofs = (ch & 1) * PB;
         const bool more = ch + 1 < CB;
         // Wait for this chunk's U here, before the next chunk's patch loads are issued: the
         // memory counter drains in issue order, so a wait for U placed after those loads (at the
@@ -233,18 +236,19 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         asm volatile("" ::"v"(ucur[0]), "v"(ucur[1]), "v"(ucur[2]), "v"(ucur[3]), "v"(ucur[4]), "v"(ucur[5]),
                      "v"(ucur[6]), "v"(ucur[7]));
         if (more) stage_load<NU, D16>(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes, st);
-        const half_t* buf = lds + (ch & 1) * PB;
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-            const int gsel = ks * 2 + (lane >> 5);
 #pragma unroll
             for (int tb = 0; tb < 2; ++tb) {
-                const half_t* wp = buf + ((size_t)gsel * PR * PC + woff[tb]) * 8;
+                const half_t* wpp = lds + buf_ofs + rbase[tb][0] + ks * 2 * PR * PC * 8;
+                const half_t* wpm = lds + buf_ofs + rbase[tb][1] + ks * 2 * PR * PC * 8;
                 half8 s3[3];
 #pragma unroll
                 for (int jj = 0; jj < 3; ++jj) {
-                    const half8 dp = *(const half8*)(wp + (rp * PC + colj[jj]) * 8);
-                    const half8 dm = *(const half8*)(wp + (rm * PC + colj[jj]) * 8);
+                    const int j = BH + jj;
+                    const int col = (((FSH + j) & 1) * (PC / 2) + ((FSH + j) >> 1)) * 8;
+                    const half8 dp = *(const half8*)(wpp + col);
+                    const half8 dm = *(const half8*)(wpm + col);
                     s3[jj] = __builtin_elementwise_fma(dm, sg, dp);
                 }
                 half8 v[2];
@@ -265,7 +269,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         }
         if (more) {
             load_u(ucur, ch + 1);
-            stage_store<NU>(st, lds + ((ch + 1) & 1) * PB, PC / 2);
+            stage_store<NU>(st, lds + (PB - buf_ofs), PC / 2);
         }
         __syncthreads();
     };
@@ -320,7 +324,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     const int blk = wave >> 1;  // output block of this wave, e/4 groups 2*(wave&1) .. +1
     const int m = blk >> 1, tb = blk & 1;
     const int t = tb * 32 + (lane & 31);
-    const int th = th0 + t / FBW, tw = tw0 + (t - (t / FBW) * FBW);
+    const int th = th0 + t / FBW, tw = tw0 + t % FBW;
     // Output of this image through a buffer resource: 32-bit byte offsets (the host checks
     // K*Oh*Ow*2 < 2^31), two FP16 values of a row in one 4-byte store when every row starts
     // 4-byte aligned (even Ow, aligned base; then a tile's second column is always inside).
@@ -376,26 +380,35 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     }
 }
 
-template <bool D16>
+// the four 64-tile block shapes, indexed as fused_shape_index numbers them
+template <int S> struct BlockShape;
+template <> struct BlockShape<0> { static constexpr int h = 8, w = 8; };
+template <> struct BlockShape<1> { static constexpr int h = 4, w = 16; };
+template <> struct BlockShape<2> { static constexpr int h = 16, w = 4; };
+template <> struct BlockShape<3> { static constexpr int h = 2, w = 32; };
+
+// S = 2 * (block shape index) + column shift FSH
+template <bool D16, int S>
 __device__ inline void fused_item_dispatch(const mykernelParamType& prm, int kblk, int gblk) {
+    constexpr int FBH = BlockShape<S / 2>::h, FBW = BlockShape<S / 2>::w, FSH = S & 1;
     // waves 0-3 take BH = 0, waves 4-7 BH = 1 (wave-uniform; both paths pass the same barriers)
     if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 8) == 0)
-        fused_item<D16, 0>(prm, kblk, gblk);
+        fused_item<D16, 0, FBH, FBW, FSH>(prm, kblk, gblk);
     else
-        fused_item<D16, 1>(prm, kblk, gblk);
+        fused_item<D16, 1, FBH, FBW, FSH>(prm, kblk, gblk);
 }
 
-template <bool D16>
+template <bool D16, int S>
 __global__ void __launch_bounds__(512, 1) wino_fused_kernel(mykernelParamType prm) {
     int kblk, gblk;
     xcd_remap(blockIdx.x, gridDim.x, (prm.k + 63) / 64, &kblk, &gblk);
-    fused_item_dispatch<D16>(prm, kblk, gblk);
+    fused_item_dispatch<D16, S>(prm, kblk, gblk);
 }
 
 // Multi-layer form: one launch over the fused items of several layers (descriptor i owns items
 // [s3_first, s3_first + s3_items), nk = 64-channel blocks), every layer with the same staging
-// load width.
-template <bool D16>
+// load width, block shape and column shift.
+template <bool D16, int S>
 __global__ void __launch_bounds__(512, 1)
 wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
     int item, unused;
@@ -407,20 +420,42 @@ wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, in
     }
     const BatchLayer& bl = layers[lo];
     const int local = item - bl.s3_first;
-    fused_item_dispatch<D16>(bl.prm, local % bl.nk, local / bl.nk);
+    fused_item_dispatch<D16, S>(bl.prm, local % bl.nk, local / bl.nk);
 }
 
-template __global__ void wino_fused_kernel<false>(mykernelParamType);
-template __global__ void wino_fused_kernel<true>(mykernelParamType);
-template __global__ void wino_fused_grouped_kernel<false>(const BatchLayer*, int, int);
-template __global__ void wino_fused_grouped_kernel<true>(const BatchLayer*, int, int);
+#define FUSED_INSTANCES(D16, S)                                                      \
+    template __global__ void wino_fused_kernel<D16, S>(mykernelParamType);           \
+    template __global__ void wino_fused_grouped_kernel<D16, S>(const BatchLayer*, int, int);
+FUSED_INSTANCES(false, 0)
+FUSED_INSTANCES(false, 1)
+FUSED_INSTANCES(false, 2)
+FUSED_INSTANCES(false, 3)
+FUSED_INSTANCES(false, 4)
+FUSED_INSTANCES(false, 5)
+FUSED_INSTANCES(false, 6)
+FUSED_INSTANCES(false, 7)
+FUSED_INSTANCES(true, 0)
+FUSED_INSTANCES(true, 1)
+FUSED_INSTANCES(true, 2)
+FUSED_INSTANCES(true, 3)
+FUSED_INSTANCES(true, 4)
+FUSED_INSTANCES(true, 5)
+FUSED_INSTANCES(true, 6)
+FUSED_INSTANCES(true, 7)
 
 int fused_threads() { return kFusedThreads; }
 
-const void* fused_kernel_ptr(bool d16) {
-    return d16 ? (const void*)&wino_fused_kernel<true> : (const void*)&wino_fused_kernel<false>;
+#define FUSED_ROW(K, D16)                                                                              \
+    {(const void*)&K<D16, 0>, (const void*)&K<D16, 1>, (const void*)&K<D16, 2>, (const void*)&K<D16, 3>, \
+     (const void*)&K<D16, 4>, (const void*)&K<D16, 5>, (const void*)&K<D16, 6>, (const void*)&K<D16, 7>}
+
+const void* fused_kernel_ptr(bool d16, int shape, int fsh) {
+    static const void* const table[2][8] = {FUSED_ROW(wino_fused_kernel, false), FUSED_ROW(wino_fused_kernel, true)};
+    return table[d16 ? 1 : 0][((shape & 3) << 1) | (fsh & 1)];
 }
 
-const void* fused_grouped_kernel_ptr(bool d16) {
-    return d16 ? (const void*)&wino_fused_grouped_kernel<true> : (const void*)&wino_fused_grouped_kernel<false>;
+const void* fused_grouped_kernel_ptr(bool d16, int shape, int fsh) {
+    static const void* const table[2][8] = {FUSED_ROW(wino_fused_grouped_kernel, false),
+                                            FUSED_ROW(wino_fused_grouped_kernel, true)};
+    return table[d16 ? 1 : 0][((shape & 3) << 1) | (fsh & 1)];
 }
