@@ -276,50 +276,56 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     for (int ch = 0; ch < CB; ++ch) chunk(ch);
 
     // ---- epilogue; LDS Z layout (floats): [a][j][block = m*2 + tb][e/4][lane][4] ----
+    // Row a's fold Z_0 = (M0 + M1) + M2, Z_1 = (M1 - M2) - M3 needs both halves of the row. For
+    // channel block m = 0 the BH = 0 wave hands over M0 + M1 and M1 and the BH = 1 wave folds;
+    // for m = 1 the BH = 1 wave hands over M2 and M3 and the BH = 0 wave folds, so both halves
+    // work in both phases. Z overwrites the handed-over values in place.
     float* Z = (float*)lds;
-    if (BH == 0) {
+    constexpr int give = BH;      // channel block whose values this wave hands over
+    constexpr int fold = 1 - BH;  // channel block this wave folds
+    auto zslot = [&](int j, int m, int tb, int e4) {
+        return Z + ((((a * 2 + j) * 4 + m * 2 + tb) * 4 + e4) * 64 + lane) * 4;
+    };
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+    for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-            for (int tb = 0; tb < 2; ++tb)
+        for (int e4 = 0; e4 < 4; ++e4) {
+            floatx4 v0, v1;
 #pragma unroll
-                for (int e4 = 0; e4 < 4; ++e4) {
-                    floatx4 p, m1v;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int e = e4 * 4 + q;
-                        p[q] = acc[0][m][tb][e] + acc[1][m][tb][e];  // M0 + M1
-                        m1v[q] = acc[1][m][tb][e];                   // M1
-                    }
-                    const int blk = m * 2 + tb;
-                    *(floatx4*)(Z + ((((a * 2 + 0) * 4 + blk) * 4 + e4) * 64 + lane) * 4) = p;
-                    *(floatx4*)(Z + ((((a * 2 + 1) * 4 + blk) * 4 + e4) * 64 + lane) * 4) = m1v;
-                }
-    }
+            for (int q = 0; q < 4; ++q) {
+                const int e = e4 * 4 + q;
+                const float x0 = acc[0][give][tb][e], x1 = acc[1][give][tb][e];
+                v0[q] = BH == 0 ? x0 + x1 : x0;  // M0 + M1 | M2
+                v1[q] = x1;                      // M1      | M3
+            }
+            *(floatx4*)zslot(0, give, tb, e4) = v0;
+            *(floatx4*)zslot(1, give, tb, e4) = v1;
+        }
     __syncthreads();
-    if (BH == 1) {
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+    for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-            for (int tb = 0; tb < 2; ++tb)
+        for (int e4 = 0; e4 < 4; ++e4) {
+            float* z0p = zslot(0, fold, tb, e4);
+            float* z1p = zslot(1, fold, tb, e4);
+            const floatx4 h0 = *(const floatx4*)z0p, h1 = *(const floatx4*)z1p;
+            floatx4 z0, z1;
 #pragma unroll
-                for (int e4 = 0; e4 < 4; ++e4) {
-                    const int blk = m * 2 + tb;
-                    float* z0p = Z + ((((a * 2 + 0) * 4 + blk) * 4 + e4) * 64 + lane) * 4;
-                    float* z1p = Z + ((((a * 2 + 1) * 4 + blk) * 4 + e4) * 64 + lane) * 4;
-                    const floatx4 p = *(const floatx4*)z0p, m1v = *(const floatx4*)z1p;
-                    floatx4 z0, z1;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int e = e4 * 4 + q;
-                        const float M2 = acc[0][m][tb][e], M3 = acc[1][m][tb][e];
-                        z0[q] = p[q] + M2;              // (M0 + M1) + M2
-                        z1[q] = (m1v[q] - M2) - M3;     // (M1 - M2) - M3
-                    }
-                    *(floatx4*)z0p = z0;
-                    *(floatx4*)z1p = z1;
-                }
-    }
+            for (int q = 0; q < 4; ++q) {
+                const int e = e4 * 4 + q;
+                const float x0 = acc[0][fold][tb][e], x1 = acc[1][fold][tb][e];
+                // BH = 1 holds M2, M3 and received M0 + M1, M1; BH = 0 holds M0, M1 and
+                // received M2, M3
+                const float P = BH == 1 ? h0[q] : x0 + x1;
+                const float M1 = BH == 1 ? h1[q] : x1;
+                const float M2 = BH == 1 ? x0 : h0[q];
+                const float M3 = BH == 1 ? x1 : h1[q];
+                z0[q] = P + M2;          // (M0 + M1) + M2
+                z1[q] = (M1 - M2) - M3;  // (M1 - M2) - M3
+            }
+            *(floatx4*)z0p = z0;
+            *(floatx4*)z1p = z1;
+        }
     __syncthreads();
     const int blk = wave >> 1;  // output block of this wave, e/4 groups 2*(wave&1) .. +1
     const int m = blk >> 1, tb = blk & 1;
