@@ -222,9 +222,9 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     Staging<NU> st;
     stage_setup<NU, D16, 512, PR, PC>(prm, ih0, iw0, st);
     stage_load<NU, D16>(in_rsrc, 0u, chan_bytes, st);
-    stage_store<NU>(st, lds, PC / 2);
     half8 ucur[8];
-    load_u(ucur, 0);
+    load_u(ucur, 0);  // in flight while the first patch is stored
+    stage_store<NU>(st, lds, PC / 2);
     __syncthreads();
 
     auto chunk = [&](int ch) {
