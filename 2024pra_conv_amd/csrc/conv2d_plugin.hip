@@ -178,11 +178,11 @@ bool use_fused(const problem_t* pb, const Geometry& g) {
     if (g.direct || !g.fused_ok) return false;
     if (tuned_config_lookup(pb) >= 0) return false;
     // Measured on 45 shapes (both forms timed after each had run 50 ms, so at sustained
-    // clocks): with K <= 192 the fused form was faster on every shape (up to 2.6x); with
-    // C*K <= 256^2 it was faster on average (0.44-1.17x of the staged time, mean 0.91); with a
-    // larger C*K the staged GEMM's long reduction wins from N*T*K*C ~ 8e8 (up to 1.39x).
+    // clocks): with K <= 320 the fused form was faster on all but three (at most 9% slower,
+    // up to 2.9x faster); with K >= 384 the staged GEMM's 128-row tiles win from
+    // N*T*K*C ~ 1.6e9 (up to 1.3x), and from 8e8 when C < K.
     const double work = (double)g.NT * (double)pb->k * (double)pb->c;
-    return pb->k <= 192 || work < 8e8 || (double)pb->c * (double)pb->k <= 65536.0;
+    return pb->k <= 320 || work < 8e8 || (work < 1.2e9 && pb->c >= pb->k);
 }
 
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {

@@ -342,10 +342,11 @@ def test_batched_autotune_keeps_results(runner, wc, cuda):
 
 
 def test_batched_mixes_fused_and_staged_layers(runner, wc, cuda):
-    """A batch with both forms (small, odd-width and K <= 192 layers fused; a 12x512x56x56 K=256
-    layer, N*T*K*C = 1.23e9, staged) equals separate launches, in eager and graph launches."""
+    """A batch with both forms (small, odd-width and K <= 320 layers fused; an 8x256x56x56
+    K=512 layer, N*T*K*C = 8.2e8 with C < K, staged) equals separate launches, in eager and
+    graph launches."""
     _log("case batched mixed forms")
-    shapes = [(1, 64, 14, 64), (12, 512, 56, 256), (1, 96, 27, 128), (3, 130, 13, 140), (1, 16, 28, 16)]
+    shapes = [(1, 64, 14, 64), (8, 256, 56, 512), (1, 96, 27, 128), (3, 130, 13, 140), (1, 16, 28, 16)]
     convs = _batch_convs(runner, cuda, shapes, seed=800)
     b = runner.Batch(convs)
     forms = b.layer_forms()
