@@ -90,7 +90,7 @@ def test_getkernelinfo_fills_slots_and_param(wc):
     for i in range(1, 5):
         s = ki.slot(i)
         assert s["kernel"] and all(s["grid"]) and all(s["block"])
-    # this layer (C*K = 256^2) takes the fused form: filter transform, fused kernel, two empty
+    # this layer (K = 256 <= 320) takes the fused form: filter transform, fused kernel, two empty
     assert prm.form == wc.FORM_FUSED
     assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 4, 1) and ki.slot(1)["block"] == (64, 1, 1)
     assert ki.slot(2)["grid"] == (32 * prm.fnbh * prm.fnbw * 4, 1, 1) and ki.slot(2)["block"] == (512, 1, 1)
