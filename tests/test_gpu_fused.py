@@ -32,6 +32,15 @@ SHAPES = [
     (2, 96, 27, 27, 96, 1, 1),
     (1, 40, 13, 13, 20, 0, 0),
     (1, 8, 4, 131, 32, 2, 2),      # odd width with 2x32 blocks and pad 2
+    # the remaining kernel instances (block shape x column shift q & 1 x staging width)
+    (3, 64, 8, 64, 96, 0, 0),      # 4x16, shift 0, 4-byte
+    (2, 32, 8, 63, 64, 1, 1),      # 4x16, shift 1, 2-byte
+    (1, 33, 70, 6, 16, 1, 1),      # 16x4, shift 1, 4-byte
+    (1, 24, 70, 7, 16, 0, 0),      # 16x4, shift 0, 2-byte
+    (1, 24, 70, 7, 16, 1, 1),      # 16x4, shift 1, 2-byte
+    (1, 8, 4, 130, 32, 0, 0),      # 2x32, shift 0, 4-byte
+    (1, 8, 4, 131, 32, 0, 0),      # 2x32, shift 0, 2-byte
+    (1, 8, 4, 131, 32, 1, 1),      # 2x32, shift 1, 2-byte
 ]
 
 
@@ -95,16 +104,18 @@ def test_fused_unaligned_input_base_takes_2byte_loads(runner, wc, cuda):
 
 
 def test_fused_block_shapes_cover_every_variant(runner, cuda):
-    """The shapes above exercise each 64-tile block shape (every patch needs 2 staging units
-    per thread of the 512-thread workgroup: 648-816 units of 8 channels x 2 pixels)."""
+    """The shapes above run every kernel instance: 4 tile-block shapes x column shift (q & 1)
+    x staging width (odd W: 2-byte loads). Every patch needs 2 staging units per thread of the
+    512-thread workgroup (648-816 units of 8 channels x 2 pixels)."""
     seen = set()
     for (n, c, h, w, k, p, q) in SHAPES:
         x = torch.zeros((n, c, h, w), dtype=torch.float16, device=cuda)
         wt = torch.zeros((k, c, 3, 3), dtype=torch.float16, device=cuda)
         prm = runner.WinogradConv(x, wt, pad=(p, q)).param()
-        seen.add((prm.fbh, prm.fbw, prm.fnu))
-    assert {(s[0], s[1]) for s in seen} == {(8, 8), (4, 16), (16, 4), (2, 32)}
-    assert {s[2] for s in seen} == {2}
+        assert prm.fnu == 2
+        seen.add((prm.fbh, prm.fbw, prm.fsh, w % 2))
+    assert seen == {(bh, bw, sh, odd) for (bh, bw) in [(8, 8), (4, 16), (16, 4), (2, 32)]
+                    for sh in (0, 1) for odd in (0, 1)}
 
 
 def test_fused_full_size_against_fp64_gpu(runner, cuda):
