@@ -168,12 +168,8 @@ int select_config(const problem_t* pb, const Geometry& g) {
 }
 
 // Form choice on the Winograd path. The fused kernel recomputes V once per 64 output
-// channels instead of writing and reading it through HBM, so it wins when K is small (up to
-// 192: 1.2-2.4x on 112x112 and 56x56 layers with C = 256-512, ties elsewhere) and on layers too
-// small to fill the chip with staged GEMM tiles (N*T*K*C below 1.2e9: 1.2-2.4x); the staged
-// GEMM ties on the 56x56 C = K = 256 layer and wins on 28x28 C = K = 512 (1.35x). Odd widths
-// need 2-byte staging loads and are taken only when small. A tuned shape keeps its staged
-// configuration.
+// channels instead of writing and reading it through HBM; the staged GEMM's 128 x 128 tiles
+// reuse each loaded operand more. A tuned shape keeps its staged configuration.
 bool use_fused(const problem_t* pb, const Geometry& g) {
     if (g.direct || !g.fused_ok) return false;
     if (tuned_config_lookup(pb) >= 0) return false;
