@@ -294,6 +294,21 @@ def test_tile_configs_and_selection(wc):
         assert c["bk"] % 16 == 0 and c["bt"] % 16 == 0 and c["threads"] % 64 == 0 and c["threads"] <= 1024
 
 
+@pytest.mark.parametrize("shape,want", [((32, 512, 28, 28, 512), 9), ((8, 512, 56, 56, 512), 9),
+                                        ((32, 256, 56, 56, 256), 9), ((16, 512, 28, 28, 512), 11),
+                                        ((16, 128, 56, 56, 128), 11), ((4, 64, 28, 28, 128), 10),
+                                        ((1, 64, 56, 56, 64), 10)])
+def test_tile_heuristic(wc, shape, want):
+    """Untuned shapes: 32x64 (configuration 10) below one 64x128 workgroup per two compute
+    units; the 128x128 LDS-DMA tile (9) for C >= 256 with at least one of its workgroups per
+    two compute units (196 workgroups for 32x512x28x28x512); else 64x128 (11)."""
+    L = wc.load_library()
+    L.conv2d_tuned_clear()
+    pb = wc.make_problem(*shape, ptrs=FAKE)
+    c = ctypes.c_int()
+    assert L.conv2d_select_tile_config(ctypes.byref(pb), ctypes.byref(c)) == wc.OK and c.value == want
+
+
 def test_tuned_table_roundtrip(wc, tmp_path):
     L = wc.load_library()
     L.conv2d_tuned_clear()
