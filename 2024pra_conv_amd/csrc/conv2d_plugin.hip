@@ -49,8 +49,9 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
     g->KB32 = ((int)pb->k + 31) / 32;
     g->v_bytes = (size_t)16 * 2 * g->CB * (size_t)g->G32 * kFragHalves * 2;
     g->u_bytes = (size_t)16 * 2 * g->CB * (size_t)g->KB32 * kFragHalves * 2;
-    // input-transform band: largest of 8/4/2/1 tile rows whose LDS staging fits in 64 KiB,
-    // preferring 4 (keeps several workgroups per CU)
+    // input-transform band: the first of 4/2/1/8 tile rows whose LDS staging fits in 64 KiB
+    // (preferring 4 keeps several workgroups per CU); for wide images one tile row in up to
+    // 160 KiB (the whole LDS of a gfx950 compute unit, one workgroup per unit: w up to 630)
     g->band = 0;
     const int cands[4] = {4, 2, 1, 8};
     for (int i = 0; i < 4 && !g->band; ++i) {
@@ -58,6 +59,7 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         if (b > 1 && b > g->tiles_h) continue;
         if (input_lds_bytes(b, g->tiles_w, (int)pb->w, (int)pb->q) <= 65536) g->band = b;
     }
+    if (!g->band && input_lds_bytes(1, g->tiles_w, (int)pb->w, (int)pb->q) <= 163840) g->band = 1;
     g->nsteps = 16 * g->CB;
     g->direct = !(g->winograd_ok && g->band);
     if (g->direct) {

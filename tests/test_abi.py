@@ -259,6 +259,20 @@ def test_winograd_geometry_reports_winograd(wc):
         assert L.conv2d_method(ctypes.byref(pb), ctypes.byref(m)) == wc.OK and m.value == wc.METHOD_WINOGRAD
 
 
+@pytest.mark.parametrize("w,method,band", [(224, 0, 1), (512, 0, 1), (629, 0, 1), (631, 1, 0)])
+def test_wide_images_stage_one_tile_row(wc, w, method, band):
+    """The input transform stages whole input rows: one tile row (4 rows x 32 channels) of a
+    w-wide image needs 256*(w + 9) bytes of LDS, so images up to w = 629 stay on the Winograd
+    path (up to 160 KiB, one workgroup per compute unit beyond 64 KiB); wider ones take the
+    direct path."""
+    L = wc.load_library()
+    m = ctypes.c_int()
+    pb = wc.make_problem(1, 64, 6, w, 64, ptrs=FAKE)
+    assert L.conv2d_method(ctypes.byref(pb), ctypes.byref(m)) == wc.OK and m.value == method
+    rc, ki, prm = _ki(wc, pb)
+    assert rc == wc.OK and prm.band == band
+
+
 def test_getkernelinfo_null_pointers(wc):
     L = wc.load_library()
     pb = wc.make_problem(1, 16, 8, 8, 16)  # no device pointers

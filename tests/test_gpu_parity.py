@@ -32,6 +32,8 @@ SMALL = [
     (2, 96, 15, 15, 96, 1, 1),   # ragged 96-channel
     (1, 40, 9, 9, 20, 1, 1),
     (1, 130, 12, 12, 140, 1, 1), # K > 128 (two k-blocks), C not multiple of 32
+    (1, 16, 6, 300, 16, 1, 1),   # wide: one tile row of input staging needs 79 KiB of LDS
+    (1, 40, 5, 627, 24, 2, 2),   # near the widest Winograd staging (158 KiB of LDS), pad 2
 ]
 
 
