@@ -281,6 +281,12 @@ int conv2d_batch_autotune(conv2d_batch* b, int reps, void* stream) {
     std::vector<float> times(kNumGemmConfigs);
     for (size_t i = 0; i < b->layers.size(); ++i) {
         const problem_t* pb = &b->layers[i];
+        wino::Geometry g0;
+        if (int rc0 = wino::require_winograd(pb, &g0)) return rc0;
+        if (!g0.staged_ok) {  // too wide for the input transform: the fused form only
+            b->form[i] = CONV2D_FORM_FUSED;
+            continue;
+        }
         int best = -1;
         int rc = wino::autotune_subset(pb, reps, stream, all.data(), kNumGemmConfigs, &best, times.data());
         if (rc) return rc;

@@ -61,28 +61,8 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
     }
     if (!g->band && input_lds_bytes(1, g->tiles_w, (int)pb->w, (int)pb->q) <= 163840) g->band = 1;
     g->nsteps = 16 * g->CB;
-    g->direct = !(g->winograd_ok && g->band);
-    if (g->direct) {
-        // general-geometry path: GEMM columns are output pixels, steps run over 32 values
-        // of the flattened reduction index j = tap * C + c, padded with zero steps to a
-        // multiple of 8 (the deepest ring of the direct configurations)
-        g->tiles_h = g->Oh;
-        g->tiles_w = g->Ow;
-        g->T = (long long)g->Oh * g->Ow;
-        g->NT = g->T * pb->n;
-        if (g->NT > (1LL << 30)) return CONV2D_ERR_OVERFLOW;
-        g->G32 = (g->NT + 31) / 32;
-        const long long jn = (long long)pb->r * pb->s * pb->c;
-        const long long steps = ((jn + 31) / 32 + 7) / 8 * 8;
-        // the packing kernels index j in 32 bits and launch one thread per 16 bytes of an
-        // operand (a 1-D grid of fewer than 2^32 threads): at most 64 GiB per operand
-        const long long rows = g->G32 > g->KB32 ? g->G32 : g->KB32;
-        if (jn > (1LL << 30) || steps * 2 * rows * 64 > (1LL << 32) - 256) return CONV2D_ERR_OVERFLOW;
-        g->nsteps = (int)steps;
-        g->band = 0;
-        g->v_bytes = (size_t)steps * 2 * (size_t)g->G32 * kFragHalves * 2;
-        g->u_bytes = (size_t)steps * 2 * (size_t)g->KB32 * kFragHalves * 2;
-    } else {
+    g->fused_ok = false;
+    if (g->winograd_ok) {
         // fused form: the 64-tile block shape that pads the tile grid least (ties: the
         // squarer block, whose input patch is smaller)
         long long best = -1;
@@ -112,6 +92,31 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         g->fused_ok = g->fnu <= 2 && g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
                       out_img_bytes < (1LL << 31) && (long long)g->u_bytes < (1LL << 31);
     }
+    // the Winograd path needs the staged form's input-transform band or the fused form
+    g->direct = !(g->winograd_ok && (g->band || g->fused_ok));
+    g->staged_ok = !g->direct && g->band > 0;
+    if (g->direct) {
+        // general-geometry path: GEMM columns are output pixels, steps run over 32 values
+        // of the flattened reduction index j = tap * C + c, padded with zero steps to a
+        // multiple of 8 (the deepest ring of the direct configurations)
+        g->tiles_h = g->Oh;
+        g->tiles_w = g->Ow;
+        g->T = (long long)g->Oh * g->Ow;
+        g->NT = g->T * pb->n;
+        if (g->NT > (1LL << 30)) return CONV2D_ERR_OVERFLOW;
+        g->G32 = (g->NT + 31) / 32;
+        const long long jn = (long long)pb->r * pb->s * pb->c;
+        const long long steps = ((jn + 31) / 32 + 7) / 8 * 8;
+        // the packing kernels index j in 32 bits and launch one thread per 16 bytes of an
+        // operand (a 1-D grid of fewer than 2^32 threads): at most 64 GiB per operand
+        const long long rows = g->G32 > g->KB32 ? g->G32 : g->KB32;
+        if (jn > (1LL << 30) || steps * 2 * rows * 64 > (1LL << 32) - 256) return CONV2D_ERR_OVERFLOW;
+        g->nsteps = (int)steps;
+        g->band = 0;
+        g->fused_ok = false;
+        g->v_bytes = (size_t)steps * 2 * (size_t)g->G32 * kFragHalves * 2;
+        g->u_bytes = (size_t)steps * 2 * (size_t)g->KB32 * kFragHalves * 2;
+    }
     return CONV2D_OK;
 }
 
@@ -120,6 +125,12 @@ int require_winograd(const problem_t* pb, Geometry* g) {
     if (rc) return rc;
     if (g->direct) return CONV2D_ERR_UNSUPPORTED;
     return CONV2D_OK;
+}
+
+int require_staged(const problem_t* pb, Geometry* g) {
+    int rc = require_winograd(pb, g);
+    if (rc) return rc;
+    return g->staged_ok ? CONV2D_OK : CONV2D_ERR_UNSUPPORTED;
 }
 
 static int compute_units() {
@@ -177,6 +188,7 @@ int select_config(const problem_t* pb, const Geometry& g) {
 // reuse each loaded operand more. A tuned shape keeps its staged configuration.
 bool use_fused(const problem_t* pb, const Geometry& g) {
     if (g.direct || !g.fused_ok) return false;
+    if (!g.staged_ok) return true;  // too wide for the input transform's LDS staging
     if (tuned_config_lookup(pb) >= 0) return false;
     // Measured on 45 shapes (both forms timed after each had run 50 ms, so at sustained
     // clocks): with K <= 320 the fused form was faster on all but three (at most 9% slower,
@@ -420,6 +432,7 @@ int conv2d_input_transform(const problem_t* pb, void* stream) {
     if (rc) return rc;
     Geometry g;
     if ((rc = derive_geometry(pb, &g))) return rc;
+    if (!g.direct && !g.staged_ok) return CONV2D_ERR_UNSUPPORTED;
     mykernelParamType a;
     fill_param(pb, g, 0, &a);
     return launch(slot_input(pb, g), &a, (hipStream_t)stream);
@@ -430,6 +443,7 @@ int conv2d_gemm(const problem_t* pb, int cfg, void* stream) {
     if (rc) return rc;
     Geometry g;
     if ((rc = derive_geometry(pb, &g))) return rc;
+    if (!g.direct && !g.staged_ok) return CONV2D_ERR_UNSUPPORTED;
     if (cfg < 0) cfg = select_config(pb, g);
     if (cfg >= (g.direct ? kNumDirectConfigs : kNumGemmConfigs)) return CONV2D_ERR_INVALID;
     mykernelParamType a;
@@ -448,6 +462,7 @@ int conv2d_forward(const problem_t* pb, int cfg, void* stream) {
         a.form = CONV2D_FORM_FUSED;
         return launch(slot_fused(g), &a, (hipStream_t)stream);
     }
+    if (!g.direct && !g.staged_ok) return CONV2D_ERR_UNSUPPORTED;
     if (cfg < 0) cfg = select_config(pb, g);
     if (cfg >= (g.direct ? kNumDirectConfigs : kNumGemmConfigs)) return CONV2D_ERR_INVALID;
     mykernelParamType a;
@@ -479,6 +494,7 @@ int conv2d_forward_epilogue(const problem_t* pb, int cfg, const void* bias, int 
         a.act = act;
         return launch(slot_fused(g), &a, (hipStream_t)stream);
     }
+    if (!g.direct && !g.staged_ok) return CONV2D_ERR_UNSUPPORTED;
     if (cfg < 0) cfg = select_config(pb, g);
     if (cfg >= (g.direct ? kNumDirectConfigs : kNumGemmConfigs)) return CONV2D_ERR_INVALID;
     mykernelParamType a;
@@ -574,7 +590,7 @@ static int fp8_heuristic_config(const problem_t* pb) {
 }
 
 static int fp8_prepare(const problem_t* pb, int cfg, Geometry* g, mykernelParamType* a, int* out_cfg) {
-    int rc = require_winograd(pb, g);
+    int rc = require_staged(pb, g);
     if (rc) return rc;
     if (cfg < 0) cfg = fp8_heuristic_config(pb);
     if (cfg >= kNumFp8Configs) return CONV2D_ERR_INVALID;
@@ -614,7 +630,7 @@ static Launch fp8_slot_gemm(const Geometry& g, int cfg) {
 int conv2d_fp8_workspace_size(const problem_t* pb, size_t* pin_bytes, size_t* filter_bytes) {
     if (!pb) return CONV2D_ERR_NULL;
     Geometry g;
-    int rc = require_winograd(pb, &g);
+    int rc = require_staged(pb, &g);
     if (rc) return rc;
     if (pin_bytes) *pin_bytes = fp8_operand_bytes((int)pb->c, g.G32);
     if (filter_bytes) *filter_bytes = fp8_u8_offset(g.CB, g.KB32) + fp8_operand_bytes((int)pb->c, g.KB32);
@@ -672,7 +688,7 @@ int conv2d_fp8_tile_config(int cfg, int* bk, int* bt, int* threads) {
 int conv2d_fp8_select_tile_config(const problem_t* pb, int* cfg) {
     if (!pb || !cfg) return CONV2D_ERR_NULL;
     Geometry g;
-    int rc = require_winograd(pb, &g);
+    int rc = require_staged(pb, &g);
     if (rc) return rc;
     *cfg = fp8_heuristic_config(pb);
     return CONV2D_OK;

@@ -21,6 +21,7 @@ struct Geometry {
     size_t v_bytes, u_bytes;
     // fused form: block of fbh x fbw tiles, fnbh x fnbw blocks per image (fused_ok: even w)
     bool fused_ok;
+    bool staged_ok;  // Winograd path with an input-transform band (the staged form applies)
     bool fd16;  // 2-byte staging loads (odd width, or an input base not 4-byte aligned)
     int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
     long long fitems;
@@ -37,6 +38,8 @@ struct Launch {
 int derive_geometry(const problem_t* pb, Geometry* g);
 // As derive_geometry, but only for problems on the Winograd path (else CONV2D_ERR_UNSUPPORTED).
 int require_winograd(const problem_t* pb, Geometry* g);
+// As require_winograd, and the staged form applies (an input-transform band fits in LDS).
+int require_staged(const problem_t* pb, Geometry* g);
 int heuristic_config(const problem_t* pb, const Geometry& g);
 int select_config(const problem_t* pb, const Geometry& g);
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a);

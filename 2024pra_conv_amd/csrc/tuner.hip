@@ -178,7 +178,7 @@ int autotune_subset(const problem_t* pb, int reps, void* stream, const int* cfgs
     for (int i = 0; i < ncfg; ++i)
         if (cfgs[i] < 0 || cfgs[i] >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
     Geometry g;
-    int rc = require_winograd(pb, &g);
+    int rc = require_staged(pb, &g);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const size_t nout = (size_t)pb->n * pb->k * g.Oh * g.Ow;

@@ -259,18 +259,27 @@ def test_winograd_geometry_reports_winograd(wc):
         assert L.conv2d_method(ctypes.byref(pb), ctypes.byref(m)) == wc.OK and m.value == wc.METHOD_WINOGRAD
 
 
-@pytest.mark.parametrize("w,method,band", [(224, 0, 1), (512, 0, 1), (629, 0, 1), (631, 1, 0)])
-def test_wide_images_stage_one_tile_row(wc, w, method, band):
+@pytest.mark.parametrize("w,band", [(224, 1), (512, 1), (629, 1), (631, 0), (1500, 0)])
+def test_wide_images_stage_one_tile_row(wc, w, band):
     """The input transform stages whole input rows: one tile row (4 rows x 32 channels) of a
-    w-wide image needs 256*(w + 9) bytes of LDS, so images up to w = 629 stay on the Winograd
-    path (up to 160 KiB, one workgroup per compute unit beyond 64 KiB); wider ones take the
-    direct path."""
+    w-wide image needs 256*(w + 9) bytes of LDS, so up to w = 629 (160 KiB, one workgroup per
+    compute unit beyond 64 KiB) both forms apply; wider images stay on the Winograd path in the
+    fused form only (its patches are per tile block), and the staged-only entry points refuse
+    them before touching the device."""
     L = wc.load_library()
-    m = ctypes.c_int()
+    m, f = ctypes.c_int(), ctypes.c_int()
     pb = wc.make_problem(1, 64, 6, w, 64, ptrs=FAKE)
-    assert L.conv2d_method(ctypes.byref(pb), ctypes.byref(m)) == wc.OK and m.value == method
+    assert L.conv2d_method(ctypes.byref(pb), ctypes.byref(m)) == wc.OK and m.value == wc.METHOD_WINOGRAD
     rc, ki, prm = _ki(wc, pb)
     assert rc == wc.OK and prm.band == band
+    if band == 0:
+        assert prm.form == wc.FORM_FUSED and L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK
+        assert f.value == wc.FORM_FUSED
+        assert L.conv2d_input_transform(ctypes.byref(pb), None) == wc.ERR_UNSUPPORTED
+        assert L.conv2d_gemm(ctypes.byref(pb), 0, None) == wc.ERR_UNSUPPORTED
+        assert L.conv2d_forward(ctypes.byref(pb), 0, None) == wc.ERR_UNSUPPORTED
+        vb, ub = ctypes.c_size_t(), ctypes.c_size_t()
+        assert L.conv2d_fp8_workspace_size(ctypes.byref(pb), ctypes.byref(vb), ctypes.byref(ub)) == wc.ERR_UNSUPPORTED
 
 
 def test_getkernelinfo_null_pointers(wc):

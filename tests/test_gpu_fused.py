@@ -103,6 +103,32 @@ def test_fused_unaligned_input_base_takes_2byte_loads(runner, wc, cuda):
     assert np.all(np.abs(y - y_model) <= tol), f"max excess {np.max(np.abs(y - y_model) - tol)}"
 
 
+def test_wide_image_runs_fused_only(runner, wc, cuda):
+    """An image too wide for the input transform's LDS staging (w = 700: one tile row needs
+    181 KiB) stays on the Winograd path in the fused form: the four-slot protocol and
+    forward(-1) run it and meet the oracle bound."""
+    n, c, h, w, k, p, q = 1, 24, 6, 700, 40, 1, 1
+    print("case wide 700", flush=True)
+    geo = wg.Geometry(n, c, h, w, k, p, q)
+    x = wg.uniform_pm1(960, (n, c, h, w))
+    wt = wg.uniform_pm1(961, (k, c, 3, 3))
+    conv = runner.WinogradConv(_to_dev(x, cuda), _to_dev(wt, cuda), pad=(p, q))
+    assert conv.method() == wc.METHOD_WINOGRAD and conv.form() == wc.FORM_FUSED and conv.param().band == 0
+    y_model, scale = wg.fused_gpu_model(x, wt, geo)
+    e = (c + 16) * 2.0 ** -24 * scale
+    tol = 2.0 ** -11 * (np.abs(y_model) + e) + e + 2.0 ** -25
+    for run in ("slots", "forward"):
+        conv.out.fill_(float("nan"))
+        if run == "slots":
+            conv.launch_slots()
+        else:
+            conv.prepare_weights()
+            conv.forward()
+        torch.cuda.synchronize()
+        y = conv.out.cpu().numpy().astype(np.float64)
+        assert np.all(np.abs(y - y_model) <= tol), f"{run}: max excess {np.max(np.abs(y - y_model) - tol)}"
+
+
 def test_fused_block_shapes_cover_every_variant(runner, cuda):
     """The shapes above run every kernel instance: 4 tile-block shapes x column shift (q & 1)
     x staging width (odd W: 2-byte loads). Every patch needs 2 staging units per thread of the
