@@ -259,12 +259,16 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
                     v[0] = sub8(s3[1], s3[0], m1);  // s2 - s1
                     v[1] = sub8(s3[0], s3[2], m1);  // s1 - s3
                 }
+                // issue priority while this wave queues its MFMAs, so the other wave of the SIMD
+                // does not interleave its loads and transform between them (about 2% faster)
+                __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                 for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
                     for (int m = 0; m < 2; ++m)
                         acc[bb][m][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ucur[(ks * 2 + bb) * 2 + m], v[bb],
                                                                                acc[bb][m][tb], 0, 0, 0);
+                __builtin_amdgcn_s_setprio(0);
             }
         }
         if (more) {
