@@ -22,8 +22,8 @@ struct conv2d_batch {
     struct Segment {
         int cfg;      // staged: GEMM tile configuration of every layer in the segment
         int fused;    // 1 for a fused segment, 0 for a staged one
-        bool d16;     // fused: 2-byte staging loads
-        int shape;    // fused: 2 * tile-block shape index (fused_shape_index) + column shift
+        int lw;       // fused: staging load width (FusedLoad)
+        int shape;    // fused: 4 * tile-block shape index (fused_shape_index) + column shift
         int first;    // first descriptor of the segment
         int count;    // layers in the segment
         int items;    // work items of the segment's queue
@@ -58,7 +58,7 @@ int upload(conv2d_batch* b) {
         if (rc) return rc;
     }
     auto key = [&](int i) {
-        return b->form[i] == CONV2D_FORM_FUSED ? 1000 + 4 * fused_shape_index(geo[i].fbh, geo[i].fbw) + 2 * geo[i].fsh + (geo[i].fd16 ? 1 : 0) : b->cfg[i];
+        return b->form[i] == CONV2D_FORM_FUSED ? 1000 + 16 * geo[i].flw + 4 * fused_shape_index(geo[i].fbh, geo[i].fbw) + geo[i].fsh : b->cfg[i];
     };
     auto cost = [&](int i) {
         if (b->form[i] == CONV2D_FORM_FUSED) return (double)b->layers[i].c;
@@ -83,11 +83,11 @@ int upload(conv2d_batch* b) {
         const int cfg = b->cfg[i];
         const bool fused = b->form[i] == CONV2D_FORM_FUSED;
         const int fz = fused ? 1 : 0;
-        const bool d16 = fused && g.fd16;
-        const int shape = fused ? 2 * fused_shape_index(g.fbh, g.fbw) + g.fsh : 0;
-        if (b->segs.empty() || b->segs.back().fused != fz || b->segs.back().d16 != d16 ||
+        const int lw = fused ? g.flw : 0;
+        const int shape = fused ? 4 * fused_shape_index(g.fbh, g.fbw) + g.fsh : 0;
+        if (b->segs.empty() || b->segs.back().fused != fz || b->segs.back().lw != lw ||
             b->segs.back().shape != shape || (!fused && b->segs.back().cfg != cfg))
-            b->segs.push_back(conv2d_batch::Segment{cfg, fz, d16, shape, qi, 0, 0, 0u});
+            b->segs.push_back(conv2d_batch::Segment{cfg, fz, lw, shape, qi, 0, 0, 0u});
         conv2d_batch::Segment& seg = b->segs.back();
         BatchLayer& bl = b->host[qi];
         wino::fill_param(&pb, g, cfg, &bl.prm);
@@ -152,7 +152,7 @@ int launch_forward(conv2d_batch* b, hipStream_t st) {
         int count = seg.count, items = seg.items;
         void* args[3] = {(void*)&layers, (void*)&count, (void*)&items};
         const hipError_t e =
-            seg.fused ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.d16, seg.shape >> 1, seg.shape & 1), dim3(seg.items), dim3(fused_threads()), args, seg.lds, st)
+            seg.fused ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.lw, seg.shape >> 2, seg.shape & 3), dim3(seg.items), dim3(fused_threads()), args, seg.lds, st)
                    : hipLaunchKernel(kGemmConfigs[seg.cfg].grouped_kernel, dim3(seg.items),
                                      dim3(kGemmConfigs[seg.cfg].threads), args, 0, st);
         if (e != hipSuccess) return CONV2D_ERR_HIP;

@@ -77,11 +77,18 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         }
         g->fnbh = (g->tiles_h + g->fbh - 1) / g->fbh;
         g->fnbw = (g->tiles_w + g->fbw - 1) / g->fbw;
-        g->fsh = (int)(pb->q & 1);
-        g->fd16 = (pb->w % 2) != 0 || ((uintptr_t)pb->in & 3) != 0;
-        g->fpr = 2 * g->fbh + 2;
-        g->fpc = fused_patch_cols(g->fbw);
-        const int units = 4 * g->fpr * (g->fpc / 2);
+        // staging load width: 8-byte pixel quads when the width and base allow them and the
+        // quad patch fits one unit per thread, else 4-byte pairs, else 2-byte pixels
+        const uintptr_t in_addr = (uintptr_t)pb->in;
+        g->fpr = fused_patch_rows(g->fbh);
+        g->flw = (pb->w % 2) != 0 || (in_addr & 3) != 0 ? kFusedLoad2 : kFusedLoad4;
+        if (g->flw == kFusedLoad4 && pb->w % 4 == 0 && (in_addr & 7) == 0) {
+            const int pc8 = fused_patch_cols(kFusedLoad8, g->fbw, fused_col_shift(kFusedLoad8, pb->q));
+            if (4 * g->fpr * (pc8 / 4) <= kFusedThreads) g->flw = kFusedLoad8;
+        }
+        g->fsh = fused_col_shift(g->flw, pb->q);
+        g->fpc = fused_patch_cols(g->flw, g->fbw, g->fsh);
+        const int units = 4 * g->fpr * (g->fpc / fused_unit_pixels(g->flw));
         g->fnu = (units + kFusedThreads - 1) / kFusedThreads;
         g->fitems = (long long)pb->n * g->fnbh * g->fnbw * ((pb->k + 63) / 64);
         // buffer offsets are 32-bit: one input image with 32 channels of slack under 2^31
@@ -89,7 +96,8 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         // under 2^31 bytes
         const long long img_bytes = ((long long)pb->c + 32) * pb->h * pb->w * 2;
         const long long out_img_bytes = (long long)pb->k * g->Oh * g->Ow * 2;
-        g->fused_ok = g->fnu <= 2 && g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
+        g->fused_ok = g->fnu <= 2 && fused_kernel_ptr(g->flw, fused_shape_index(g->fbh, g->fbw), g->fsh) &&
+                      g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
                       out_img_bytes < (1LL << 31) && (long long)g->u_bytes < (1LL << 31);
     }
     // the Winograd path needs the staged form's input-transform band or the fused form
@@ -270,7 +278,7 @@ Launch slot_filter(const Geometry& g) {
 // input patches during the channel loop and the 128 KiB Z exchange after it.
 Launch slot_fused(const Geometry& g) {
     Launch L;
-    L.kernel = fused_kernel_ptr(g.fd16, fused_shape_index(g.fbh, g.fbw), g.fsh);
+    L.kernel = fused_kernel_ptr(g.flw, fused_shape_index(g.fbh, g.fbw), g.fsh);
     L.grid = dim3((unsigned)g.fitems, 1, 1);
     L.block = dim3((unsigned)fused_threads(), 1, 1);
     const unsigned patches = 2u * 4u * (unsigned)g.fpr * (unsigned)g.fpc * 16u;

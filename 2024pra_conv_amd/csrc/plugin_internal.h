@@ -22,7 +22,7 @@ struct Geometry {
     // fused form: block of fbh x fbw tiles, fnbh x fnbw blocks per image (fused_ok: even w)
     bool fused_ok;
     bool staged_ok;  // Winograd path with an input-transform band (the staged form applies)
-    bool fd16;  // 2-byte staging loads (odd width, or an input base not 4-byte aligned)
+    int flw;    // staging load width (FusedLoad: 2-, 4- or 8-byte loads per channel)
     int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
     long long fitems;
 };

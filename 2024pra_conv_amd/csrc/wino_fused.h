@@ -12,11 +12,28 @@ inline int fused_shape_index(int fbh, int fbw) {
         if (kFusedShapes[i][0] == fbh && kFusedShapes[i][1] == fbw) return i;
     return -1;
 }
-// Patch columns of a block fbw tiles wide (room for the one-column shift fsh).
-inline int fused_patch_cols(int fbw) { return 2 * fbw + 4; }
+// Staging load width of the fused kernels, per channel of a staging unit: one 2-byte pixel
+// (odd width, or an input base that is not 4-byte aligned), one 4-byte pixel pair, or one
+// 8-byte pixel quad (width a multiple of 4, base 8-byte aligned, patch within one unit per
+// thread).
+enum FusedLoad { kFusedLoad2 = 0, kFusedLoad4 = 1, kFusedLoad8 = 2 };
+// Pixels per staging unit (8 channels each).
+constexpr int fused_unit_pixels(int lw) { return lw == kFusedLoad8 ? 4 : 2; }
+// Column shift: the patch starts fsh columns left of 2*tw0 - q (tw0 the block's first tile
+// column, a multiple of 4), so that its first column is even (q & 1) or, with 8-byte loads,
+// a multiple of 4 ((4 - q % 4) % 4).
+constexpr int fused_col_shift(int lw, int q) { return lw == kFusedLoad8 ? (4 - q % 4) % 4 : (q & 1); }
+// Patch columns of a block fbw tiles wide: the windows reach column 2*fbw + fsh + 1; a whole
+// number of staging units per row.
+constexpr int fused_patch_cols(int lw, int fbw, int fsh) {
+    return lw == kFusedLoad8 ? (2 * fbw + fsh + 2 + 3) & ~3 : 2 * fbw + 4;
+}
+// Patch rows of a block fbh tiles high.
+constexpr int fused_patch_rows(int fbh) { return 2 * fbh + 2; }
 
-// Kernel for 2-byte (d16: odd width or a base that is not 4-byte aligned) or 4-byte staging
-// loads, block shape index `shape` and column shift fsh (q & 1), and its multi-layer
-// (grouped) form with arguments (const BatchLayer* layers, int nlayers, int total items).
-const void* fused_kernel_ptr(bool d16, int shape, int fsh);
-const void* fused_grouped_kernel_ptr(bool d16, int shape, int fsh);
+// Kernel for staging load width lw, block shape index `shape` and column shift fsh, and its
+// multi-layer (grouped) form with arguments (const BatchLayer* layers, int nlayers, int total
+// items); null for a combination without an instance (8-byte loads with a patch that needs
+// more than one unit per thread).
+const void* fused_kernel_ptr(int lw, int shape, int fsh);
+const void* fused_grouped_kernel_ptr(int lw, int shape, int fsh);

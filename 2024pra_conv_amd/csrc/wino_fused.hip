@@ -37,6 +37,8 @@
 #include "wino_tile_ops.h"
 #include "wino_fused.h"
 
+#include <utility>
+
 namespace {
 
 // Raw buffer resource (stride 0, byte offsets); dword 3 = the gfx9 default data format.
@@ -135,21 +137,101 @@ __device__ inline void stage_store(const Staging<NU>& st, half_t* buf, int half_
     }
 }
 
+// 8-byte staging (W a multiple of 4, 8-byte aligned base, patch starting on a column that is
+// a multiple of 4): one unit = (channel group g, patch row, pixel quad) = 8 channels x 4
+// adjacent pixels, which lie entirely inside or outside the image: one 8-byte buffer load per
+// channel, half the load instructions of 4-byte staging. Out-of-image quads and channels >= C
+// read zero past the resource's range, as above.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+struct QuadStaging {
+    uint32_t off;  // byte offset of the unit's first pixel in channel g*8 of chunk 0
+    int lds;       // LDS offset (halves) of the unit's first pixel
+    u32x2 w[8];    // loaded quads: pixel pairs 0 and 1 (first pixel of each in the low half)
+};
+
+template <int PR, int PC>
+__device__ inline void quad_setup(const mykernelParamType& prm, int ih0, int iw0, QuadStaging& st) {
+    constexpr int PQ = PC >> 2;
+    constexpr int nunits = 4 * PR * PQ;
+    static_assert(PC % 4 == 0 && nunits <= 512, "one quad unit per thread");
+    const int H = prm.h, W = prm.w;
+    const int u = threadIdx.x;
+    const int g = u / (PR * PQ);
+    const int rem = u - g * (PR * PQ);
+    const int pr = rem / PQ;
+    const int m = rem - pr * PQ;
+    const int ih = ih0 + pr, iw = iw0 + 4 * m;
+    const bool pix = u < nunits && ih >= 0 && ih < H && iw >= 0 && iw < W;
+    st.off = pix ? (uint32_t)(((g * 8 * H + ih) * W + iw) * 2) : 0x80000000u;
+    // quad m: pixel pairs 2m and 2m+1; a thread slot without a unit stores to a dummy area past
+    // both patch buffers
+    st.lds = u < nunits ? ((g * PR + pr) * PC + 2 * m) * 8 : 2 * 4 * PR * PC * 8;
+}
+
+__device__ inline void quad_load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes,
+                                 QuadStaging& st) {
+    const uint32_t base = st.off + chunk_bytes;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+        st.w[e] = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(in, (int)(base + e * chan_bytes), 0, 0));
+}
+
+template <int PC>
+__device__ inline void quad_store(const QuadStaging& st, half_t* buf) {
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        u32x4 p0, p1;  // pixels 2(2m+d) and 2(2m+d)+1, 8 channels each
+        p0[0] = pack_lo(st.w[0][d], st.w[1][d]);
+        p0[1] = pack_lo(st.w[2][d], st.w[3][d]);
+        p0[2] = pack_lo(st.w[4][d], st.w[5][d]);
+        p0[3] = pack_lo(st.w[6][d], st.w[7][d]);
+        p1[0] = pack_hi(st.w[0][d], st.w[1][d]);
+        p1[1] = pack_hi(st.w[2][d], st.w[3][d]);
+        p1[2] = pack_hi(st.w[4][d], st.w[5][d]);
+        p1[3] = pack_hi(st.w[6][d], st.w[7][d]);
+        *(u32x4*)(buf + st.lds + d * 8) = p0;
+        *(u32x4*)(buf + st.lds + d * 8 + (PC / 2) * 8) = p1;
+    }
+}
+
+// The staging of one thread for load width LW: 8-byte quads, or up to two 2- or 4-byte pair
+// units (512 threads x 2 units cover the largest pair patch, 816 units).
+template <int LW, int PR, int PC>
+struct PatchStager {
+    static constexpr bool quad = LW == kFusedLoad8;
+    static constexpr bool d16 = LW == kFusedLoad2;
+    static constexpr int NU = 2;
+    QuadStaging q;
+    Staging<NU> p;
+    __device__ void setup(const mykernelParamType& prm, int ih0, int iw0) {
+        if constexpr (quad) quad_setup<PR, PC>(prm, ih0, iw0, q);
+        else stage_setup<NU, d16, 512, PR, PC>(prm, ih0, iw0, p);
+    }
+    __device__ void load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes) {
+        if constexpr (quad) quad_load(in, chunk_bytes, chan_bytes, q);
+        else stage_load<NU, d16>(in, chunk_bytes, chan_bytes, p);
+    }
+    __device__ void store(half_t* buf) const {
+        if constexpr (quad) quad_store<PC>(q, buf);
+        else stage_store<NU>(p, buf, PC / 2);
+    }
+};
+
 }  // namespace
 
 // One work item: 64 output channels (block kblk) x one FBH x FBW = 64-tile block (gblk,
 // image-major) of the problem in prm, by the waves of one half (BH) of a 512-thread
-// workgroup, in the launch's dynamic LDS. The patch is PR = 2*FBH + 2 rows by
-// PC = 2*FBW + 4 columns (room for the column shift FSH = q & 1), so every LDS offset
-// except the lane's two bases (window rows rp, rm) and the chunk's buffer is a
-// compile-time constant.
-template <bool D16, int BH, int FBH, int FBW, int FSH>
+// workgroup, in the launch's dynamic LDS, with staging load width LW. The patch is
+// PR = 2*FBH + 2 rows by PC columns starting FSH columns left of 2*tw0 - q
+// (fused_patch_cols, fused_col_shift), so every LDS offset except the lane's two bases
+// (window rows rp, rm) and the chunk's buffer is a compile-time constant.
+template <int LW, int BH, int FBH, int FBW, int FSH>
 __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gblk) {
-    constexpr int PR = 2 * FBH + 2, PC = 2 * FBW + 4;
+    constexpr int PR = fused_patch_rows(FBH), PC = fused_patch_cols(LW, FBW, FSH);
     constexpr int PB = 4 * PR * PC * 8;  // halves per patch buffer
     static_assert(FBH * FBW == 64, "64 tiles per block");
+    static_assert(2 * PB * 2 + PC * 16 <= 131072, "two patch buffers and the dummy area within the Z area");
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
-    constexpr int NU = 2;  // 512 threads x 2 units cover the largest patch (1024 units)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -219,12 +301,12 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
 
     // the next chunk's patch loads are in flight during this chunk's MFMAs; U has one
     // register set (two waves per SIMD hide its L2 latency), reloaded after its last use
-    Staging<NU> st;
-    stage_setup<NU, D16, 512, PR, PC>(prm, ih0, iw0, st);
-    stage_load<NU, D16>(in_rsrc, 0u, chan_bytes, st);
+    PatchStager<LW, PR, PC> st;
+    st.setup(prm, ih0, iw0);
+    st.load(in_rsrc, 0u, chan_bytes);
     half8 ucur[8];
     load_u(ucur, 0);  // in flight while the first patch is stored
-    stage_store<NU>(st, lds, PC / 2);
+    st.store(lds);
     __syncthreads();
 
     auto chunk = [&](int ch) {
@@ -235,7 +317,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         // first MFMA) would also wait for them and expose their whole latency every chunk.
         asm volatile("" ::"v"(ucur[0]), "v"(ucur[1]), "v"(ucur[2]), "v"(ucur[3]), "v"(ucur[4]), "v"(ucur[5]),
                      "v"(ucur[6]), "v"(ucur[7]));
-        if (more) stage_load<NU, D16>(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes, st);
+        if (more) st.load(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes);
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
 #pragma unroll
@@ -273,7 +355,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         }
         if (more) {
             load_u(ucur, ch + 1);
-            stage_store<NU>(st, lds + (PB - buf_ofs), PC / 2);
+            st.store(lds + (PB - buf_ofs));
         }
         __syncthreads();
     };
@@ -397,28 +479,28 @@ template <> struct BlockShape<1> { static constexpr int h = 4, w = 16; };
 template <> struct BlockShape<2> { static constexpr int h = 16, w = 4; };
 template <> struct BlockShape<3> { static constexpr int h = 2, w = 32; };
 
-// S = 2 * (block shape index) + column shift FSH
-template <bool D16, int S>
+// S = 4 * (block shape index) + column shift FSH
+template <int LW, int S>
 __device__ inline void fused_item_dispatch(const mykernelParamType& prm, int kblk, int gblk) {
-    constexpr int FBH = BlockShape<S / 2>::h, FBW = BlockShape<S / 2>::w, FSH = S & 1;
+    constexpr int FBH = BlockShape<S / 4>::h, FBW = BlockShape<S / 4>::w, FSH = S & 3;
     // waves 0-3 take BH = 0, waves 4-7 BH = 1 (wave-uniform; both paths pass the same barriers)
     if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 8) == 0)
-        fused_item<D16, 0, FBH, FBW, FSH>(prm, kblk, gblk);
+        fused_item<LW, 0, FBH, FBW, FSH>(prm, kblk, gblk);
     else
-        fused_item<D16, 1, FBH, FBW, FSH>(prm, kblk, gblk);
+        fused_item<LW, 1, FBH, FBW, FSH>(prm, kblk, gblk);
 }
 
-template <bool D16, int S>
+template <int LW, int S>
 __global__ void __launch_bounds__(512, 1) wino_fused_kernel(mykernelParamType prm) {
     int kblk, gblk;
     xcd_remap(blockIdx.x, gridDim.x, (prm.k + 63) / 64, &kblk, &gblk);
-    fused_item_dispatch<D16, S>(prm, kblk, gblk);
+    fused_item_dispatch<LW, S>(prm, kblk, gblk);
 }
 
 // Multi-layer form: one launch over the fused items of several layers (descriptor i owns items
 // [s3_first, s3_first + s3_items), nk = 64-channel blocks), every layer with the same staging
 // load width, block shape and column shift.
-template <bool D16, int S>
+template <int LW, int S>
 __global__ void __launch_bounds__(512, 1)
 wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
     int item, unused;
@@ -430,42 +512,40 @@ wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, in
     }
     const BatchLayer& bl = layers[lo];
     const int local = item - bl.s3_first;
-    fused_item_dispatch<D16, S>(bl.prm, local % bl.nk, local / bl.nk);
+    fused_item_dispatch<LW, S>(bl.prm, local % bl.nk, local / bl.nk);
 }
-
-#define FUSED_INSTANCES(D16, S)                                                      \
-    template __global__ void wino_fused_kernel<D16, S>(mykernelParamType);           \
-    template __global__ void wino_fused_grouped_kernel<D16, S>(const BatchLayer*, int, int);
-FUSED_INSTANCES(false, 0)
-FUSED_INSTANCES(false, 1)
-FUSED_INSTANCES(false, 2)
-FUSED_INSTANCES(false, 3)
-FUSED_INSTANCES(false, 4)
-FUSED_INSTANCES(false, 5)
-FUSED_INSTANCES(false, 6)
-FUSED_INSTANCES(false, 7)
-FUSED_INSTANCES(true, 0)
-FUSED_INSTANCES(true, 1)
-FUSED_INSTANCES(true, 2)
-FUSED_INSTANCES(true, 3)
-FUSED_INSTANCES(true, 4)
-FUSED_INSTANCES(true, 5)
-FUSED_INSTANCES(true, 6)
-FUSED_INSTANCES(true, 7)
 
 int fused_threads() { return kFusedThreads; }
 
-#define FUSED_ROW(K, D16)                                                                              \
-    {(const void*)&K<D16, 0>, (const void*)&K<D16, 1>, (const void*)&K<D16, 2>, (const void*)&K<D16, 3>, \
-     (const void*)&K<D16, 4>, (const void*)&K<D16, 5>, (const void*)&K<D16, 6>, (const void*)&K<D16, 7>}
-
-const void* fused_kernel_ptr(bool d16, int shape, int fsh) {
-    static const void* const table[2][8] = {FUSED_ROW(wino_fused_kernel, false), FUSED_ROW(wino_fused_kernel, true)};
-    return table[d16 ? 1 : 0][((shape & 3) << 1) | (fsh & 1)];
+// Instances: 2- and 4-byte staging for every block shape with column shift q & 1; 8-byte
+// staging with the shifts (4 - q % 4) % 4 of q = 0, 1, 2 wherever the quad patch fits one unit
+// per thread (all but the 16 x 4 block with shift 3). The tables map (lw, shape, fsh) to them.
+namespace {
+template <int LW, int S>
+constexpr bool fused_instance() {
+    return LW == kFusedLoad8 ? ((S & 3) != 1 && !(S / 4 == 2 && (S & 3) == 3)) : (S & 3) <= 1;
 }
 
-const void* fused_grouped_kernel_ptr(bool d16, int shape, int fsh) {
-    static const void* const table[2][8] = {FUSED_ROW(wino_fused_grouped_kernel, false),
-                                            FUSED_ROW(wino_fused_grouped_kernel, true)};
-    return table[d16 ? 1 : 0][((shape & 3) << 1) | (fsh & 1)];
+template <int LW, int S, bool Grouped>
+const void* fused_ptr_one() {
+    if constexpr (!fused_instance<LW, S>()) return nullptr;
+    else if constexpr (Grouped) return (const void*)&wino_fused_grouped_kernel<LW, S>;
+    else return (const void*)&wino_fused_kernel<LW, S>;
+}
+
+template <bool Grouped, int... I>
+const void* fused_ptr_lookup(int idx, std::integer_sequence<int, I...>) {
+    static const void* const table[] = {fused_ptr_one<I / 16, I % 16, Grouped>()...};
+    return table[idx];
+}
+}  // namespace
+
+const void* fused_kernel_ptr(int lw, int shape, int fsh) {
+    if (lw < 0 || lw > 2 || shape < 0 || shape > 3 || fsh < 0 || fsh > 3) return nullptr;
+    return fused_ptr_lookup<false>(lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 48>{});
+}
+
+const void* fused_grouped_kernel_ptr(int lw, int shape, int fsh) {
+    if (lw < 0 || lw > 2 || shape < 0 || shape > 3 || fsh < 0 || fsh > 3) return nullptr;
+    return fused_ptr_lookup<true>(lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 48>{});
 }

@@ -259,6 +259,31 @@ def test_winograd_geometry_reports_winograd(wc):
         assert L.conv2d_method(ctypes.byref(pb), ctypes.byref(m)) == wc.OK and m.value == wc.METHOD_WINOGRAD
 
 
+@pytest.mark.parametrize("shape,in_addr,fnu,fsh,fpc", [
+    ((32, 256, 56, 56, 256, 1, 1), 256, 1, 3, 40),  # width 4k, 8-byte base: quads, shift 3 (4x16)
+    ((1, 32, 16, 16, 64, 1, 0), 256, 1, 0, 20),     # q = 0: quads, shift 0
+    ((1, 32, 16, 16, 64, 1, 2), 256, 1, 2, 20),     # q = 2: quads, shift 2
+    ((2, 40, 14, 22, 70, 1, 1), 256, 2, 1, 20),     # width 2 mod 4: pixel pairs, shift q & 1
+    ((32, 256, 56, 56, 256, 1, 1), 260, 2, 1, 36),  # base 4 mod 8: pixel pairs
+    ((32, 256, 56, 56, 256, 1, 1), 258, 2, 1, 36),  # base 2 mod 4: 2-byte pixels
+    ((1, 33, 70, 8, 16, 1, 1), 256, 2, 1, 12),      # 16x4 block, shift 3: 544 quads > 512 -> pairs
+    ((1, 33, 70, 4, 16, 1, 2), 256, 1, 2, 12),      # 16x4 block, shift 2: 408 quads
+])
+def test_fused_staging_width(wc, shape, in_addr, fnu, fsh, fpc):
+    """The fused form stages the input patch with 8-byte pixel-quad loads (one unit per
+    thread, fnu = 1) when the width is a multiple of 4, the input base 8-byte aligned and the
+    quad patch fits 512 units; the patch then starts (4 - q % 4) % 4 columns left of 2*tw0 - q
+    and is a multiple of 4 columns wide. Otherwise 4-byte pairs (or 2-byte pixels), two units
+    per thread, shift q & 1, 2*fbw + 4 columns."""
+    n, c, h, w, k, p, q = shape
+    pb = wc.make_problem(n, c, h, w, k, p=p, q=q, ptrs=dict(FAKE, in_=in_addr))
+    rc, ki, prm = _ki(wc, pb)
+    assert rc == wc.OK and prm.form == wc.FORM_FUSED
+    assert (prm.fnu, prm.fsh, prm.fpc) == (fnu, fsh, fpc)
+    assert prm.fpr == 2 * prm.fbh + 2
+    assert ki.slot(2)["lds"] >= 2 * 4 * prm.fpr * prm.fpc * 16
+
+
 @pytest.mark.parametrize("w,band", [(224, 1), (512, 1), (629, 1), (631, 0), (1500, 0)])
 def test_wide_images_stage_one_tile_row(wc, w, band):
     """The input transform stages whole input rows: one tile row (4 rows x 32 channels) of a

@@ -41,7 +41,24 @@ SHAPES = [
     (1, 8, 4, 130, 32, 0, 0),      # 2x32, shift 0, 4-byte
     (1, 8, 4, 131, 32, 0, 0),      # 2x32, shift 0, 2-byte
     (1, 8, 4, 131, 32, 1, 1),      # 2x32, shift 1, 2-byte
+    # 8-byte staging (width a multiple of 4; shift 0, 3, 2 for q = 0, 1, 2) and the 4-byte
+    # cases that remain with it (width 2 mod 4)
+    (1, 8, 4, 132, 32, 0, 0),      # 2x32, shift 0, 8-byte
+    (1, 8, 2, 132, 32, 2, 2),      # 2x32, shift 2, 8-byte
+    (1, 8, 4, 132, 32, 1, 1),      # 2x32, shift 3, 8-byte
+    (3, 64, 8, 66, 96, 0, 0),      # 4x16, shift 0, 4-byte
+    (3, 64, 8, 62, 96, 1, 1),      # 4x16, shift 1, 4-byte
+    (3, 64, 8, 60, 96, 1, 2),      # 4x16, shift 2, 8-byte
+    (1, 32, 16, 16, 64, 1, 2),     # 8x8, shift 2, 8-byte
+    (1, 33, 70, 8, 16, 1, 0),      # 16x4, shift 0, 8-byte
+    (1, 33, 70, 4, 16, 1, 2),      # 16x4, shift 2, 8-byte
 ]
+
+
+def _load_width(w, prm):
+    """Staging load width in bytes: 2 for odd widths, else 8 when the patch takes one quad
+    unit per thread (fnu = 1) and 4 for pixel pairs (always two units per thread)."""
+    return 2 if w % 2 else (8 if prm.fnu == 1 else 4)
 
 
 def _ids(s):
@@ -77,21 +94,23 @@ def test_fused_matches_oracle(shape, runner, cuda):
     assert m["finite"] and m["max_rel"] <= 1e-2
 
 
-def test_fused_unaligned_input_base_takes_2byte_loads(runner, wc, cuda):
-    """An even-width input whose base address is 2 bytes past a 4-byte boundary cannot use
-    4-byte pixel-pair loads: the fused kernel takes its 2-byte staging path and still meets
-    the oracle bound."""
-    n, c, h, w, k, p, q = 2, 40, 14, 22, 70, 1, 1
-    print("case unaligned base", flush=True)
+@pytest.mark.parametrize("shift,fnu", [(1, 2), (2, 2)], ids=["base2mod4", "base4mod8"])
+def test_fused_unaligned_input_base_takes_narrower_loads(shift, fnu, runner, wc, cuda):
+    """A width-24 input (8-byte quads possible) whose base address is 2 bytes past a 4-byte
+    boundary takes the 2-byte staging path, and one 4 bytes past an 8-byte boundary the
+    4-byte pair path (fnu = 2, not the quads' 1); both meet the oracle bound."""
+    n, c, h, w, k, p, q = 2, 40, 14, 24, 70, 1, 1
+    print(f"case unaligned base +{2 * shift} bytes", flush=True)
     geo = wg.Geometry(n, c, h, w, k, p, q)
     x = wg.uniform_pm1(950, (n, c, h, w))
     wt = wg.uniform_pm1(951, (k, c, 3, 3))
-    flat = torch.empty(n * c * h * w + 1, dtype=torch.float16, device=cuda)
-    X = flat[1:].view(n, c, h, w)
+    flat = torch.empty(n * c * h * w + shift, dtype=torch.float16, device=cuda)
+    X = flat[shift:].view(n, c, h, w)
     X.copy_(_to_dev(x, cuda))
-    assert X.data_ptr() % 4 == 2
+    assert X.data_ptr() % 8 == 2 * shift
     conv = runner.WinogradConv(X, _to_dev(wt, cuda), pad=(p, q))
     assert conv.form() == wc.FORM_FUSED
+    assert conv.param().fnu == fnu
     conv.prepare_weights()
     conv.out.fill_(float("nan"))
     conv.launch_slots()
@@ -130,18 +149,24 @@ def test_wide_image_runs_fused_only(runner, wc, cuda):
 
 
 def test_fused_block_shapes_cover_every_variant(runner, cuda):
-    """The shapes above run every kernel instance: 4 tile-block shapes x column shift (q & 1)
-    x staging width (odd W: 2-byte loads). Every patch needs 2 staging units per thread of the
-    512-thread workgroup (648-816 units of 8 channels x 2 pixels)."""
+    """The shapes above run every kernel instance: 4 tile-block shapes x staging width x
+    column shift: 2- and 4-byte loads with shift q & 1 (2 pixel-pair units per thread,
+    648-816 units of 8 channels x 2 pixels), 8-byte loads with shift (4 - q % 4) % 4 (one
+    quad unit per thread, 360-432 units of 8 channels x 4 pixels; the 16x4 block with shift 3
+    would need 544 and takes 4-byte loads)."""
     seen = set()
     for (n, c, h, w, k, p, q) in SHAPES:
         x = torch.zeros((n, c, h, w), dtype=torch.float16, device=cuda)
         wt = torch.zeros((k, c, 3, 3), dtype=torch.float16, device=cuda)
         prm = runner.WinogradConv(x, wt, pad=(p, q)).param()
-        assert prm.fnu == 2
-        seen.add((prm.fbh, prm.fbw, prm.fsh, w % 2))
-    assert seen == {(bh, bw, sh, odd) for (bh, bw) in [(8, 8), (4, 16), (16, 4), (2, 32)]
-                    for sh in (0, 1) for odd in (0, 1)}
+        lw = _load_width(w, prm)
+        assert prm.fnu == (1 if lw == 8 else 2)
+        assert prm.fsh == ((4 - q % 4) % 4 if lw == 8 else q & 1)
+        seen.add((prm.fbh, prm.fbw, prm.fsh, lw))
+    blocks = [(8, 8), (4, 16), (16, 4), (2, 32)]
+    want = {(bh, bw, sh, lw) for (bh, bw) in blocks for sh in (0, 1) for lw in (2, 4)}
+    want |= {(bh, bw, sh, 8) for (bh, bw) in blocks for sh in (0, 2, 3) if (bh, bw, sh) != (16, 4, 3)}
+    assert seen == want
 
 
 def test_fused_full_size_against_fp64_gpu(runner, cuda):
