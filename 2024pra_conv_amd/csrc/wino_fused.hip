@@ -208,7 +208,12 @@ struct PatchStager {
         else stage_setup<NU, d16, 512, PR, PC>(prm, ih0, iw0, p);
     }
     __device__ void load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes) {
-        if constexpr (quad) quad_load(in, chunk_bytes, chan_bytes, q);
+        if constexpr (quad) {
+            // 360-432 quad units: the waves past the last unit issue no loads (one wave-uniform
+            // branch around all eight; their stores go to the dummy area)
+            constexpr int nunits = 4 * PR * (PC / 4);
+            if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 < nunits) quad_load(in, chunk_bytes, chan_bytes, q);
+        }
         else stage_load<NU, d16>(in, chunk_bytes, chan_bytes, p);
     }
     __device__ void store(half_t* buf) const {
