@@ -97,11 +97,12 @@ __device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw
     }
 }
 
-template <int NU, bool D16>
+// Loads units first .. first + N - 1 of the thread's staging.
+template <int N, bool D16, int NU>
 __device__ inline void stage_load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes,
-                                  Staging<NU>& st) {
+                                  Staging<NU>& st, int first) {
 #pragma unroll
-    for (int i = 0; i < NU; ++i) {
+    for (int i = first; i < first + N; ++i) {
         const uint32_t base = st.off[i] + chunk_bytes;
         if (D16) {
             const uint32_t base1 = st.off1[i] + chunk_bytes;
@@ -214,7 +215,14 @@ struct PatchStager {
             constexpr int nunits = 4 * PR * (PC / 4);
             if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 < nunits) quad_load(in, chunk_bytes, chan_bytes, q);
         }
-        else stage_load<NU, d16>(in, chunk_bytes, chan_bytes, p);
+        else {
+            // 648-816 pair units: the second unit set (threads + 512) ends in wave 2-4; the
+            // later waves skip its loads (wave-uniform branch; their stores go to the dummy pair)
+            constexpr int nunits = 4 * PR * (PC / 2);
+            stage_load<1, d16>(in, chunk_bytes, chan_bytes, p, 0);
+            if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 + 512 < nunits)
+                stage_load<1, d16>(in, chunk_bytes, chan_bytes, p, 1);
+        }
     }
     __device__ void store(half_t* buf) const {
         if constexpr (quad) quad_store<PC>(q, buf);
