@@ -50,10 +50,11 @@ __device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t 
 // 8 channels x 2 adjacent pixels. With W even (D16 = false) a pair starts on an even input
 // column, so it is entirely inside or outside the image and its offset is 4-byte aligned: one
 // 4-byte buffer load per channel. With W odd or a base that is not 4-byte aligned (D16) each
-// pixel is a 2-byte load with its own offset. Loads are never predicated: a pixel outside the
-// image gets a byte offset of 2^31 or more, and a channel >= C lands at or beyond the image's
-// last byte; both are past the resource's range (the image, C*H*W*2 < 2^31 bytes), where
-// buffer loads return zero without accessing memory.
+// pixel is a 2-byte load with its own offset. Loads are never predicated per lane (a whole
+// wave skips a unit set none of its lanes holds): a pixel outside the image, or of a thread
+// slot without a unit, gets a byte offset of 2^31 or more, and a channel >= C lands at or
+// beyond the image's last byte; both are past the resource's range (the image,
+// C*H*W*2 < 2^31 bytes), where buffer loads return zero without accessing memory.
 template <int NU>
 struct Staging {
     uint32_t off[NU];   // byte offset of the unit's first pixel in channel g*8 of chunk 0
@@ -214,8 +215,7 @@ struct PatchStager {
             // branch around all eight; their stores go to the dummy area)
             constexpr int nunits = 4 * PR * (PC / 4);
             if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 < nunits) quad_load(in, chunk_bytes, chan_bytes, q);
-        }
-        else {
+        } else {
             // 648-816 pair units: the second unit set (threads + 512) ends in wave 2-4; the
             // later waves skip its loads (wave-uniform branch; their stores go to the dummy pair)
             constexpr int nunits = 4 * PR * (PC / 2);
