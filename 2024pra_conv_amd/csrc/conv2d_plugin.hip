@@ -199,11 +199,11 @@ bool use_fused(const problem_t* pb, const Geometry& g) {
     if (!g.staged_ok) return true;  // too wide for the input transform's LDS staging
     if (tuned_config_lookup(pb) >= 0) return false;
     // Measured on 45 shapes (both forms timed after each had run 50 ms, so at sustained
-    // clocks): with K <= 320 the fused form was faster on all but three (at most 9% slower,
-    // up to 2.9x faster); with K >= 384 the staged GEMM's 128-row tiles win from
-    // N*T*K*C ~ 1.6e9 (up to 1.3x), and from 8e8 when C < K.
+    // clocks): with K <= 384 the fused form was faster on all but two (at most 6% slower,
+    // up to 3.2x faster); with K = 512 the staged GEMM's 128-row tiles win from
+    // N*T*K*C ~ 1.6e9 (up to 1.15x), and from 8e8 when C < K.
     const double work = (double)g.NT * (double)pb->k * (double)pb->c;
-    return pb->k <= 320 || work < 8e8 || (work < 1.2e9 && pb->c >= pb->k);
+    return pb->k <= 384 || work < 8e8 || (work < 1.2e9 && pb->c >= pb->k);
 }
 
 void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {

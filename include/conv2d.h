@@ -192,8 +192,8 @@ int conv2d_weight_hash(const void* host_bytes, size_t nbytes, unsigned long long
 
 /* Which form getkernelInfo and conv2d_forward(-1) use for a Winograd-path problem:
  * CONV2D_FORM_FUSED when the fused form applies, the shape has no tuned-table entry and
- * K <= 320, or N*T*K*C < 8e8, or N*T*K*C < 1.2e9 with C >= K (T = 2x2 tiles per image; where
- * the fused form measured faster on MI355X, by up to 2.9x), else CONV2D_FORM_STAGED (also
+ * K <= 384, or N*T*K*C < 8e8, or N*T*K*C < 1.2e9 with C >= K (T = 2x2 tiles per image; where
+ * the fused form measured faster on MI355X, by up to 3.2x), else CONV2D_FORM_STAGED (also
  * for the direct path). */
 #define CONV2D_FORM_STAGED 0
 #define CONV2D_FORM_FUSED 1

@@ -344,7 +344,7 @@ def test_batched_autotune_keeps_results(runner, wc, cuda):
 
 
 def test_batched_mixes_fused_and_staged_layers(runner, wc, cuda):
-    """A batch with both forms (small, odd-width and K <= 320 layers fused; an 8x256x56x56
+    """A batch with both forms (small, odd-width and K <= 384 layers fused; an 8x256x56x56
     K=512 layer, N*T*K*C = 8.2e8 with C < K, staged) equals separate launches, in eager and
     graph launches."""
     _log("case batched mixed forms")
