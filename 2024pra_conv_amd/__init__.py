@@ -118,6 +118,7 @@ _SIGS = {
     "conv2d_fp8_tile_config": ([ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_fp8_select_tile_config": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_tuned_set": ([_P(Problem), ctypes.c_int], ctypes.c_int),
+    "conv2d_tuned_set_form": ([_P(Problem), ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "conv2d_tuned_clear": ([], ctypes.c_int),
     "conv2d_tuned_count": ([], ctypes.c_int),
     "conv2d_tuned_save": ([ctypes.c_char_p], ctypes.c_int),
@@ -147,6 +148,8 @@ _SIGS = {
                           ctypes.c_void_p], ctypes.c_int),
     "conv2d_mlp_select": ([ctypes.c_void_p, _P(Problem), _P(ctypes.c_int), _P(ctypes.c_float), ctypes.c_void_p],
                           ctypes.c_int),
+    "conv2d_mlp_select_form": ([ctypes.c_void_p, ctypes.c_void_p, _P(Problem), _P(ctypes.c_int), _P(ctypes.c_int),
+                                _P(ctypes.c_float), ctypes.c_void_p], ctypes.c_int),
     "conv2d_version": ([], ctypes.c_char_p),
 }
 

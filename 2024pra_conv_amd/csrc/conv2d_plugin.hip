@@ -193,11 +193,13 @@ int select_config(const problem_t* pb, const Geometry& g) {
 
 // Form choice on the Winograd path. The fused kernel recomputes V once per 64 output
 // channels instead of writing and reading it through HBM; the staged GEMM's 128 x 128 tiles
-// reuse each loaded operand more. A tuned shape keeps its staged configuration.
+// reuse each loaded operand more. A shape in the tuned table takes the form the autotuner
+// measured faster.
 bool use_fused(const problem_t* pb, const Geometry& g) {
     if (g.direct || !g.fused_ok) return false;
     if (!g.staged_ok) return true;  // too wide for the input transform's LDS staging
-    if (tuned_config_lookup(pb) >= 0) return false;
+    const int tuned = tuned_form_lookup(pb);
+    if (tuned >= 0) return tuned == CONV2D_FORM_FUSED;  // measured by the autotuner
     // Measured on 45 shapes (both forms timed after each had run 50 ms, so at sustained
     // clocks): with K <= 384 the fused form was faster on all but two (at most 6% slower,
     // up to 3.2x faster); with K = 512 the staged GEMM's 128-row tiles win from
@@ -267,10 +269,10 @@ Launch slot_filter(const Geometry& g) {
         L.block = dim3(256, 1, 1);
         return L;
     }
-    // one 64-lane workgroup per (U fragment, transform row)
+    // one 4-wave workgroup per U fragment column (32 rows x 16 channels, 16 positions)
     L.kernel = (const void*)&wino_filter_transform;
-    L.grid = dim3((unsigned)(g.KB32 * 2 * g.CB), 4, 1);
-    L.block = dim3(64, 1, 1);
+    L.grid = dim3((unsigned)(g.KB32 * 2 * g.CB), 1, 1);
+    L.block = dim3(256, 1, 1);
     return L;
 }
 

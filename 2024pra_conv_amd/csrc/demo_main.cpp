@@ -1,7 +1,10 @@
 // conv2dfp16demo — command-line harness for the plugin.
 //
 // Counterpart of the reference harness (src/main.cpp:8-166), same positional arguments
-// "n c h w k r s u v p q" (main.cpp:10-20), with these changes:
+// "n c h w k r s u v p q" (main.cpp:10-20). Like the reference it is the plugin's caller: it
+// queries getParamsize / getkernelInfo and launches the four slots itself, each with
+// hipExtLaunchKernel and the parameter blob as the only argument (main.cpp:88-108, 121-124).
+// Changes:
 //   * inputs come from a seeded SplitMix64 generator instead of unseeded rand()
 //     (main.cpp:40-48); --dist positive reproduces the reference's (rand()%255)/255 range;
 //   * workspaces are sized from conv2d_workspace_size, not from 4*h*h*n*c etc.
@@ -16,6 +19,8 @@
 // Prints one JSON line.
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <cmath>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -42,6 +47,20 @@ static uint64_t splitmix64(uint64_t seed, uint64_t i) {
     return z ^ (z >> 31);
 }
 static double unit(uint64_t seed, uint64_t i) { return (double)(splitmix64(seed, i) >> 11) * 0x1p-53; }
+
+// The four slots in order, launched by the caller as the reference harness does
+// (main.cpp:105-108): grid and block from the slot, dynamic LDS from the slot, the blob as the
+// kernel's only (by-value) argument, on `st` (the reference used the null stream).
+static void launch_slots(const kernelInfo_t& ki, void* param, hipStream_t st) {
+    const dim3 groups[4] = {dim3(ki.blockx1, ki.blocky1, ki.blockz1), dim3(ki.blockx2, ki.blocky2, ki.blockz2),
+                            dim3(ki.blockx3, ki.blocky3, ki.blockz3), dim3(ki.blockx4, ki.blocky4, ki.blockz4)};
+    const dim3 threads[4] = {dim3(ki.threadx1, ki.thready1, ki.threadz1), dim3(ki.threadx2, ki.thready2, ki.threadz2),
+                             dim3(ki.threadx3, ki.thready3, ki.threadz3), dim3(ki.threadx4, ki.thready4, ki.threadz4)};
+    const unsigned lds[4] = {ki.dynmicLdsSize1, ki.dynmicLdsSize2, ki.dynmicLdsSize3, ki.dynmicLdsSize4};
+    void* const ptrs[4] = {ki.kernelPtr1, ki.kernelPtr2, ki.kernelPtr3, ki.kernelPtr4};
+    for (int i = 0; i < 4; ++i)
+        HIPCHECK(hipExtLaunchKernel(ptrs[i], groups[i], threads[i], (void**)&param, lds[i], st, nullptr, nullptr, 0));
+}
 
 int main(int argc, char** argv) {
     if (argc < 12) {
@@ -109,7 +128,7 @@ int main(int argc, char** argv) {
     }
     hipStream_t st;
     HIPCHECK(hipStreamCreate(&st));
-    if ((rc = conv2d_launch(&ki, param, st))) return 3;
+    launch_slots(ki, param, st);  // warm-up, and the result that is checked
     HIPCHECK(hipStreamSynchronize(st));
     HIPCHECK(hipMemcpy(h_out.data(), d_out, n_out * 2, hipMemcpyDeviceToHost));
 
@@ -118,8 +137,7 @@ int main(int argc, char** argv) {
     HIPCHECK(hipEventCreate(&e1));
     float ms_ref = 0.f, ms_steady = 0.f;
     HIPCHECK(hipEventRecord(e0, st));
-    for (int i = 0; i < iters; ++i)
-        if ((rc = conv2d_launch(&ki, param, st))) return 3;
+    for (int i = 0; i < iters; ++i) launch_slots(ki, param, st);
     HIPCHECK(hipEventRecord(e1, st));
     HIPCHECK(hipEventSynchronize(e1));
     HIPCHECK(hipEventElapsedTime(&ms_ref, e0, e1));
@@ -145,18 +163,23 @@ int main(int argc, char** argv) {
         se += e * e;
         sr += ref[i] * ref[i];
         // reference acceptance test (main.cpp:144): |gpu - cpu| / gpu > 1% or NaN/Inf
-        if (fabs(y - ref[i]) / y > 0.01 || isnan(y) || isinf(y)) ++ref_criterion_errors;
+        if (fabs(y - ref[i]) / y > 0.01 || std::isnan(y) || std::isinf(y)) ++ref_criterion_errors;
     }
     const double flops = 2.0 * pb.n * pb.k * pb.c * pb.r * pb.s * (double)oh * ow;
     const double us_ref = ms_ref * 1000.0 / iters, us_steady = ms_steady * 1000.0 / iters;
-    int cfg = -1;
+    int cfg = -1, form = -1, method = -1;
     conv2d_select_tile_config(&pb, &cfg);
+    conv2d_form(&pb, &form);
+    conv2d_method(&pb, &method);
     printf("{\"shape\": [%u,%u,%u,%u,%u,%u,%u,%u,%u,%u,%u], \"seed\": %llu, \"dist\": \"%s\", "
-           "\"cfg\": %d, \"us_reference_protocol\": %.3f, \"us_steady\": %.3f, "
+           "\"method\": \"%s\", \"form\": \"%s\", \"cfg\": %d, \"param_bytes\": %d, "
+           "\"us_reference_protocol\": %.3f, \"us_steady\": %.3f, "
            "\"tflops_reference_protocol\": %.3f, \"tflops_steady\": %.3f, "
            "\"max_rel\": %.3e, \"rms_rel\": %.3e, \"reference_criterion_errors\": %ld}\n",
            pb.n, pb.c, pb.h, pb.w, pb.k, pb.r, pb.s, pb.u, pb.v, pb.p, pb.q,
-           (unsigned long long)seed, positive ? "positive" : "uniform", cfg, us_ref, us_steady,
+           (unsigned long long)seed, positive ? "positive" : "uniform",
+           method == CONV2D_METHOD_DIRECT ? "direct" : "winograd", form == CONV2D_FORM_FUSED ? "fused" : "staged",
+           cfg, psize, us_ref, us_steady,
            flops / us_ref * 1e-6, flops / us_steady * 1e-6, max_err / (max_ref > 0 ? max_ref : 1),
            sqrt(se / (sr > 0 ? sr : 1)), ref_criterion_errors);
     free(param);

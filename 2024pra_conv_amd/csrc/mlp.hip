@@ -267,6 +267,28 @@ int conv2d_mlp_train(float* params, float* m, float* v, int t0, const float* X, 
     return rc;
 }
 
+// Predictions of the MLP with parameters `params` for nrows feature rows (host in, host out):
+// one GPU inference launch.
+static int mlp_predict(const float* params, const std::vector<float>& feats, int nrows, float* y, hipStream_t st) {
+    float *dX = nullptr, *dY = nullptr;
+    int rc = CONV2D_OK;
+    if (hipMalloc(&dX, feats.size() * sizeof(float)) != hipSuccess || hipMalloc(&dY, nrows * sizeof(float)) != hipSuccess)
+        rc = CONV2D_ERR_HIP;
+    if (!rc && hipMemcpyAsync(dX, feats.data(), feats.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = CONV2D_ERR_HIP;
+    if (!rc) {
+        hipLaunchKernelGGL(mlp_forward_kernel, dim3(nrows), dim3(64), 0, st, params, (const float*)dX, nrows,
+                           (float*)nullptr, (float*)nullptr, dY);
+        rc = launch_ok();
+    }
+    if (!rc && (hipMemcpyAsync(y, dY, nrows * sizeof(float), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess))
+        rc = CONV2D_ERR_HIP;
+    if (dX) (void)hipFree(dX);
+    if (dY) (void)hipFree(dY);
+    return rc;
+}
+
 // Predicts the time of every tile configuration for this shape with the MLP (inference on
 // the GPU) and returns the configuration with the smallest prediction. pred_us (optional)
 // receives kNumGemmConfigs predicted times in microseconds.
@@ -278,23 +300,7 @@ int conv2d_mlp_select(const float* params, const problem_t* pb, int* best_cfg, f
         int rc = conv2d_mlp_features(pb, c, &feats[(size_t)c * kIn]);
         if (rc) return rc;
     }
-    hipStream_t st = (hipStream_t)stream;
-    float *dX = nullptr, *dY = nullptr;
-    int rc = CONV2D_OK;
-    if (hipMalloc(&dX, feats.size() * sizeof(float)) != hipSuccess || hipMalloc(&dY, nc * sizeof(float)) != hipSuccess)
-        rc = CONV2D_ERR_HIP;
-    if (!rc && hipMemcpyAsync(dX, feats.data(), feats.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess)
-        rc = CONV2D_ERR_HIP;
-    if (!rc) {
-        hipLaunchKernelGGL(mlp_forward_kernel, dim3(nc), dim3(64), 0, st, params, (const float*)dX, nc,
-                           (float*)nullptr, (float*)nullptr, dY);
-        rc = launch_ok();
-    }
-    if (!rc && (hipMemcpyAsync(y.data(), dY, nc * sizeof(float), hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess))
-        rc = CONV2D_ERR_HIP;
-    if (dX) (void)hipFree(dX);
-    if (dY) (void)hipFree(dY);
+    int rc = mlp_predict(params, feats, nc, y.data(), (hipStream_t)stream);
     if (rc) return rc;
     int best = 0;
     for (int c = 1; c < nc; ++c)
@@ -302,6 +308,40 @@ int conv2d_mlp_select(const float* params, const problem_t* pb, int* best_cfg, f
     *best_cfg = best;
     if (pred_us)
         for (int c = 0; c < nc; ++c) pred_us[c] = expf(y[c]);
+    return CONV2D_OK;
+}
+
+// Form and configuration choice from two networks of the same architecture: params_staged
+// predicts the staged form's time (input transform + GEMM) per configuration, params_fused the
+// fused kernel's time (feature row of configuration 0). The fused form is chosen when the
+// problem admits it and its prediction is the smallest. pred_us (optional) receives the 16
+// staged predictions then the fused one (-1 when the fused form does not apply).
+int conv2d_mlp_select_form(const float* params_staged, const float* params_fused, const problem_t* pb, int* form,
+                           int* best_cfg, float* pred_us, void* stream) {
+    if (!params_staged || !params_fused || !pb || !form || !best_cfg) return CONV2D_ERR_NULL;
+    wino::Geometry g;
+    int rc = wino::require_staged(pb, &g);
+    if (rc) return rc;
+    const int nc = kNumGemmConfigs < 16 ? kNumGemmConfigs : 16;
+    std::vector<float> feats((size_t)nc * kIn), y(nc);
+    for (int c = 0; c < nc; ++c)
+        if ((rc = conv2d_mlp_features(pb, c, &feats[(size_t)c * kIn]))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = mlp_predict(params_staged, feats, nc, y.data(), st))) return rc;
+    int best = 0;
+    for (int c = 1; c < nc; ++c)
+        if (y[c] < y[best]) best = c;
+    float yf = 0.f;
+    if (g.fused_ok) {
+        std::vector<float> f0(feats.begin(), feats.begin() + kIn);
+        if ((rc = mlp_predict(params_fused, f0, 1, &yf, st))) return rc;
+    }
+    *best_cfg = best;
+    *form = g.fused_ok && yf < y[best] ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED;
+    if (pred_us) {
+        for (int c = 0; c < nc; ++c) pred_us[c] = expf(y[c]);
+        pred_us[nc] = g.fused_ok ? expf(yf) : -1.f;
+    }
     return CONV2D_OK;
 }
 

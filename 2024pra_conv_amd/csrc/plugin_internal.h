@@ -50,8 +50,10 @@ Launch slot_gemm(const Geometry& g, int cfg);
 Launch slot_noop();
 int launch(const Launch& L, mykernelParamType* a, hipStream_t stream);
 
-// Tuned-table lookup (tuner.hip): configuration index for this exact shape, or -1.
+// Tuned-table lookup (tuner.hip): configuration index / form (CONV2D_FORM_*) recorded for this
+// exact shape, or -1.
 int tuned_config_lookup(const problem_t* pb);
+int tuned_form_lookup(const problem_t* pb);
 
 // Autotuner over a subset of GEMM configurations (tuner.hip): best-of-reps timing of each,
 // gated on bit-identity with the first one's output; does not touch the tuned table.

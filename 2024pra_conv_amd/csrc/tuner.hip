@@ -16,6 +16,7 @@ namespace {
 struct TunedEntry {
     unsigned n, c, h, w, k, p, q;
     int cfg;
+    int form;  // CONV2D_FORM_STAGED or CONV2D_FORM_FUSED
 };
 
 std::mutex g_mu;
@@ -37,23 +38,35 @@ int tuned_config_lookup(const problem_t* pb) {
     return -1;
 }
 
+int tuned_form_lookup(const problem_t* pb) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (const auto& e : g_table)
+        if (same_shape(e, pb)) return e.form;
+    return -1;
+}
+
 }  // namespace wino
 
 extern "C" {
 
-// Records (or replaces) the configuration for one exact shape.
-int conv2d_tuned_set(const problem_t* pb, int cfg) {
+// Records (or replaces) the form and staged configuration for one exact shape.
+int conv2d_tuned_set_form(const problem_t* pb, int cfg, int form) {
     if (!pb) return CONV2D_ERR_NULL;
     if (cfg < 0 || cfg >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
+    if (form != CONV2D_FORM_STAGED && form != CONV2D_FORM_FUSED) return CONV2D_ERR_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto& e : g_table)
         if (same_shape(e, pb)) {
             e.cfg = cfg;
+            e.form = form;
             return CONV2D_OK;
         }
-    g_table.push_back({pb->n, pb->c, pb->h, pb->w, pb->k, pb->p, pb->q, cfg});
+    g_table.push_back({pb->n, pb->c, pb->h, pb->w, pb->k, pb->p, pb->q, cfg, form});
     return CONV2D_OK;
 }
+
+// Records (or replaces) the configuration for one exact shape, in the staged form.
+int conv2d_tuned_set(const problem_t* pb, int cfg) { return conv2d_tuned_set_form(pb, cfg, CONV2D_FORM_STAGED); }
 
 // Empties the table (every shape falls back to the heuristic).
 int conv2d_tuned_clear(void) {
@@ -67,16 +80,16 @@ int conv2d_tuned_count(void) {
     return (int)g_table.size();
 }
 
-// Text format, one shape per line: "n c h w k p q cfg". Lines starting with '#' are
-// comments.
+// Text format, one shape per line: "n c h w k p q cfg form" (form 0 staged, 1 fused; files
+// without the form column load as staged). Lines starting with '#' are comments.
 int conv2d_tuned_save(const char* path) {
     if (!path) return CONV2D_ERR_NULL;
     FILE* f = fopen(path, "w");
     if (!f) return CONV2D_ERR_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
-    fprintf(f, "# winoconv tuned tile configurations: n c h w k p q cfg\n");
+    fprintf(f, "# winoconv tuned tile configurations: n c h w k p q cfg form\n");
     for (const auto& e : g_table)
-        fprintf(f, "%u %u %u %u %u %u %u %d\n", e.n, e.c, e.h, e.w, e.k, e.p, e.q, e.cfg);
+        fprintf(f, "%u %u %u %u %u %u %u %d %d\n", e.n, e.c, e.h, e.w, e.k, e.p, e.q, e.cfg, e.form);
     fclose(f);
     return CONV2D_OK;
 }
@@ -91,9 +104,11 @@ int conv2d_tuned_load(const char* path) {
     while (fgets(line, sizeof line, f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
         TunedEntry e;
-        if (sscanf(line, "%u %u %u %u %u %u %u %d", &e.n, &e.c, &e.h, &e.w, &e.k, &e.p, &e.q,
-                   &e.cfg) != 8 ||
-            e.cfg < 0 || e.cfg >= kNumGemmConfigs) {
+        e.form = CONV2D_FORM_STAGED;
+        const int got = sscanf(line, "%u %u %u %u %u %u %u %d %d", &e.n, &e.c, &e.h, &e.w, &e.k, &e.p,
+                               &e.q, &e.cfg, &e.form);
+        if ((got != 8 && got != 9) || e.cfg < 0 || e.cfg >= kNumGemmConfigs ||
+            (e.form != CONV2D_FORM_STAGED && e.form != CONV2D_FORM_FUSED)) {
             rc = CONV2D_ERR_INVALID;
             break;
         }
@@ -108,6 +123,7 @@ int conv2d_tuned_load(const char* path) {
             if (e.n == r.n && e.c == r.c && e.h == r.h && e.w == r.w && e.k == r.k &&
                 e.p == r.p && e.q == r.q) {
                 e.cfg = r.cfg;
+                e.form = r.form;
                 found = true;
             }
         if (!found) g_table.push_back(r);
@@ -128,6 +144,21 @@ __global__ void tuner_count_mismatch(const uint16_t* __restrict__ a, const uint1
          i += (size_t)gridDim.x * blockDim.x)
         local += a[i] != b[i];
     if (local) atomicAdd(count, local);
+}
+
+// max |a - b| and max |b| over n FP16 values, as the bit patterns of non-negative floats (which
+// order like the floats; a NaN difference orders above every finite one): the fused form's gate.
+__global__ void tuner_max_abs_diff(const _Float16* __restrict__ a, const _Float16* __restrict__ b, size_t n,
+                                   unsigned* __restrict__ out2) {
+    float d = 0.f, m = 0.f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float x = (float)a[i], y = (float)b[i];
+        const float e = fabsf(x - y);
+        d = (e > d || e != e) ? e : d;
+        m = fmaxf(m, fabsf(y));
+    }
+    if (d != 0.f) atomicMax(out2, __float_as_uint(d));
+    if (m != 0.f) atomicMax(out2 + 1, __float_as_uint(m));
 }
 
 namespace {
@@ -233,6 +264,42 @@ int autotune_subset(const problem_t* pb, int reps, void* stream, const int* cfgs
 
 }  // namespace wino
 
+// Times the fused kernel on the problem (the output it writes is gated against the staged
+// output in `golden`: max |fused - staged| <= 1e-2 * max |staged|, the library's accuracy bound,
+// since the fused form's FP16 input transform rounds differently and is not bit-identical).
+// *us = best of reps, or -1 when the fused form does not apply or fails the gate.
+static int time_fused(const problem_t* pb, const wino::Geometry& g, int reps, hipStream_t st,
+                      const uint16_t* golden, float* us) {
+    *us = -1.f;
+    if (!g.fused_ok) return CONV2D_OK;
+    mykernelParamType a;
+    wino::fill_param(pb, g, 0, &a);
+    a.form = CONV2D_FORM_FUSED;
+    float t = 0.f;
+    int rc = wino::time_launch_us(wino::slot_fused(g), &a, reps, st, &t);
+    if (rc) return rc;
+    const size_t nout = (size_t)pb->n * pb->k * g.Oh * g.Ow;
+    unsigned* d2 = nullptr;
+    unsigned h2[2] = {0u, 0u};
+    if (hipMalloc(&d2, 2 * sizeof(unsigned)) != hipSuccess) return CONV2D_ERR_HIP;
+    rc = CONV2D_OK;
+    if (hipMemsetAsync(d2, 0, 2 * sizeof(unsigned), st) != hipSuccess) rc = CONV2D_ERR_HIP;
+    if (!rc) {
+        hipLaunchKernelGGL(tuner_max_abs_diff, dim3(1024), dim3(256), 0, st, (const _Float16*)pb->out,
+                           (const _Float16*)golden, nout, d2);
+        if (hipMemcpyAsync(h2, d2, sizeof h2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = CONV2D_ERR_HIP;
+    }
+    (void)hipFree(d2);
+    if (rc) return rc;
+    float dmax, ymax;
+    memcpy(&dmax, &h2[0], 4);
+    memcpy(&ymax, &h2[1], 4);
+    if (dmax <= 1e-2f * ymax) *us = t;
+    return CONV2D_OK;
+}
+
 extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int* best_cfg,
                                float* times_us, int ntimes) {
     std::vector<int> all(kNumGemmConfigs);
@@ -241,8 +308,30 @@ extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int*
     int best = -1;
     int rc = wino::autotune_subset(pb, reps, stream, all.data(), kNumGemmConfigs, &best, t.data());
     if (rc) return rc;
-    if (times_us)
+    wino::Geometry g;
+    if ((rc = wino::require_staged(pb, &g))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    // the staged output of the winning configuration is the fused form's reference
+    const size_t nout = (size_t)pb->n * pb->k * g.Oh * g.Ow;
+    uint16_t* golden = nullptr;
+    if (hipMalloc(&golden, nout * 2) != hipSuccess) return CONV2D_ERR_HIP;
+    mykernelParamType a;
+    wino::fill_param(pb, g, best, &a);
+    float t_s2 = -1.f, t_fused = -1.f;
+    rc = wino::launch(wino::slot_gemm(g, best), &a, st);
+    if (!rc && hipMemcpyAsync(golden, pb->out, nout * 2, hipMemcpyDeviceToDevice, st) != hipSuccess) rc = CONV2D_ERR_HIP;
+    if (!rc) rc = wino::time_launch_us(wino::slot_input(pb, g), &a, reps, st, &t_s2);
+    if (!rc) rc = time_fused(pb, g, reps, st, golden, &t_fused);
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = CONV2D_ERR_HIP;
+    (void)hipFree(golden);
+    if (rc) return rc;
+    if (times_us) {
         for (int i = 0; i < ntimes && i < kNumGemmConfigs; ++i) times_us[i] = t[i];
+        if (ntimes > kNumGemmConfigs) times_us[kNumGemmConfigs] = t_s2;
+        if (ntimes > kNumGemmConfigs + 1) times_us[kNumGemmConfigs + 1] = t_fused;
+    }
     if (best_cfg) *best_cfg = best;
-    return conv2d_tuned_set(pb, best);
+    // the fused form replaces the input transform and the GEMM
+    const bool fused = t_fused > 0.f && t_fused < t_s2 + t[best];
+    return conv2d_tuned_set_form(pb, best, fused ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED);
 }

@@ -11,12 +11,11 @@
 // ---------------------------------------------------------------------------------------
 // S1: filter transform.
 //   G = [[1,0,0],[1/2,1/2,1/2],[1/2,-1/2,1/2],[0,0,1]]   (reference preprocessing.cuh:77-81)
-// Workgroup = one fragment (32 rows k x 16 channels) and one row a of the 4x4 transform
-// domain (blockIdx.y): each lane produces positions 4a .. 4a+3 of one (k, 8-channel group),
-// so each of its 4 stores is one full 16-byte lane slot and the wave stores whole 1 KiB
-// fragments. Splitting by row gives KB32 * 2CB * 4 workgroups (512 for K = C = 256); the
-// 9-value filters of a lane's 8 channels are 144 contiguous bytes, loaded as nine 16-byte
-// vectors when C % 8 == 0 and the weight base is 16-byte aligned.
+// Workgroup = one fragment column (32 rows k x 16 channels, all 16 positions), 4 waves: the
+// 32 x 16 filters (32 rows of 144 contiguous halves when C % 16 == 0) are staged once in LDS
+// with coalesced 16-byte loads, then wave a produces positions 4a .. 4a+3 of every
+// (k, 8-channel group): each of a lane's 4 stores is one full 16-byte lane slot, so a wave
+// stores whole 1 KiB fragments. KB32 * 2CB workgroups (128 for K = C = 256).
 // ---------------------------------------------------------------------------------------
 // Row a of U = G g G^T for one 3x3 filter g (row-major), FP32: t = row a of G g, then
 // u[b] = (t G^T)[b]. The same expressions, in the same order, as every other producer of U.
@@ -33,40 +32,45 @@ __device__ inline void filter_row_fp32(const float g[9], int a, float u[4]) {
     u[3] = t[2];
 }
 
-__global__ void __launch_bounds__(64) wino_filter_transform(mykernelParamType prm) {
+__global__ void __launch_bounds__(256) wino_filter_transform(mykernelParamType prm) {
+    __shared__ __attribute__((aligned(16))) half_t wl[32 * 144];
     const int frag = blockIdx.x;
-    const int a = blockIdx.y;
-    const int lane = threadIdx.x;
     const int KB = prm.KB32, CB16 = 2 * prm.CB;
     const int kb = frag % KB;
     const int cb = frag / KB;  // 16-channel block
-    const int k = kb * 32 + (lane & 31);
-    const int c0 = cb * 16 + (lane >> 5) * 8;
     const int K = prm.k, C = prm.c;
-    const half_t* src = (const half_t*)prm.pweight + ((size_t)k * C + c0) * 9;
-
-    // the lane's 72 filter values (8 channels x 9), zero outside K and C; with C % 8 == 0 a
-    // lane's channels are all inside C or all outside it
-    half_t w[72];
-    const bool vec = (C & 7) == 0 && (((uintptr_t)prm.pweight) & 15) == 0;
-    if (k < K && c0 < C && vec) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const half8 v = *(const half8*)(src + 8 * i);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) w[8 * i + e] = v[e];
+    const int tid = threadIdx.x;
+    const half_t* wgt = (const half_t*)prm.pweight;
+    // staging: row r = filter k = kb*32 + r, its channels cb*16 .. +15 as 144 halves (channel
+    // major, 9 taps each), zero outside K and C
+    if ((C & 7) == 0 && (((uintptr_t)wgt) & 15) == 0) {
+        // 18 16-byte vectors per row; vector j holds taps of channel group j / 9 (8 channels);
+        // with C % 8 == 0 a group is entirely inside or outside C
+        for (int i = tid; i < 32 * 18; i += 256) {
+            const int r = i / 18, j = i - (i / 18) * 18;
+            const int k = kb * 32 + r;
+            const int cg = cb * 16 + (j / 9) * 8;
+            half8 v = {};
+            if (k < K && cg < C) v = *(const half8*)(wgt + ((size_t)k * C + cb * 16) * 9 + 8 * j);
+            *(half8*)(wl + r * 144 + 8 * j) = v;
         }
     } else {
-#pragma unroll
-        for (int i = 0; i < 72; ++i) w[i] = (k < K && c0 + i / 9 < C) ? src[i] : (half_t)0.0f;
+        for (int i = tid; i < 32 * 144; i += 256) {
+            const int r = i / 144, e = i - (i / 144) * 144;
+            const int k = kb * 32 + r, c = cb * 16 + e / 9;
+            wl[i] = (k < K && c < C) ? wgt[((size_t)k * C + c) * 9 + e % 9] : (half_t)0.0f;
+        }
     }
-
+    __syncthreads();
+    const int a = tid >> 6;
+    const int lane = tid & 63;
+    const half_t* g8 = wl + (lane & 31) * 144 + (lane >> 5) * 72;
     half8 outv[4];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         float g[9], u[4];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] = (float)w[9 * e + i];
+        for (int i = 0; i < 9; ++i) g[i] = (float)g8[9 * e + i];
         filter_row_fp32(g, a, u);
 #pragma unroll
         for (int b = 0; b < 4; ++b) outv[b][e] = (half_t)u[b];
@@ -160,9 +164,12 @@ __global__ void wino_batch_reset(unsigned* counters, int n) {
 __global__ void __launch_bounds__(256) wino_batch_input_transform(const BatchLayer* __restrict__ layers,
                                                                   int nlayers, int total, unsigned* counter) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
-    __shared__ int slot;
+    // The queue slot is the first word of the dynamic staging area (no static LDS on top of a
+    // staging that can fill all 160 KiB): it is written after the previous item's closing
+    // barrier and read before the barrier that precedes the next staging's zero fill.
+    int* slot = (int*)lds;
     for (;;) {
-        const int item = batch_next_item(counter, &slot);
+        const int item = batch_next_item(counter, slot);
         if (item >= total) break;
         const int L = batch_find_layer(layers, nlayers, item, false);
         const BatchLayer& bl = layers[L];

@@ -117,14 +117,16 @@ class WinogradConv:
         """Fused form: input transform inside the GEMM (needs prepare_weights; V unused)."""
         check(self.L.conv2d_fused_gemm(ctypes.byref(self.pb), _stream_handle(stream)), "conv2d_fused_gemm")
 
-    def autotune(self, reps: int = 5, stream=None):
-        """Runs the library's autotuner; returns (best_cfg, per-config best times in us, -1 = gated out)."""
+    def autotune(self, reps: int = 5, stream=None, forms: bool = False):
+        """Runs the library's autotuner; returns (best_cfg, per-config best times in us, -1 = gated out)
+        and, with forms=True, also (input-transform us, fused-kernel us or -1)."""
         n = self.L.conv2d_num_tile_configs()
-        times = (ctypes.c_float * n)()
+        times = (ctypes.c_float * (n + 2))()
         best = ctypes.c_int(-1)
         check(self.L.conv2d_autotune(ctypes.byref(self.pb), reps, _stream_handle(stream), ctypes.byref(best),
-                                     times, n), "conv2d_autotune")
-        return best.value, list(times)
+                                     times, n + 2), "conv2d_autotune")
+        t = list(times)
+        return (best.value, t[:n], t[n], t[n + 1]) if forms else (best.value, t[:n])
 
     def selected_config(self) -> int:
         c = ctypes.c_int()

@@ -20,6 +20,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 from __future__ import annotations
 
 import argparse
+import ctypes
 import importlib
 import json
 import os
@@ -52,6 +53,41 @@ def _events_time(fn, iters: int, stream=None) -> float:
     return e0.elapsed_time(e1) / iters
 
 
+class _Dim3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint), ("y", ctypes.c_uint), ("z", ctypes.c_uint)]
+
+
+class SlotLauncher:
+    """The caller's side of the reference protocol (main.cpp:105-108): each launch slot that
+    getkernelInfo filled, launched with hipExtLaunchKernel and the parameter blob as the only
+    argument, on the given stream."""
+
+    def __init__(self, ki, param, stream):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        f = self.hip.hipExtLaunchKernel
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, _Dim3, _Dim3, ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t,
+                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        self._f = f
+        self.blob = ctypes.cast(param, ctypes.c_void_p)
+        self.args = (ctypes.c_void_p * 1)(self.blob.value)
+        self.stream = ctypes.c_void_p(stream.cuda_stream)
+        self.slots = []
+        for i in range(1, 5):
+            sl = ki.slot(i)
+            self.slots.append((ctypes.c_void_p(sl["kernel"]), _Dim3(*sl["grid"]), _Dim3(*sl["block"]), sl["lds"]))
+
+    def launch(self, i: int) -> None:
+        k, g, b, lds = self.slots[i]
+        rc = self._f(k, g, b, self.args, lds, self.stream, None, None, 0)
+        if rc != 0:
+            raise RuntimeError(f"hipExtLaunchKernel slot {i + 1} returned {rc}")
+
+    def step(self) -> None:
+        for i in range(4):
+            self.launch(i)
+
+
 def clock_warmup(fn, min_ms: float = 60.0, chunk: int = 20, max_calls: int = 4000) -> int:
     """Runs fn in chunks until min_ms of GPU time has passed, so the timed steps see the
     device's sustained clocks (from idle, the step time settles over about 20 ms: 175-180 us
@@ -65,9 +101,10 @@ def clock_warmup(fn, min_ms: float = 60.0, chunk: int = 20, max_calls: int = 400
 
 def cpu_baseline(n, c, h, w, k) -> dict:
     """The reference's CPU checker conv2dcpu (include/verfiy.h:5-43), compiled from its source
-    into oracle/_ref when available, else the oracle's faithful C restatement, timed on one
-    image of the workload (the cost is linear in N) with one thread as the reference runs it;
-    plus the same checker over one image per thread on up to 16 host cores."""
+    into oracle/_ref when available (clang++ -O3 -march=x86-64-v3), else the oracle's faithful C
+    restatement, timed on one image of the workload (the cost is linear in N) with one thread as
+    the reference runs it; plus the same checker over one image per thread on the host's CPU
+    share for one GPU (16 threads on the GPU boxes, fewer if the process may use fewer cores)."""
     import os
     from concurrent.futures import ThreadPoolExecutor
 
@@ -83,7 +120,12 @@ def cpu_baseline(n, c, h, w, k) -> dict:
     t0 = time.perf_counter()
     fn(x, wt)
     dt = time.perf_counter() - t0
-    threads = max(1, min(16, os.cpu_count() or 1))
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    threads = max(1, min(CPU_SHARE_THREADS, usable))
     xs = [rng.standard_normal((1, c, h, w)).astype(np.float16) for _ in range(threads)]
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
@@ -96,9 +138,16 @@ def cpu_baseline(n, c, h, w, k) -> dict:
     return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": kind,
             "sample": f"1 of {WORKLOAD['n']} images (N=1, C={c}, {h}x{w}, K={k}), single thread, "
                       f"{dt:.2f} s, {flops / dt / 1e9:.3f} GFLOP/s; host {model}",
+            "host": {"model": model, "nproc": nproc, "usable_cpus": usable},
             "all_cores": {"value": threads * flops / dt_all / 1e12, "unit": "TFLOP/s", "cores": threads,
-                          "sample": f"{threads} images, one per thread, {dt_all:.2f} s"},
+                          "sample": f"{threads} images, one per thread, {dt_all:.2f} s; threads = min({CPU_SHARE_THREADS}, "
+                                    f"usable CPUs {usable}) of nproc {nproc}: {CPU_SHARE_THREADS} is the host CPU share "
+                                    "of one GPU on the benchmark boxes"},
             "seconds": dt}
+
+
+# worker threads the host-side baseline may use: the CPU share of one GPU on the benchmark boxes
+CPU_SHARE_THREADS = 16
 
 
 PEAK_FP8_TFLOPS = 5033.2  # dense FP8 MFMA
@@ -339,8 +388,15 @@ def main() -> int:
     in_bytes = 2.0 * n * c * h * w
     out_bytes = 2.0 * n * k * oh * ow
 
-    step = (lambda: conv.launch_slots()) if args.cfg < 0 else (
-        lambda: (conv.prepare_weights(), conv.input_transform(), conv.gemm(cfg)))
+    stream = torch.cuda.current_stream()
+    ki, param = conv.kernel_info()
+    slots = SlotLauncher(ki, param, stream)
+    # the dominant kernel of the step: slot 2 (fused kernel) in the fused form, slot 3 (GEMM) staged
+    dom = 1 if fused else 2
+    if args.cfg < 0:
+        step = slots.step
+    else:
+        step = lambda: (conv.prepare_weights(), conv.input_transform(), conv.gemm(cfg))
     if args.copy_mib > 0:
         src = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)
@@ -351,14 +407,32 @@ def main() -> int:
         step()
     extra_warmup = clock_warmup(step) if args.clock_warmup else 0
     torch.cuda.synchronize()
+    # timed region: K four-slot steps between two events; inside each step, events around the
+    # dominant kernel's slot give its launch duration on the stream it runs on
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    inner = [(ev(), ev()) for _ in range(args.steps)] if args.cfg < 0 else []
+    e0, e1 = ev(), ev()
     t0 = time.perf_counter()
-    ms = _events_time(step, args.steps)
-    torch.cuda.synchronize()
+    e0.record(stream)
+    for i in range(args.steps):
+        if args.cfg < 0:
+            for j in range(4):
+                if j == dom:
+                    inner[i][0].record(stream)
+                slots.launch(j)
+                if j == dom:
+                    inner[i][1].record(stream)
+        else:
+            step()
+    e1.record(stream)
+    e1.synchronize()
     wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    ms_dom_live = sum(a.elapsed_time(b) for a, b in inner) / len(inner) if inner else None
 
-    # per-stage and steady-state timings (same stream, HIP events); the staged stages are
-    # reported for either form
+    # per-slot and per-stage times (same stream, HIP events), the staged stages for either form
     conv.prepare_weights()
+    ms_slot = [_events_time(lambda j=j: slots.launch(j), args.steps) for j in range(4)] if args.cfg < 0 else None
     ms_s1 = _events_time(conv.prepare_weights, args.steps)
     ms_fused = _events_time(conv.fused_gemm, args.steps) if fused else None
     ms_s2 = _events_time(conv.input_transform, args.steps)
@@ -378,12 +452,15 @@ def main() -> int:
 
     # roofline of the dominant kernel of the form the step runs
     if fused:
-        k_ms, k_bytes = ms_fused, in_bytes + u_bytes + out_bytes
+        k_bytes = in_bytes + u_bytes + out_bytes
         k_name, k_traffic = ("wino_fused_kernel (input transform + GEMM + output transform)",
                              measured_traffic("fused", "form=fused"))
     else:
-        k_ms, k_bytes = ms_s3, v_bytes + u_bytes + out_bytes
+        k_bytes = v_bytes + u_bytes + out_bytes
         k_name, k_traffic = "wino_gemm (S3, fused output transform)", measured_traffic("staged", f"cfg={cfg}")
+    # the kernel's average duration inside the timed steps (events around its slot), else
+    # (a forced staged configuration) the stage timed on its own
+    k_ms = ms_dom_live if ms_dom_live is not None else ms_s3
     k_tflops = gemm_flops / (k_ms * 1e-3) / 1e12
     k_gbs = k_bytes / (k_ms * 1e-3) / 1e9
     mfma_floor = gemm_flops / (PEAK_FP16_TFLOPS * 1e12)
@@ -395,6 +472,9 @@ def main() -> int:
         roof = {"bound": "mfma", "achieved": round(k_tflops, 1), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(k_tflops / PEAK_FP16_TFLOPS, 4)}
     roof.update({"kernel": k_name, "traffic": k_traffic, "us_per_launch": round(k_ms * 1e3, 2),
+                 "us_per_launch_timed_alone": round((ms_fused if fused else ms_s3) * 1e3, 2),
+                 "duration_source": "HIP events around its slot inside the timed steps" if ms_dom_live is not None
+                 else "HIP events around the stage timed alone",
                  "mfma_frac": round(k_tflops / PEAK_FP16_TFLOPS, 4),
                  "hbm_frac": round(k_gbs / PEAK_HBM_GBS, 4),
                  "algorithmic_bytes": k_bytes, "algorithmic_flops": gemm_flops})
@@ -418,7 +498,8 @@ def main() -> int:
                                "four-slot reference protocol per step (filter transform included)",
                    "form": "fused" if fused else "staged", "tile_config": None if fused else cfg},
         "roofline": roof,
-        "stages_us": {"S1_filter": round(ms_s1 * 1e3, 2),
+        "stages_us": {"slots": [round(t * 1e3, 2) for t in ms_slot] if ms_slot else None,
+                      "S1_filter": round(ms_s1 * 1e3, 2),
                       "fused_kernel": round(ms_fused * 1e3, 2) if fused else None,
                       "staged_S2_input": round(ms_s2 * 1e3, 2), "staged_S3_gemm": round(ms_s3 * 1e3, 2),
                       "staged_S2_S3": round(ms_staged * 1e3, 2), "staged_tile_config": cfg,

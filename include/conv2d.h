@@ -11,9 +11,11 @@
  * output N,K,Oh,Ow, all FP16, with Oh = (h - r + 2p)/u + 1 and Ow = (w - s + 2q)/v + 1
  * (readme.txt:17-19).
  *
- * The Winograd path supports r = s = 3, u = v = 1 and any pad p, q in [0, 2]. Other
- * geometries are rejected with CONV2D_ERR_UNSUPPORTED (the reference computed garbage
- * for them, conv2d.cpp:254-494).
+ * The Winograd path takes r = s = 3, u = v = 1 and any pad p, q in [0, 2]. Every other valid
+ * geometry runs on the general-geometry path, an implicit-GEMM direct convolution behind the
+ * same four slots (the reference computed garbage for them, conv2d.cpp:254-494);
+ * conv2d_method reports which path a problem takes. The FP8, multi-layer and autotuning forms
+ * are Winograd-only and return CONV2D_ERR_UNSUPPORTED for other geometry.
  */
 #ifndef CONV2D_MI355X_H_
 #define CONV2D_MI355X_H_
@@ -82,14 +84,23 @@ typedef struct {
     unsigned int q;
 } problem_t;
 
-/* Four launch slots, as in the reference (conv2d.h:29-66). This implementation fills:
- *   slot 1: filter transform  U = G g G^T        (wino_filter_transform)
- *   slot 2: input transform   V = B^T d B        (wino_input_transform)
- *   slot 3: 16-position FP16 MFMA GEMM with the output transform A^T M A fused into its
- *           epilogue (wino_gemm_*)
- *   slot 4: an empty one-workgroup kernel (the fourth reference stage, conv2d.cpp:17-250,
- *           no longer exists as a separate pass), kept so callers that launch all four
- *           slots unconditionally stay valid. */
+/* Four launch slots, as in the reference (conv2d.h:29-66). This implementation fills, for the
+ * form conv2d_form reports:
+ *   staged form (Winograd path):
+ *     slot 1: filter transform  U = G g G^T        (wino_filter_transform)
+ *     slot 2: input transform   V = B^T d B        (wino_input_transform)
+ *     slot 3: 16-position FP16 MFMA GEMM with the output transform A^T M A fused into its
+ *             epilogue (wino_gemm_*)
+ *     slot 4: an empty one-workgroup kernel (the fourth reference stage, conv2d.cpp:17-250,
+ *             no longer exists as a separate pass)
+ *   fused form (Winograd path):
+ *     slot 1: filter transform; slot 2: wino_fused_kernel (input transform, GEMM and output
+ *     transform in one launch, V never written); slots 3 and 4: empty kernels
+ *   general-geometry path:
+ *     slot 1: filter-tap packing; slot 2: im2col input packing; slot 3: the MFMA GEMM;
+ *     slot 4: an empty kernel
+ * Every slot holds a valid kernel, so callers that launch all four unconditionally (as the
+ * reference harness does, main.cpp:105-108) stay valid. */
 typedef struct {
     unsigned int blockx1, blocky1, blockz1;
     unsigned int threadx1, thready1, threadz1;
@@ -190,11 +201,12 @@ int conv2d_weights_load(const problem_t* problem, const char* path, void* stream
 /* FNV-1a 64 of host bytes, the hash the weight cache uses. */
 int conv2d_weight_hash(const void* host_bytes, size_t nbytes, unsigned long long* hash);
 
-/* Which form getkernelInfo and conv2d_forward(-1) use for a Winograd-path problem:
- * CONV2D_FORM_FUSED when the fused form applies, the shape has no tuned-table entry and
- * K <= 384, or N*T*K*C < 8e8, or N*T*K*C < 1.2e9 with C >= K (T = 2x2 tiles per image; where
- * the fused form measured faster on MI355X, by up to 3.2x), else CONV2D_FORM_STAGED (also
- * for the direct path). */
+/* Which form getkernelInfo and conv2d_forward(-1) use for a Winograd-path problem: the form
+ * recorded for the shape in the tuned table (conv2d_autotune records the faster one it
+ * measured), else CONV2D_FORM_FUSED when the fused form applies and K <= 384, or
+ * N*T*K*C < 8e8, or N*T*K*C < 1.2e9 with C >= K (T = 2x2 tiles per image; where the fused
+ * form measured faster on MI355X, by up to 3.2x), else CONV2D_FORM_STAGED (also for the
+ * direct path). */
 #define CONV2D_FORM_STAGED 0
 #define CONV2D_FORM_FUSED 1
 int conv2d_form(const problem_t* problem, int* form);
@@ -246,21 +258,30 @@ int conv2d_fp8_tile_config(int cfg, int* bk, int* bt, int* threads);
 int conv2d_fp8_select_tile_config(const problem_t* problem, int* cfg);
 
 /* ---- Tuned tile-configuration table (replaces the reference's hard-coded (n, c) dispatch,
- * conv2d.cpp:273-494). getkernelInfo consults it before the shape heuristic. ---- */
+ * conv2d.cpp:273-494). getkernelInfo consults it before the shape heuristic. Each entry holds
+ * the staged GEMM configuration and the form (CONV2D_FORM_STAGED or CONV2D_FORM_FUSED). ---- */
+/* Staged form with configuration cfg. */
 int conv2d_tuned_set(const problem_t* problem, int cfg);
+/* Form `form` and staged configuration cfg. */
+int conv2d_tuned_set_form(const problem_t* problem, int cfg, int form);
 int conv2d_tuned_clear(void);
 int conv2d_tuned_count(void);
-/* Text file, one line per shape: "n c h w k p q cfg". */
+/* Text file, one line per shape: "n c h w k p q cfg form" (files without the form column load
+ * as the staged form). */
 int conv2d_tuned_save(const char* path);
 int conv2d_tuned_load(const char* path);
 
 /* Autotuner (replaces the reference's hand-picked per-shape kernels, conv2d.cpp:273-494):
  * times every GEMM tile configuration on this problem (U and V must already be prepared;
  * problem->out is overwritten) with a deterministic best-of-`reps` protocol on `stream`,
- * gates each configuration on its output being bit-identical to configuration 0's, and
- * records the fastest passing one in the tuned table. times_us[i] (i < ntimes) receives the
- * best time of configuration i in microseconds, or -1 if it failed the gate. Allocates a
- * temporary output-sized buffer. */
+ * gates each configuration on its output being bit-identical to configuration 0's, then times
+ * the input transform and, where the fused form applies, the fused kernel (gated on
+ * max |fused - staged| <= 1e-2 max |staged|: its FP16 input transform rounds differently), and
+ * records the fastest passing configuration and the faster form (fused when it beats input
+ * transform + GEMM) in the tuned table. times_us[i] (i < ntimes) receives: for i < 16 the best
+ * time of configuration i in microseconds, or -1 if it failed the gate; times_us[16] the input
+ * transform; times_us[17] the fused kernel (-1 when the fused form does not apply or failed
+ * its gate). Allocates temporary output-sized buffers. */
 int conv2d_autotune(const problem_t* problem, int reps, void* stream, int* best_cfg,
                     float* times_us, int ntimes);
 
@@ -315,6 +336,14 @@ int conv2d_mlp_train(float* params, float* m, float* v, int t0, const float* X, 
  * configurations); pred_us (host, optional) receives each prediction. */
 int conv2d_mlp_select(const float* params, const problem_t* problem, int* best_cfg,
                       float* pred_us, void* stream);
+/* Form and configuration from two networks of this architecture: params_staged predicts
+ * log(input transform + GEMM time) per configuration, params_fused log(fused kernel time) from
+ * the feature row of configuration 0. *form = CONV2D_FORM_FUSED when the fused form applies and
+ * its prediction is the smallest, else CONV2D_FORM_STAGED with *best_cfg the configuration of
+ * smallest prediction. pred_us (host, optional, 17 floats): the 16 staged predictions, then the
+ * fused one (-1 when the fused form does not apply). Staged-form (Winograd) problems only. */
+int conv2d_mlp_select_form(const float* params_staged, const float* params_fused, const problem_t* problem,
+                           int* form, int* best_cfg, float* pred_us, void* stream);
 
 /* Library version string. */
 const char* conv2d_version(void);

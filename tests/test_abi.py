@@ -92,7 +92,7 @@ def test_getkernelinfo_fills_slots_and_param(wc):
         assert s["kernel"] and all(s["grid"]) and all(s["block"])
     # this layer (K = 256 <= 320) takes the fused form: filter transform, fused kernel, two empty
     assert prm.form == wc.FORM_FUSED
-    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 4, 1) and ki.slot(1)["block"] == (64, 1, 1)
+    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 1, 1) and ki.slot(1)["block"] == (256, 1, 1)
     assert ki.slot(2)["grid"] == (32 * prm.fnbh * prm.fnbw * 4, 1, 1) and ki.slot(2)["block"] == (512, 1, 1)
     assert ki.slot(3)["grid"] == (1, 1, 1) and ki.slot(4)["grid"] == (1, 1, 1)
 
@@ -109,7 +109,7 @@ def test_getkernelinfo_staged_slots(wc):
     # one-dimensional grid over (column block, k-block) pairs, remapped per XCD in the kernel
     assert ki.slot(3)["grid"] == (-(-prm.G32 // (cfg["bt"] // 32)) * -(-prm.KB32 // (cfg["bk"] // 32)), 1, 1)
     assert ki.slot(2)["grid"] == (-(-14 // prm.band), 16, 64)
-    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 4, 1) and ki.slot(1)["block"] == (64, 1, 1)
+    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 1, 1) and ki.slot(1)["block"] == (256, 1, 1)
 
 
 def test_weight_hash_is_fnv1a64():
@@ -378,6 +378,40 @@ def test_tuned_table_roundtrip(wc, tmp_path):
     assert L.conv2d_tuned_set(ctypes.byref(pb), 999) == wc.ERR_INVALID
     bad = tmp_path / "bad.txt"
     bad.write_text("1 2 3\n")
+    assert L.conv2d_tuned_load(str(bad).encode()) == wc.ERR_INVALID
+    L.conv2d_tuned_clear()
+
+
+def test_tuned_table_records_the_form(wc, tmp_path):
+    """A tuned entry carries the form: getkernelInfo fills the fused slots for a fused entry and
+    the staged ones for a staged entry, whatever the hand rule would pick; the form survives a
+    save/load round trip, and a file without the form column loads as staged."""
+    L = wc.load_library()
+    L.conv2d_tuned_clear()
+    pb = wc.make_problem(32, 256, 56, 56, 256, ptrs=FAKE)
+    f = ctypes.c_int()
+    assert L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK and f.value == wc.FORM_FUSED  # K <= 384
+    assert L.conv2d_tuned_set_form(ctypes.byref(pb), 9, wc.FORM_STAGED) == wc.OK
+    assert L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK and f.value == wc.FORM_STAGED
+    rc, ki, prm = _ki(wc, pb)
+    assert rc == wc.OK and prm.form == wc.FORM_STAGED and prm.cfg == 9
+    assert L.conv2d_tuned_set_form(ctypes.byref(pb), 9, wc.FORM_FUSED) == wc.OK
+    rc, ki, prm = _ki(wc, pb)
+    assert rc == wc.OK and prm.form == wc.FORM_FUSED
+    path = str(tmp_path / "tuned.txt").encode()
+    assert L.conv2d_tuned_save(path) == wc.OK
+    assert (tmp_path / "tuned.txt").read_text().splitlines()[1].split() == ["32", "256", "56", "56", "256", "1", "1", "9", "1"]
+    L.conv2d_tuned_clear()
+    assert L.conv2d_tuned_load(path) == wc.OK
+    assert L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK and f.value == wc.FORM_FUSED
+    old = tmp_path / "old.txt"
+    old.write_text("32 256 56 56 256 1 1 9\n")
+    L.conv2d_tuned_clear()
+    assert L.conv2d_tuned_load(str(old).encode()) == wc.OK
+    assert L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK and f.value == wc.FORM_STAGED
+    assert L.conv2d_tuned_set_form(ctypes.byref(pb), 9, 7) == wc.ERR_INVALID
+    bad = tmp_path / "bad.txt"
+    bad.write_text("32 256 56 56 256 1 1 9 5\n")
     assert L.conv2d_tuned_load(str(bad).encode()) == wc.ERR_INVALID
     L.conv2d_tuned_clear()
 

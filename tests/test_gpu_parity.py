@@ -263,6 +263,38 @@ def test_autotuner_gates_times_and_records(runner, wc, cuda):
     conv.L.conv2d_tuned_clear()
 
 
+def test_autotuner_keeps_the_faster_form(runner, wc, cuda):
+    """On the N=32 C=256 56x56 K=256 layer the autotuner times the staged configurations, the
+    input transform and the fused kernel; the form it records (and getkernelInfo then fills
+    into the slots) is the fused one exactly when the fused kernel beat input transform + GEMM."""
+    n, c, h, k = 32, 256, 56, 256
+    _log("case autotune form 32x256x56x256")
+    g = torch.Generator(device=cuda)
+    g.manual_seed(78)
+    x = torch.randn((n, c, h, h), generator=g, device=cuda).half()
+    w = (torch.randn((k, c, 3, 3), generator=g, device=cuda) * (2.0 / (9 * c)) ** 0.5).half()
+    conv = runner.WinogradConv(x, w)
+    conv.L.conv2d_tuned_clear()
+    conv.prepare_weights()
+    conv.input_transform()
+    best, times, t_s2, t_fused = conv.autotune(reps=3, forms=True)
+    _log(f"  best cfg {best} gemm {times[best]:.1f} us, input transform {t_s2:.1f} us, fused {t_fused:.1f} us")
+    assert all(t > 0 for t in times) and t_s2 > 0 and t_fused > 0
+    fused_faster = t_fused < t_s2 + times[best]
+    assert conv.L.conv2d_tuned_count() == 1
+    assert conv.form() == (wc.FORM_FUSED if fused_faster else wc.FORM_STAGED)
+    conv._ki = None
+    assert conv.param().form == conv.form()
+    # the slots run the recorded form and meet the accuracy bound
+    conv.out.fill_(float("nan"))
+    conv.launch_slots()
+    ref = conv.direct_f64()
+    torch.cuda.synchronize()
+    d = (conv.out.double() - ref).abs()
+    assert (d.max() / ref.abs().max()).item() <= 1e-2
+    conv.L.conv2d_tuned_clear()
+
+
 # ---- multi-layer (batched) form --------------------------------------------------------------
 BATCH_SMALL = [(1, 3, 7, 5), (4, 32, 13, 64), (2, 64, 14, 32), (1, 96, 27, 128), (2, 128, 7, 3),
                (1, 40, 9, 20), (3, 130, 12, 140), (1, 16, 28, 16)]
