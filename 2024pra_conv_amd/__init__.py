@@ -96,6 +96,8 @@ _SIGS = {
     "conv2d_gemm": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_forward": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_fused_gemm": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_forward_static": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_static_scratch_bytes": ([], ctypes.c_size_t),
     "conv2d_form": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_weights_save": ([_P(Problem), ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
     "conv2d_weights_load": ([_P(Problem), ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),

@@ -31,6 +31,7 @@ LIB_SOURCES = [
     "batch.hip",
     "u_cache.hip",
     "mlp.hip",
+    "static_scratch.hip",
 ]
 
 # Packed FP32 VALU instructions are kept out of all device code: the compiler would otherwise

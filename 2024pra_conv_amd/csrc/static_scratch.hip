@@ -1,0 +1,80 @@
+// Allocation-free forward pass: conv2d_forward with the transformed input (V on the Winograd
+// path's staged form, the im2col input on the general-geometry path) in a library-owned
+// device array instead of the caller's transform_pin, for callers that cannot allocate
+// (code under graph capture, tight serving loops). No counterpart in the reference, whose
+// harness allocates every workspace itself (main.cpp:33-38).
+//
+// The array `scratch` is a file-scope __device__ object: every GPU the library runs on holds
+// its own copy (kStaticScratchBytes) until the process ends. Host code never uses the array's
+// name as an address: at the first call on a device, with that device current, its address
+// is looked up once with hipGetSymbolAddress and cached per device; kernels receive that
+// address. All calls on one GPU share its copy, which keeps the last call's contents (every
+// kernel writing it overwrites the whole region it later reads, padding included), so calls
+// that overlap on one GPU race.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "conv2d.h"
+#include "plugin_internal.h"
+#include "wino_gemm.h"
+
+namespace {
+
+constexpr size_t kStaticScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+std::mutex g_mu;
+void* g_ptr[kMaxDevices];  // device address of `scratch` per device, once looked up
+
+// Device address of this device's `scratch` (looked up on the first call on the device).
+int scratch_address(void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CONV2D_ERR_HIP;
+    if (dev < 0 || dev >= kMaxDevices) return CONV2D_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CONV2D_ERR_HIP;
+        g_ptr[dev] = p;
+    }
+    *out = g_ptr[dev];
+    return CONV2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t conv2d_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+int conv2d_forward_static(const problem_t* pb, int cfg, void* stream) {
+    if (!pb) return CONV2D_ERR_NULL;
+    if (!pb->in || !pb->out || !pb->transform_filter) return CONV2D_ERR_NULL;
+    wino::Geometry g;
+    int rc = wino::derive_geometry(pb, &g);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    mykernelParamType a;
+    if (cfg < 0 && wino::use_fused(pb, g)) {
+        // the fused form keeps V on chip: nothing beyond the inputs and outputs
+        wino::fill_param(pb, g, wino::select_config(pb, g), &a);
+        a.form = CONV2D_FORM_FUSED;
+        return wino::launch(wino::slot_fused(g), &a, st);
+    }
+    if (!g.direct && !g.staged_ok) return CONV2D_ERR_UNSUPPORTED;
+    if (cfg < 0) cfg = wino::select_config(pb, g);
+    if (cfg >= (g.direct ? kNumDirectConfigs : kNumGemmConfigs)) return CONV2D_ERR_INVALID;
+    if (g.v_bytes > kStaticScratchBytes) return CONV2D_ERR_WORKSPACE;
+    void* v = nullptr;
+    if ((rc = scratch_address(&v))) return rc;
+    problem_t q = *pb;
+    q.transform_pin = (conv2d_half_t*)v;
+    wino::fill_param(&q, g, cfg, &a);
+    if ((rc = wino::launch(wino::slot_input(&q, g), &a, st))) return rc;
+    return wino::launch(wino::slot_gemm(g, cfg), &a, st);
+}
+
+}  // extern "C"

@@ -165,6 +165,19 @@ int conv2d_gemm(const problem_t* problem, int cfg, void* stream);
 /* S2 + S3 (steady state: U prepared once). cfg as for conv2d_gemm. */
 int conv2d_forward(const problem_t* problem, int cfg, void* stream);
 
+/* Allocation-free forward pass. No counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same as conv2d_forward, but the transformed input (V of the staged
+ * Winograd form, the im2col input of the general-geometry path) lives in a library-owned
+ * device array of conv2d_static_scratch_bytes() bytes (64 MiB; one copy per GPU, held until the
+ * process ends) instead of problem->transform_pin, which is not used and may be NULL. U must be
+ * prepared in transform_filter (conv2d_prepare_weights). Calls no allocating routine. A problem
+ * whose transformed input exceeds the array returns CONV2D_ERR_WORKSPACE before launching
+ * anything (the fused form needs no array). Every call on one GPU shares its array: calls that
+ * overlap on one GPU (concurrent streams or threads) race. */
+int conv2d_forward_static(const problem_t* problem, int cfg, void* stream);
+size_t conv2d_static_scratch_bytes(void);
+
 /* Output epilogue (no counterpart in the reference, whose stage 4 stores the plain
  * convolution, conv2d.cpp:37-47): every output becomes act(y + bias[k]), computed in FP32
  * before the output's one FP16 rounding. bias: K FP16 values in device memory, or NULL;
