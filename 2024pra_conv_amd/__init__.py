@@ -23,6 +23,7 @@ OK, ERR_NULL, ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_WORKSPACE, ERR_HIP, ERR_INVALID
 METHOD_WINOGRAD, METHOD_DIRECT = 0, 1
 ACT_NONE, ACT_RELU = 0, 1
 FORM_STAGED, FORM_FUSED = 0, 1
+IO_NCHW, IO_IN_NHWC, IO_OUT_NHWC, IO_OUT_BF16 = 0, 1, 2, 4
 ERROR_NAMES = {
     ERR_NULL: "CONV2D_ERR_NULL",
     ERR_UNSUPPORTED: "CONV2D_ERR_UNSUPPORTED",
@@ -79,7 +80,7 @@ class ParamType(ctypes.Structure):
                [(f, ctypes.c_int) for f in
                 ("tiles_h", "tiles_w", "T", "NT", "G32", "CB", "KB32", "band", "cfg", "flags", "nsteps",
                  "fbh", "fbw", "fnbh", "fnbw", "fpr", "fpc", "fsh", "fnu", "act", "form")] + \
-               [("bias", ctypes.c_void_p)]
+               [("bias", ctypes.c_void_p), ("io", ctypes.c_int), ("io_pad_", ctypes.c_int)]
 
 
 _lib = None
@@ -89,6 +90,9 @@ _P = ctypes.POINTER
 _SIGS = {
     "getParamsize": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
     "getkernelInfo": ([_P(Problem), _P(KernelInfo), ctypes.c_void_p], ctypes.c_int),
+    "getkernelInfo_io": ([_P(Problem), ctypes.c_int, _P(KernelInfo), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_forward_io": ([_P(Problem), ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_direct_f64_io": ([_P(Problem), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "conv2d_workspace_size": ([_P(Problem), _P(ctypes.c_size_t), _P(ctypes.c_size_t), _P(ctypes.c_size_t)], ctypes.c_int),
     "conv2d_launch": ([_P(KernelInfo), ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "conv2d_prepare_weights": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),

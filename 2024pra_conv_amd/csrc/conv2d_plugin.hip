@@ -20,9 +20,13 @@
 
 namespace wino {
 
-int derive_geometry(const problem_t* pb, Geometry* g) {
+int derive_geometry(const problem_t* pb, Geometry* g) { return derive_geometry_io(pb, CONV2D_IO_NCHW, g); }
+
+int derive_geometry_io(const problem_t* pb, int io, Geometry* g) {
     if (!pb || !g) return CONV2D_ERR_NULL;
     memset(g, 0, sizeof(*g));
+    if (io & ~(CONV2D_IO_IN_NHWC | CONV2D_IO_OUT_NHWC | CONV2D_IO_OUT_BF16)) return CONV2D_ERR_INVALID;
+    g->io = io;
     if (pb->n == 0 || pb->c == 0 || pb->h == 0 || pb->w == 0 || pb->k == 0 || pb->r == 0 ||
         pb->s == 0 || pb->u == 0 || pb->v == 0)
         return CONV2D_ERR_INVALID;
@@ -86,6 +90,11 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
             const int pc8 = fused_patch_cols(kFusedLoad8, g->fbw, fused_col_shift(kFusedLoad8, pb->q));
             if (4 * g->fpr * (pc8 / 4) <= kFusedThreads) g->flw = kFusedLoad8;
         }
+        // channel-contiguous input: 16-byte loads of 8 channels per pixel (whole channel groups
+        // and 16-byte aligned pixels needed; otherwise the layer has no fused form)
+        const bool nhwc_in = (io & CONV2D_IO_IN_NHWC) != 0;
+        const bool nhwc_ok = pb->c % 8 == 0 && (in_addr & 15) == 0;
+        if (nhwc_in) g->flw = kFusedLoadNHWC;
         g->fsh = fused_col_shift(g->flw, pb->q);
         g->fpc = fused_patch_cols(g->flw, g->fbw, g->fsh);
         const int units = 4 * g->fpr * (g->fpc / fused_unit_pixels(g->flw));
@@ -96,7 +105,8 @@ int derive_geometry(const problem_t* pb, Geometry* g) {
         // under 2^31 bytes
         const long long img_bytes = ((long long)pb->c + 32) * pb->h * pb->w * 2;
         const long long out_img_bytes = (long long)pb->k * g->Oh * g->Ow * 2;
-        g->fused_ok = g->fnu <= 2 && fused_kernel_ptr(g->flw, fused_shape_index(g->fbh, g->fbw), g->fsh) &&
+        g->fused_ok = (!nhwc_in || nhwc_ok) && g->fnu <= (nhwc_in ? kFusedNhwcUnits : 2) &&
+                      fused_kernel_ptr(g->flw, fused_shape_index(g->fbh, g->fbw), g->fsh) &&
                       g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
                       out_img_bytes < (1LL << 31) && (long long)g->u_bytes < (1LL << 31);
     }
@@ -247,6 +257,7 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
     a->fpc = g.fpc;
     a->fsh = g.fsh;
     a->fnu = g.fnu;
+    a->io = g.io;
     // widest input-row load the staging may use (rows start at multiples of w halves)
     const uintptr_t in_addr = (uintptr_t)pb->in;
     if (pb->w % 8 == 0 && (in_addr & 15) == 0)
@@ -359,12 +370,16 @@ int getParamsize(problem_t* problem, int* paramSize) {
 }
 
 int getkernelInfo(problem_t* problem, kernelInfo_t* ki, void* param) {
+    return getkernelInfo_io(problem, CONV2D_IO_NCHW, ki, param);
+}
+
+int getkernelInfo_io(problem_t* problem, int io, kernelInfo_t* ki, void* param) {
     if (!problem || !ki || !param) return CONV2D_ERR_NULL;
     if (!problem->in || !problem->weight || !problem->out || !problem->transform_pin ||
         !problem->transform_filter)
         return CONV2D_ERR_NULL;
     Geometry g;
-    int rc = derive_geometry(problem, &g);
+    int rc = derive_geometry_io(problem, io, &g);
     if (rc) return rc;
     const int cfg = select_config(problem, g);
     const bool fused = use_fused(problem, g);
@@ -461,11 +476,13 @@ int conv2d_gemm(const problem_t* pb, int cfg, void* stream) {
     return launch(slot_gemm(g, cfg), &a, (hipStream_t)stream);
 }
 
-int conv2d_forward(const problem_t* pb, int cfg, void* stream) {
+int conv2d_forward(const problem_t* pb, int cfg, void* stream) { return conv2d_forward_io(pb, cfg, CONV2D_IO_NCHW, stream); }
+
+int conv2d_forward_io(const problem_t* pb, int cfg, int io, void* stream) {
     int rc = check_ptrs(pb, true, false, true, true, true);
     if (rc) return rc;
     Geometry g;
-    if ((rc = derive_geometry(pb, &g))) return rc;
+    if ((rc = derive_geometry_io(pb, io, &g))) return rc;
     if (cfg < 0 && use_fused(pb, g)) {
         mykernelParamType a;
         fill_param(pb, g, select_config(pb, g), &a);
@@ -574,9 +591,13 @@ int conv2d_direct_tile_config(int cfg, int* bk, int* bt, int* threads) {
 }
 
 int conv2d_direct_f64(const problem_t* pb, double* out_f64, void* stream) {
+    return conv2d_direct_f64_io(pb, CONV2D_IO_NCHW, out_f64, stream);
+}
+
+int conv2d_direct_f64_io(const problem_t* pb, int io, double* out_f64, void* stream) {
     if (!pb || !out_f64 || !pb->in || !pb->weight) return CONV2D_ERR_NULL;
     Geometry g;
-    int rc = derive_geometry(pb, &g);
+    int rc = derive_geometry_io(pb, io, &g);
     if (rc) return rc;
     mykernelParamType a;
     fill_param(pb, g, 0, &a);

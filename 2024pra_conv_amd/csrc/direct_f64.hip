@@ -2,7 +2,8 @@
 //
 // Same cross-correlation as the reference's CPU checker conv2dcpu (include/verfiy.h:5-43):
 // window origin (oh*u - p, ow*v - q), zero padding by bounds test (verfiy.h:30), NCHW
-// input and KCRS weight indexing (verfiy.h:32), loop order c -> r -> s. It differs in one
+// input (or N,H,W,C with CONV2D_IO_IN_NHWC) and KCRS weight indexing (verfiy.h:32), loop
+// order c -> r -> s; the FP64 result is N,K,Oh,Ow. It differs in one
 // respect, on purpose: each FP16 x FP16 product is formed exactly in FP64 (the reference
 // rounds it to FP16 first), and the result stays FP64. Since every product is exact, the
 // sum in this order is bit-identical to a CPU FP64 loop in the same order.
@@ -25,8 +26,11 @@ __global__ void __launch_bounds__(256) conv_direct_f64_kernel(mykernelParamType 
         const int n = (int)(idx / ((size_t)Ow * Oh * K));
         const int h0 = oh * su - pp, w0 = ow * sv - qq;
         double sum = 0.0;
+        const bool nhwc = (prm.io & CONV2D_IO_IN_NHWC) != 0;
         for (int c = 0; c < C; ++c) {
-            const half_t* ip = in + ((size_t)n * C + c) * H * W;
+            // element (ih, iw) of channel c at ip[ih * rs + iw * cs]
+            const half_t* ip = nhwc ? in + (size_t)n * C * H * W + c : in + ((size_t)n * C + c) * H * W;
+            const size_t rs = nhwc ? (size_t)W * C : (size_t)W, cs = nhwc ? (size_t)C : 1;
             const half_t* wp = wt + ((size_t)k * C + c) * R * S;
             for (int r = 0; r < R; ++r) {
                 const int ih = h0 + r;
@@ -34,7 +38,7 @@ __global__ void __launch_bounds__(256) conv_direct_f64_kernel(mykernelParamType 
                 for (int s = 0; s < S; ++s) {
                     const int iw = w0 + s;
                     if (iw < 0 || iw >= W) continue;
-                    sum += (double)ip[ih * W + iw] * (double)wp[r * S + s];
+                    sum += (double)ip[ih * rs + iw * cs] * (double)wp[r * S + s];
                 }
             }
         }

@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(256) direct_pack_weights(mykernelParamType prm
 // One lane per (16-value block L, 32-pixel block pb, fragment lane): output pixel
 // g = pb*32 + lane%32, reduction indices j0 .. j0+7 of the im2col input; zero outside the
 // image, for j >= r*s*C and g >= NT. Consecutive lanes take consecutive output pixels
-// (adjacent input columns for stride 1) and write one contiguous 1 KiB fragment.
+// (adjacent input columns for stride 1) and write one contiguous 1 KiB fragment. The input is
+// N,C,H,W, or N,H,W,C with CONV2D_IO_IN_NHWC (the same image base, channel-contiguous pixels).
 __global__ void __launch_bounds__(256) direct_pack_input(mykernelParamType prm) {
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int P32 = prm.G32;
@@ -74,7 +75,8 @@ __global__ void __launch_bounds__(256) direct_pack_input(mykernelParamType prm) 
     for (int e = 0; e < 8; ++e) {
         const int ih = ih0 + ri, iw = iw0 + si;
         const bool ok = g < prm.NT && j0 + e < jn && ih >= 0 && ih < H && iw >= 0 && iw < W;
-        v[e] = ok ? x[((size_t)c * H + ih) * W + iw] : (half_t)0.0f;
+        const size_t xi = (prm.io & CONV2D_IO_IN_NHWC) ? ((size_t)ih * W + iw) * C + c : ((size_t)c * H + ih) * W + iw;
+        v[e] = ok ? x[xi] : (half_t)0.0f;
         if (++c == C) {
             c = 0;
             if (++si == S) si = 0, ++ri;

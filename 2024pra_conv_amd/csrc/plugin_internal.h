@@ -25,6 +25,7 @@ struct Geometry {
     int flw;    // staging load width (FusedLoad: 2-, 4- or 8-byte loads per channel)
     int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
     long long fitems;
+    int io;  // CONV2D_IO_* tensor layouts and output type
 };
 
 struct Launch {
@@ -34,7 +35,9 @@ struct Launch {
     unsigned lds;
 };
 
-// Validates a problem and derives its geometry (Winograd or general-geometry path).
+// Validates a problem and derives its geometry (Winograd or general-geometry path) for the
+// tensor layouts and output type of io (CONV2D_IO_*; derive_geometry: N,C,H,W FP16).
+int derive_geometry_io(const problem_t* pb, int io, Geometry* g);
 int derive_geometry(const problem_t* pb, Geometry* g);
 // As derive_geometry, but only for problems on the Winograd path (else CONV2D_ERR_UNSUPPORTED).
 int require_winograd(const problem_t* pb, Geometry* g);

@@ -14,6 +14,27 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // 16 B vector of raw bits, used for global <-> LDS copies.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// Two FP32 values rounded to nearest even and packed (a in the low half): one gfx950
+// v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32 instead of two conversions and a pack.
+__device__ inline uint32_t pack_f16x2(float a, float b) {
+    const float2v v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, half2_t));
+}
+__device__ inline uint32_t pack_bf16x2(float a, float b) {
+    const float2v v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+__device__ inline uint32_t pack_out2(bool bf16, float a, float b) {
+    return bf16 ? pack_bf16x2(a, b) : pack_f16x2(a, b);
+}
+// A compile-time flag passed to generic lambdas (one instantiation per output type).
+template <bool B>
+struct BoolTag {
+    static constexpr bool value = B;
+};
 
 // Elements of one packed MFMA fragment (32 rows x 16 channels of FP16 = 64 lanes x 8 halves).
 constexpr int kFragHalves = 512;

@@ -16,18 +16,26 @@ inline int fused_shape_index(int fbh, int fbw) {
 // (odd width, or an input base that is not 4-byte aligned), one 4-byte pixel pair, or one
 // 8-byte pixel quad (width a multiple of 4, base 8-byte aligned, patch within one unit per
 // thread).
-enum FusedLoad { kFusedLoad2 = 0, kFusedLoad4 = 1, kFusedLoad8 = 2 };
+// kFusedLoadNHWC: channel-contiguous (N,H,W,C) input, one 16-byte load of 8 channels per pixel
+// (C % 8 == 0, 16-byte aligned base).
+enum FusedLoad { kFusedLoad2 = 0, kFusedLoad4 = 1, kFusedLoad8 = 2, kFusedLoadNHWC = 3 };
+constexpr int kFusedLoadWidths = 4;
 // Pixels per staging unit (8 channels each).
-constexpr int fused_unit_pixels(int lw) { return lw == kFusedLoad8 ? 4 : 2; }
+constexpr int fused_unit_pixels(int lw) { return lw == kFusedLoad8 ? 4 : (lw == kFusedLoadNHWC ? 1 : 2); }
 // Column shift: the patch starts fsh columns left of 2*tw0 - q (tw0 the block's first tile
 // column, a multiple of 4), so that its first column is even (q & 1) or, with 8-byte loads,
-// a multiple of 4 ((4 - q % 4) % 4).
-constexpr int fused_col_shift(int lw, int q) { return lw == kFusedLoad8 ? (4 - q % 4) % 4 : (q & 1); }
-// Patch columns of a block fbw tiles wide: the windows reach column 2*fbw + fsh + 1; a whole
-// number of staging units per row.
-constexpr int fused_patch_cols(int lw, int fbw, int fsh) {
-    return lw == kFusedLoad8 ? (2 * fbw + fsh + 2 + 3) & ~3 : 2 * fbw + 4;
+// a multiple of 4 ((4 - q % 4) % 4); single-pixel NHWC units need no shift.
+constexpr int fused_col_shift(int lw, int q) {
+    return lw == kFusedLoad8 ? (4 - q % 4) % 4 : (lw == kFusedLoadNHWC ? 0 : (q & 1));
 }
+// Patch columns of a block fbw tiles wide: the windows reach column 2*fbw + fsh + 1; a whole
+// number of staging units per row (and an even count, for the column-parity split).
+constexpr int fused_patch_cols(int lw, int fbw, int fsh) {
+    return lw == kFusedLoad8 ? (2 * fbw + fsh + 2 + 3) & ~3 : (lw == kFusedLoadNHWC ? 2 * fbw + 2 : 2 * fbw + 4);
+}
+// NHWC staging units per thread (4 channel groups x patch rows x patch columns: at most 1584
+// for the 2 x 32 block).
+constexpr int kFusedNhwcUnits = 4;
 // Patch rows of a block fbh tiles high.
 constexpr int fused_patch_rows(int fbh) { return 2 * fbh + 2; }
 

@@ -196,19 +196,74 @@ __device__ inline void quad_store(const QuadStaging& st, half_t* buf) {
     }
 }
 
+// Channel-contiguous (N,H,W,C) input, C % 8 == 0 and a 16-byte aligned base: one unit = (channel
+// group g, patch row, pixel) = the 8 channels of one pixel, one 16-byte load stored to LDS as it
+// is (the patch layout is already channel-interleaved), no permutes. Thread t holds units t,
+// t + 512, ... Pixels outside the image, and channel groups at or beyond C in the last chunk,
+// take an offset past the resource's range and read zero.
+struct NhwcStaging {
+    uint32_t off[kFusedNhwcUnits];  // byte offset of the unit's pixel, channel g*8 of chunk 0
+    int g8[kFusedNhwcUnits];        // first channel of the unit within a chunk (g*8)
+    int lds[kFusedNhwcUnits];       // LDS offset (halves)
+    u32x4 w[kFusedNhwcUnits];
+};
+
+template <int PR, int PC>
+__device__ inline void nhwc_setup(const mykernelParamType& prm, int ih0, int iw0, NhwcStaging& st) {
+    constexpr int nunits = 4 * PR * PC;
+    static_assert(nunits <= kFusedNhwcUnits * 512, "NHWC units per thread");
+    const int H = prm.h, W = prm.w, C = prm.c;
+#pragma unroll
+    for (int i = 0; i < kFusedNhwcUnits; ++i) {
+        const int u = threadIdx.x + 512 * i;
+        const int g = u / (PR * PC);
+        const int rem = u - g * (PR * PC);
+        const int pr = rem / PC;
+        const int col = rem - pr * PC;
+        const int ih = ih0 + pr, iw = iw0 + col;
+        const bool pix = u < nunits && ih >= 0 && ih < H && iw >= 0 && iw < W;
+        st.off[i] = pix ? (uint32_t)(((ih * W + iw) * C + g * 8) * 2) : 0x80000000u;
+        st.g8[i] = u < nunits ? g * 8 : 1 << 20;
+        // column col at parity col & 1, position col / 2 (a thread slot without a unit stores to a
+        // dummy slot past both patch buffers)
+        st.lds[i] = u < nunits ? ((g * PR + pr) * PC + (col & 1) * (PC / 2) + (col >> 1)) * 8 : 2 * 4 * PR * PC * 8;
+    }
+}
+
+// chunk ch: channels ch*32 .. +31; groups at or beyond C (only in the last chunk) read zero
+__device__ inline void nhwc_load(__amdgpu_buffer_rsrc_t in, int ch, int C, NhwcStaging& st) {
+    const int lim = C - ch * 32;
+#pragma unroll
+    for (int i = 0; i < kFusedNhwcUnits; ++i) {
+        const uint32_t o = st.g8[i] < lim ? st.off[i] + (uint32_t)ch * 64u : 0x80000000u;
+        st.w[i] = __builtin_amdgcn_raw_buffer_load_b128(in, (int)o, 0, 0);
+    }
+}
+
+__device__ inline void nhwc_store(const NhwcStaging& st, half_t* buf) {
+#pragma unroll
+    for (int i = 0; i < kFusedNhwcUnits; ++i) *(u32x4*)(buf + st.lds[i]) = st.w[i];
+}
+
 // The staging of one thread for load width LW: 8-byte quads, or up to two 2- or 4-byte pair
 // units (512 threads x 2 units cover the largest pair patch, 816 units).
 template <int LW, int PR, int PC>
 struct PatchStager {
     static constexpr bool quad = LW == kFusedLoad8;
     static constexpr bool d16 = LW == kFusedLoad2;
+    static constexpr bool nhwc = LW == kFusedLoadNHWC;
     static constexpr int NU = 2;
     QuadStaging q;
     Staging<NU> p;
+    NhwcStaging n;
     __device__ void setup(const mykernelParamType& prm, int ih0, int iw0) {
-        if constexpr (quad) quad_setup<PR, PC>(prm, ih0, iw0, q);
+        if constexpr (nhwc) nhwc_setup<PR, PC>(prm, ih0, iw0, n);
+        else if constexpr (quad) quad_setup<PR, PC>(prm, ih0, iw0, q);
         else stage_setup<NU, d16, 512, PR, PC>(prm, ih0, iw0, p);
     }
+    // NHWC: chunk index ch and the channel count; otherwise the chunk's and one channel's
+    // byte strides
+    __device__ void load_nhwc(__amdgpu_buffer_rsrc_t in, int ch, int C) { nhwc_load(in, ch, C, n); }
     __device__ void load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes) {
         if constexpr (quad) {
             // 360-432 quad units: the waves past the last unit issue no loads (one wave-uniform
@@ -225,7 +280,8 @@ struct PatchStager {
         }
     }
     __device__ void store(half_t* buf) const {
-        if constexpr (quad) quad_store<PC>(q, buf);
+        if constexpr (nhwc) nhwc_store(n, buf);
+        else if constexpr (quad) quad_store<PC>(q, buf);
         else stage_store<NU>(p, buf, PC / 2);
     }
 };
@@ -262,6 +318,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     const half_t* in_img = (const half_t*)prm.pin + (size_t)img * C * prm.h * prm.w;
     const uint32_t chan_bytes = (uint32_t)prm.h * prm.w * 2;
     const __amdgpu_buffer_rsrc_t in_rsrc = buffer_rsrc(in_img, (uint32_t)C * chan_bytes);
+    constexpr bool kNhwc = LW == kFusedLoadNHWC;
 
     // U fragments (ks, bb, m) for pos = 4a + 2BH + bb; a block beyond K reads block 0
     const __amdgpu_buffer_rsrc_t u_rsrc =
@@ -316,7 +373,11 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     // register set (two waves per SIMD hide its L2 latency), reloaded after its last use
     PatchStager<LW, PR, PC> st;
     st.setup(prm, ih0, iw0);
-    st.load(in_rsrc, 0u, chan_bytes);
+    auto stage_chunk = [&](int ch) {
+        if constexpr (kNhwc) st.load_nhwc(in_rsrc, ch, C);
+        else st.load(in_rsrc, (uint32_t)ch * 32u * chan_bytes, chan_bytes);
+    };
+    stage_chunk(0);
     half8 ucur[8];
     load_u(ucur, 0);  // in flight while the first patch is stored
     st.store(lds);
@@ -330,7 +391,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         // first MFMA) would also wait for them and expose their whole latency every chunk.
         asm volatile("" ::"v"(ucur[0]), "v"(ucur[1]), "v"(ucur[2]), "v"(ucur[3]), "v"(ucur[4]), "v"(ucur[5]),
                      "v"(ucur[6]), "v"(ucur[7]));
-        if (more) st.load(in_rsrc, (uint32_t)(ch + 1) * 32u * chan_bytes, chan_bytes);
+        if (more) stage_chunk(ch + 1);
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
 #pragma unroll
@@ -438,16 +499,117 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     const __amdgpu_buffer_rsrc_t out_rsrc =
         buffer_rsrc((const half_t*)prm.pout + (size_t)img * K * plane, (uint32_t)K * plane * 2u);
     const bool pairs = (Ow & 1) == 0 && ((uintptr_t)prm.pout & 3) == 0;
-    if (th < prm.tiles_h && tw < prm.tiles_w) {
+    // fast path (wave-uniform): NCHW output (FP16 or bfloat16, whatever the input layout),
+    // every channel of the block exists, every tile has both rows and columns, no bias or
+    // activation: row-pair stores whose channel offset goes in the scalar offset, no
+    // per-element checks
+    const bool out_bf16 = (prm.io & CONV2D_IO_OUT_BF16) != 0;
+    const bool fast = (prm.io & CONV2D_IO_OUT_NHWC) == 0 && pairs && (Oh & 1) == 0 && !prm.bias && !prm.act &&
+                      kblk * 64 + 64 <= K;
+    // one copy of the stores per output type, so that the type test is not repeated per pair
+    auto fast_stores = [&](auto bf_tag) {
+        constexpr bool BF = decltype(bf_tag)::value;
+        if (th < prm.tiles_h && tw < prm.tiles_w) {
+            const uint32_t plane2 = plane * 2u;
+            const uint32_t row_bytes = (uint32_t)Ow * 2u;
+            const uint32_t voff = ((uint32_t)(2 * th) * (uint32_t)Ow + (uint32_t)(2 * tw)) * 2u +
+                                  (uint32_t)(4 * (lane >> 5)) * plane2;
+#pragma unroll
+            for (int eh = 0; eh < 2; ++eh) {
+                const int e4 = 2 * (wave & 1) + eh;
+                floatx4 z[4][2];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        z[r][j] = *(const floatx4*)(Z + ((((r * 2 + j) * 4 + blk) * 4 + e4) * 64 + lane) * 4);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int e = e4 * 4 + q;
+                    const uint32_t kc = (uint32_t)(kblk * 64 + m * 32 + 8 * (e >> 2) + (e & 3));
+                    const int soff = (int)__builtin_amdgcn_readfirstlane(kc * plane2);
+                    const float y00 = (z[0][0][q] + z[1][0][q]) + z[2][0][q];
+                    const float y01 = (z[0][1][q] + z[1][1][q]) + z[2][1][q];
+                    const float y10 = (z[1][0][q] - z[2][0][q]) - z[3][0][q];
+                    const float y11 = (z[1][1][q] - z[2][1][q]) - z[3][1][q];
+                    __builtin_amdgcn_raw_buffer_store_b32(pack_out2(BF, y00, y01), out_rsrc, (int)voff, soff, 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(pack_out2(BF, y10, y11), out_rsrc, (int)(voff + row_bytes),
+                                                          soff, 0);
+                }
+            }
+        }
+    };
+    if (fast) {
+        if (out_bf16)
+            fast_stores(BoolTag<true>{});
+        else
+            fast_stores(BoolTag<false>{});
+    } else if (prm.io != 0) {
+        // NHWC and/or bfloat16 output: per pixel of the tile, the 4 consecutive channels of an
+        // accumulator group (one 8-byte store in NHWC when K % 4 == 0 and the base is 8-byte
+        // aligned, else one 2-byte store per element)
+        if (th < prm.tiles_h && tw < prm.tiles_w) {
+            const bool nhwc = (prm.io & CONV2D_IO_OUT_NHWC) != 0, bf = (prm.io & CONV2D_IO_OUT_BF16) != 0;
+            const bool vec = nhwc && (K & 3) == 0 && ((uintptr_t)prm.pout & 7) == 0;
+            const bool rows[2] = {true, 2 * th + 1 < Oh}, cols[2] = {true, 2 * tw + 1 < Ow};
+            auto bits = [&](float y) {
+                return bf ? (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)y)
+                          : (uint32_t)__builtin_bit_cast(uint16_t, (half_t)y);
+            };
+#pragma unroll
+            for (int eh = 0; eh < 2; ++eh) {
+                const int e4 = 2 * (wave & 1) + eh;
+                floatx4 z[4][2];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        z[r][j] = *(const floatx4*)(Z + ((((r * 2 + j) * 4 + blk) * 4 + e4) * 64 + lane) * 4);
+                const int k0 = kblk * 64 + m * 32 + 8 * e4 + 4 * (lane >> 5);
+                float yv[4][4];  // [pixel (row, column) of the tile][q]
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    float y[4] = {(z[0][0][q] + z[1][0][q]) + z[2][0][q], (z[0][1][q] + z[1][1][q]) + z[2][1][q],
+                                  (z[1][0][q] - z[2][0][q]) - z[3][0][q], (z[1][1][q] - z[2][1][q]) - z[3][1][q]};
+                    if (prm.bias || prm.act) {
+                        const float bk = k0 + q < K ? epilogue_bias(prm, k0 + q) : 0.f;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) y[i] = epilogue(prm, y[i], bk);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) yv[i][q] = y[i];
+                }
+                if (k0 >= K) continue;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (!rows[i >> 1] || !cols[i & 1]) continue;
+                    const uint32_t px = (uint32_t)(2 * th + (i >> 1)) * (uint32_t)Ow + (uint32_t)(2 * tw + (i & 1));
+                    if (vec) {
+                        const u32x2 pk2 = {pack_out2(bf, yv[i][0], yv[i][1]), pack_out2(bf, yv[i][2], yv[i][3])};
+                        __builtin_amdgcn_raw_buffer_store_b64(pk2, out_rsrc,
+                                                              (int)((px * (uint32_t)K + (uint32_t)k0) * 2u), 0, 0);
+                        continue;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (k0 + q >= K) continue;
+                        const uint32_t off = nhwc ? (px * (uint32_t)K + (uint32_t)(k0 + q)) * 2u
+                                                  : ((uint32_t)(k0 + q) * plane + px) * 2u;
+                        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)bits(yv[i][q]), out_rsrc, (int)off, 0, 0);
+                    }
+                }
+            }
+        }
+    } else if (th < prm.tiles_h && tw < prm.tiles_w) {
         const bool r1 = 2 * th + 1 < Oh, c1 = 2 * tw + 1 < Ow;
         const uint32_t pix = ((uint32_t)(2 * th) * (uint32_t)Ow + (uint32_t)(2 * tw)) * 2u;
         const uint32_t row_bytes = (uint32_t)Ow * 2u;
         auto put_row = [&](uint32_t off, float ya, float yb) {
-            const uint16_t ha = __builtin_bit_cast(uint16_t, (half_t)ya);
-            const uint16_t hb = __builtin_bit_cast(uint16_t, (half_t)yb);
             if (pairs) {
-                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)ha | ((uint32_t)hb << 16), out_rsrc, (int)off, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b32(pack_f16x2(ya, yb), out_rsrc, (int)off, 0, 0);
             } else {
+                const uint16_t ha = __builtin_bit_cast(uint16_t, (half_t)ya);
+                const uint16_t hb = __builtin_bit_cast(uint16_t, (half_t)yb);
                 __builtin_amdgcn_raw_buffer_store_b16(ha, out_rsrc, (int)off, 0, 0);
                 if (c1) __builtin_amdgcn_raw_buffer_store_b16(hb, out_rsrc, (int)(off + 2u), 0, 0);
             }
@@ -536,7 +698,8 @@ int fused_threads() { return kFusedThreads; }
 namespace {
 template <int LW, int S>
 constexpr bool fused_instance() {
-    return LW == kFusedLoad8 ? ((S & 3) != 1 && !(S / 4 == 2 && (S & 3) == 3)) : (S & 3) <= 1;
+    return LW == kFusedLoad8 ? ((S & 3) != 1 && !(S / 4 == 2 && (S & 3) == 3))
+                             : (LW == kFusedLoadNHWC ? (S & 3) == 0 : (S & 3) <= 1);
 }
 
 template <int LW, int S, bool Grouped>
@@ -554,11 +717,11 @@ const void* fused_ptr_lookup(int idx, std::integer_sequence<int, I...>) {
 }  // namespace
 
 const void* fused_kernel_ptr(int lw, int shape, int fsh) {
-    if (lw < 0 || lw > 2 || shape < 0 || shape > 3 || fsh < 0 || fsh > 3) return nullptr;
-    return fused_ptr_lookup<false>(lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 48>{});
+    if (lw < 0 || lw >= kFusedLoadWidths || shape < 0 || shape > 3 || fsh < 0 || fsh > 3) return nullptr;
+    return fused_ptr_lookup<false>(lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 16 * kFusedLoadWidths>{});
 }
 
 const void* fused_grouped_kernel_ptr(int lw, int shape, int fsh) {
-    if (lw < 0 || lw > 2 || shape < 0 || shape > 3 || fsh < 0 || fsh > 3) return nullptr;
-    return fused_ptr_lookup<true>(lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 48>{});
+    if (lw < 0 || lw >= kFusedLoadWidths || shape < 0 || shape > 3 || fsh < 0 || fsh > 3) return nullptr;
+    return fused_ptr_lookup<true>(lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 16 * kFusedLoadWidths>{});
 }

@@ -70,6 +70,24 @@ __device__ inline void stage_rows_flat(const half_t* in_n, int cb, int C, int H,
     }
 }
 
+// Row staging of channel-contiguous (N,H,W,C) input: one element per thread and step, with
+// consecutive threads on consecutive channels of one pixel (contiguous in memory); the LDS
+// layout is the same [channel][row][column] as for N,C,H,W input.
+__device__ inline void stage_rows_nhwc(const half_t* in_n, int cb, int C, int H, int W, int ih0, int LR, int LWp,
+                                       int qq, half_t* lds) {
+    const int total = LR * W * 32;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int ch = i & 31;
+        const int rest = i >> 5;
+        const int lr = rest / W;
+        const int iw = rest - lr * W;
+        const int c = cb * 32 + ch;
+        const int ih = ih0 + lr;
+        if (c >= C || ih < 0 || ih >= H) continue;
+        lds[(ch * LR + lr) * LWp + qq + 8 + iw] = in_n[((size_t)ih * W + iw) * C + c];
+    }
+}
+
 // Stages the input rows of tile-row band bi (image n, 32-channel block cb) in LDS: zero fill
 // (padding halo, rows outside the image, channels >= C), then coalesced row loads. Ends with
 // a barrier, so the staging is visible to every wave.
@@ -94,7 +112,9 @@ __device__ inline void stage_input_band(const mykernelParamType& prm, int n, int
     const half_t* in_n = (const half_t*)prm.pin + (size_t)n * C * H * W;
     const int ih0 = 2 * th0 - prm.p;
     const int njobs = 32 * LR;
-    if (prm.flags & 1) {
+    if (prm.io & CONV2D_IO_IN_NHWC) {
+        stage_rows_nhwc(in_n, cb, C, H, W, ih0, LR, LWp, qq, lds);
+    } else if (prm.flags & 1) {
         const int rsub = lane >> 3, ck = lane & 7, nck = W >> 3;
         for (int job = wave * 8 + rsub; job < njobs; job += nwaves * 8) {
             const int ch = job / LR;
@@ -179,6 +199,21 @@ __device__ inline float epilogue_bias(const mykernelParamType& prm, int k) {
     return prm.bias ? (float)((const global_half_t*)prm.bias)[k] : 0.f;
 }
 
+typedef __attribute__((address_space(1))) uint16_t global_u16_t;
+
+// One output value as the bits of its output type (FP16, or bfloat16 with CONV2D_IO_OUT_BF16),
+// rounded once to nearest even from FP32.
+__device__ inline uint16_t out_bits(const mykernelParamType& prm, float y) {
+    return (prm.io & CONV2D_IO_OUT_BF16) ? __builtin_bit_cast(uint16_t, (__bf16)y)
+                                         : __builtin_bit_cast(uint16_t, (half_t)y);
+}
+
+// Element offset of output (image, channel k, row oh, column ow) in the output layout.
+__device__ inline size_t out_index(const mykernelParamType& prm, int img, int k, int oh, int ow) {
+    return (prm.io & CONV2D_IO_OUT_NHWC) ? (((size_t)img * prm.Oh + oh) * prm.Ow + ow) * prm.k + k
+                                         : (((size_t)img * prm.k + k) * prm.Oh + oh) * prm.Ow + ow;
+}
+
 __device__ inline void store_tile(const mykernelParamType& prm, const TileCoord& tc, int k, float y00, float y01,
                                   float y10, float y11) {
     if (prm.bias || prm.act) {
@@ -189,19 +224,28 @@ __device__ inline void store_tile(const mykernelParamType& prm, const TileCoord&
         y11 = epilogue(prm, y11, b);
     }
     const int Oh = prm.Oh, Ow = prm.Ow;
-    global_half_t* o = (global_half_t*)prm.pout + (((size_t)tc.img * prm.k + k) * Oh + tc.oh) * Ow + tc.ow;
-    if ((Ow & 1) == 0 && ((uintptr_t)prm.pout & 3) == 0) {
-        // every row starts 4-byte aligned and a tile's second column is inside: one 4-byte
-        // store per row
+    const bool out_bf16 = (prm.io & CONV2D_IO_OUT_BF16) != 0;
+    if ((prm.io & CONV2D_IO_OUT_NHWC) == 0 && (Ow & 1) == 0 && ((uintptr_t)prm.pout & 3) == 0) {
+        // NCHW output whose rows all start 4-byte aligned (a tile's second column is then
+        // inside): one 4-byte store per row
         typedef __attribute__((address_space(1))) uint32_t global_u32_t;
-        auto pack = [](float a, float b) {
-            return (uint32_t)__builtin_bit_cast(uint16_t, (half_t)a) |
-                   ((uint32_t)__builtin_bit_cast(uint16_t, (half_t)b) << 16);
-        };
-        *(global_u32_t*)o = pack(y00, y01);
-        if (tc.r1) *(global_u32_t*)(o + Ow) = pack(y10, y11);
+        global_u16_t* o = (global_u16_t*)prm.pout + (((size_t)tc.img * prm.k + k) * Oh + tc.oh) * Ow + tc.ow;
+        *(global_u32_t*)o = pack_out2(out_bf16, y00, y01);
+        if (tc.r1) *(global_u32_t*)(o + Ow) = pack_out2(out_bf16, y10, y11);
         return;
     }
+    if (prm.io != 0) {
+        // NHWC and/or bfloat16 output: one 2-byte store per element
+        global_u16_t* o = (global_u16_t*)prm.pout;
+        o[out_index(prm, tc.img, k, tc.oh, tc.ow)] = out_bits(prm, y00);
+        if (tc.c1) o[out_index(prm, tc.img, k, tc.oh, tc.ow + 1)] = out_bits(prm, y01);
+        if (tc.r1) {
+            o[out_index(prm, tc.img, k, tc.oh + 1, tc.ow)] = out_bits(prm, y10);
+            if (tc.c1) o[out_index(prm, tc.img, k, tc.oh + 1, tc.ow + 1)] = out_bits(prm, y11);
+        }
+        return;
+    }
+    global_half_t* o = (global_half_t*)prm.pout + (((size_t)tc.img * prm.k + k) * Oh + tc.oh) * Ow + tc.ow;
     o[0] = (half_t)y00;
     if (tc.c1) o[1] = (half_t)y01;
     if (tc.r1) {
@@ -255,6 +299,13 @@ __device__ inline void store_plain32(const mykernelParamType& prm, const floatx1
                 if (k >= prm.k) continue;
                 float y = acc[mi][ni][j];
                 if (prm.bias || prm.act) y = epilogue(prm, y, epilogue_bias(prm, k));
+                if (prm.io != 0) {
+                    // NHWC and/or bfloat16 output: pixel pix of the image is (pix / Ow, pix % Ow)
+                    const size_t off = (prm.io & CONV2D_IO_OUT_NHWC) ? ((size_t)img * prm.T + pix) * prm.k + k
+                                                                     : ((size_t)img * prm.k + k) * prm.T + pix;
+                    ((global_u16_t*)prm.pout)[off] = out_bits(prm, y);
+                    continue;
+                }
                 o[(size_t)k * prm.T] = (half_t)y;
             }
     }

@@ -25,11 +25,17 @@ class WinogradConv:
     """
 
     def __init__(self, x: torch.Tensor, w: torch.Tensor, pad=(1, 1), out: torch.Tensor | None = None,
-                 workspace: tuple[torch.Tensor, torch.Tensor] | None = None, stride=(1, 1)):
+                 workspace: tuple[torch.Tensor, torch.Tensor] | None = None, stride=(1, 1), io: int = 0):
+        """io: CONV2D_IO_* bits; with IO_IN_NHWC x is (N, H, W, C), with IO_OUT_NHWC the output is
+        (N, Oh, Ow, K), with IO_OUT_BF16 it is bfloat16."""
         assert x.dtype == torch.float16 and w.dtype == torch.float16
         assert x.is_cuda and w.is_cuda and x.is_contiguous() and w.is_contiguous()
         self.L = load_library()
-        n, c, h, wd = x.shape
+        self.io = io
+        if io & 1:
+            n, h, wd, c = x.shape
+        else:
+            n, c, h, wd = x.shape
         k, c2, r, s = w.shape
         assert c == c2
         self.x, self.w = x, w
@@ -41,7 +47,9 @@ class WinogradConv:
         vb, ub, _ = workspace_size(probe)
         self.v_bytes, self.u_bytes = vb, ub
         dev = x.device
-        self.out = out if out is not None else torch.empty((n, k, self.oh, self.ow), dtype=torch.float16, device=dev)
+        oshape = (n, self.oh, self.ow, k) if io & 2 else (n, k, self.oh, self.ow)
+        odt = torch.bfloat16 if io & 4 else torch.float16
+        self.out = out if out is not None else torch.empty(oshape, dtype=odt, device=dev)
         if workspace is not None:
             self.V, self.U = workspace
         else:
@@ -61,7 +69,8 @@ class WinogradConv:
             assert size.value == ctypes.sizeof(ParamType)
             self._param = ctypes.create_string_buffer(size.value)
             ki = KernelInfo()
-            check(self.L.getkernelInfo(ctypes.byref(self.pb), ctypes.byref(ki), self._param), "getkernelInfo")
+            check(self.L.getkernelInfo_io(ctypes.byref(self.pb), self.io, ctypes.byref(ki), self._param),
+                  "getkernelInfo_io")
             self._ki = ki
         return self._ki, self._param
 
@@ -84,6 +93,10 @@ class WinogradConv:
         check(self.L.conv2d_gemm(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_gemm")
 
     def forward(self, cfg: int = -1, stream=None) -> None:
+        if self.io:
+            check(self.L.conv2d_forward_io(ctypes.byref(self.pb), cfg, self.io, _stream_handle(stream)),
+                  "conv2d_forward_io")
+            return
         check(self.L.conv2d_forward(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_forward")
 
     def forward_epilogue(self, bias: torch.Tensor | None, act: int, cfg: int = -1, stream=None) -> None:
@@ -139,10 +152,16 @@ class WinogradConv:
         return m.value
 
     def direct_f64(self, stream=None) -> torch.Tensor:
-        ref = torch.empty(self.out.shape, dtype=torch.float64, device=self.x.device)
-        check(self.L.conv2d_direct_f64(ctypes.byref(self.pb), ctypes.c_void_p(ref.data_ptr()), _stream_handle(stream)),
-              "conv2d_direct_f64")
+        """FP64 direct convolution, N,K,Oh,Ow (whatever the io layouts)."""
+        n, k = self.shape[0], self.shape[4]
+        ref = torch.empty((n, k, self.oh, self.ow), dtype=torch.float64, device=self.x.device)
+        check(self.L.conv2d_direct_f64_io(ctypes.byref(self.pb), self.io, ctypes.c_void_p(ref.data_ptr()),
+                                          _stream_handle(stream)), "conv2d_direct_f64_io")
         return ref
+
+    def out_nchw(self) -> torch.Tensor:
+        """The output as N,K,Oh,Ow (a permuted copy for NHWC output)."""
+        return self.out.permute(0, 3, 1, 2).contiguous() if self.io & 2 else self.out
 
     # --- raw workspace views --------------------------------------------------------------
     def u_halves(self) -> torch.Tensor:

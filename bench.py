@@ -88,10 +88,11 @@ class SlotLauncher:
             self.launch(i)
 
 
-def clock_warmup(fn, min_ms: float = 60.0, chunk: int = 20, max_calls: int = 4000) -> int:
+def clock_warmup(fn, min_ms: float = 250.0, chunk: int = 20, max_calls: int = 4000) -> int:
     """Runs fn in chunks until min_ms of GPU time has passed, so the timed steps see the
-    device's sustained clocks (from idle, the step time settles over about 20 ms: 175-180 us
-    for the first steps, 161 us after). Returns the number of calls made."""
+    device's sustained clocks (from idle, the fused kernel of the workload layer took 142,
+    120, 113, 110 and then 108 us per call over successive 7 ms rounds: it settles after about
+    100 ms of work). Returns the number of calls made."""
     calls, spent = 0, 0.0
     while spent < min_ms and calls < max_calls:
         spent += _events_time(fn, chunk) * chunk

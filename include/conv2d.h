@@ -191,6 +191,25 @@ int conv2d_set_epilogue(void* param, const void* bias, int act);
  * both the Winograd and the general-geometry path. */
 int conv2d_forward_epilogue(const problem_t* problem, int cfg, const void* bias, int act, void* stream);
 
+/* Tensor layouts and output type (no counterpart in the reference, whose tensors are NCHW
+ * FP16, readme.txt:21-24): an `io` word of these bits. The weights stay KCRS FP16 and the
+ * workspaces keep their layouts and sizes. */
+#define CONV2D_IO_NCHW 0     /* input N,C,H,W and output N,K,Oh,Ow, FP16 (the reference's layout) */
+#define CONV2D_IO_IN_NHWC 1  /* input N,H,W,C */
+#define CONV2D_IO_OUT_NHWC 2 /* output N,Oh,Ow,K */
+#define CONV2D_IO_OUT_BF16 4 /* output bfloat16, rounded to nearest even from the FP32 result */
+/* getkernelInfo for an io word: the same slots and blob, with kernels for those layouts (the
+ * fused form takes NHWC input in 16-byte loads of 8 channels when C % 8 == 0 and the input
+ * base is 16-byte aligned; otherwise such a layer runs staged). CONV2D_ERR_INVALID for
+ * unknown bits. Every path (fused, staged, general geometry) and the epilogue
+ * (conv2d_set_epilogue) work with every io word. */
+int getkernelInfo_io(problem_t* problem, int io, kernelInfo_t* kernelInfo, void* param);
+/* conv2d_forward for an io word (cfg -1: getkernelInfo_io's form and configuration). */
+int conv2d_forward_io(const problem_t* problem, int cfg, int io, void* stream);
+/* conv2d_direct_f64 reading the input in the io word's input layout (the FP64 result is always
+ * N,K,Oh,Ow). */
+int conv2d_direct_f64_io(const problem_t* problem, int io, double* out_f64, void* stream);
+
 /* Fused form of S2 + S3 (replaces BTdB*, preprocessing.cuh:273-424, and matrixMul*,
  * matrixmul.cuh:139-556, in one launch): the input transform runs inside the GEMM in FP16
  * arithmetic, as the reference's does, so transform_pin is not used. Needs U prepared

@@ -49,6 +49,10 @@ typedef struct mykernelParamType {
     int act;
     int form;  /* Winograd path: 0 staged (slots: S1, S2, S3, empty), 1 fused (S1, fused, empty x2) */
     const void* bias;
+    /* tensor layouts and output type (CONV2D_IO_* bits of conv2d.h): bit 0 input N,H,W,C (else
+       N,C,H,W), bit 1 output N,Oh,Ow,K (else N,K,Oh,Ow), bit 2 bfloat16 output (else FP16) */
+    int io;
+    int io_pad_;
 } mykernelParamType;
 
 /*

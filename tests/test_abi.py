@@ -40,8 +40,9 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", offsetof(problem_t, n), offsetof(problem_t, q), offsetof(kernelInfo_t, kernelPtr1), offsetof(kernelInfo_t, blockx4));
   printf("%zu %zu %zu %zu %zu\n", offsetof(mykernelParamType, n), offsetof(mykernelParamType, Ow),
          offsetof(mykernelParamType, tiles_h), offsetof(mykernelParamType, cfg), offsetof(mykernelParamType, flags));
-  printf("%zu %zu %zu %zu %zu\n", offsetof(mykernelParamType, nsteps), offsetof(mykernelParamType, fnu),
-         offsetof(mykernelParamType, act), offsetof(mykernelParamType, form), offsetof(mykernelParamType, bias));
+  printf("%zu %zu %zu %zu %zu %zu\n", offsetof(mykernelParamType, nsteps), offsetof(mykernelParamType, fnu),
+         offsetof(mykernelParamType, act), offsetof(mykernelParamType, form), offsetof(mykernelParamType, bias),
+         offsetof(mykernelParamType, io));
   return 0; }
 ''')
     exe = tmp_path / "layout"
@@ -52,7 +53,8 @@ int main(void) {
     P, K, M = wc.Problem, wc.KernelInfo, wc.ParamType
     assert list(map(int, lines[1].split())) == [P.n.offset, P.q.offset, K.kernelPtr1.offset, K.blockx4.offset]
     assert list(map(int, lines[2].split())) == [M.n.offset, M.Ow.offset, M.tiles_h.offset, M.cfg.offset, M.flags.offset]
-    assert list(map(int, lines[3].split())) == [M.nsteps.offset, M.fnu.offset, M.act.offset, M.form.offset, M.bias.offset]
+    assert list(map(int, lines[3].split())) == [M.nsteps.offset, M.fnu.offset, M.act.offset, M.form.offset,
+                                                M.bias.offset, M.io.offset]
     # the reference layouts (probed: problem_t 96 B, kernelInfo_t 160 B; param: pointers 0-47,
     # the 13 shorts at 48-73)
     assert ctypes.sizeof(P) == 96 and ctypes.sizeof(K) == 160
@@ -282,6 +284,55 @@ def test_fused_staging_width(wc, shape, in_addr, fnu, fsh, fpc):
     assert (prm.fnu, prm.fsh, prm.fpc) == (fnu, fsh, fpc)
     assert prm.fpr == 2 * prm.fbh + 2
     assert ki.slot(2)["lds"] >= 2 * 4 * prm.fpr * prm.fpc * 16
+
+
+def _ki_io(wc, pb, io):
+    L = wc.load_library()
+    ki = wc.KernelInfo()
+    buf = ctypes.create_string_buffer(ctypes.sizeof(wc.ParamType))
+    rc = L.getkernelInfo_io(ctypes.byref(pb), io, ctypes.byref(ki), buf)
+    return rc, ki, wc.ParamType.from_buffer_copy(buf)
+
+
+@pytest.mark.parametrize("shape,fpc,fnu", [
+    ((1, 32, 16, 16, 64, 1, 1), 18, 3),      # 8x8 block: 4 x 18 x 18 = 1296 single-pixel units
+    ((32, 256, 56, 56, 256, 1, 1), 34, 3),   # 4x16 block (28x28 tiles): 4 x 10 x 34 = 1360
+    ((1, 32, 70, 6, 16, 1, 0), 10, 3),       # 16x4 block: 4 x 34 x 10 = 1360
+    ((1, 8, 4, 130, 32, 1, 1), 66, 4),       # 2x32 block (2x65 tiles): 4 x 6 x 66 = 1584
+])
+def test_nhwc_input_staging_geometry(wc, shape, fpc, fnu):
+    """Channel-contiguous input on the fused form: one 16-byte unit (8 channels of a pixel)
+    per load, no column shift, 2*fbw + 2 patch columns, up to 4 units per thread; io is
+    carried in the blob, and output-only bits leave the NCHW staging as it is."""
+    n, c, h, w, k, p, q = shape
+    pb = wc.make_problem(n, c, h, w, k, p=p, q=q, ptrs=FAKE)
+    io = wc.IO_IN_NHWC | wc.IO_OUT_NHWC | wc.IO_OUT_BF16
+    rc, ki, prm = _ki_io(wc, pb, io)
+    assert rc == wc.OK and prm.form == wc.FORM_FUSED and prm.io == io
+    assert (prm.fsh, prm.fpc, prm.fnu) == (0, fpc, fnu)
+    assert prm.fpc == 2 * prm.fbw + 2 and prm.fpr == 2 * prm.fbh + 2
+    _, _, nchw = _ki(wc, pb)
+    rc, _, outonly = _ki_io(wc, pb, wc.IO_OUT_NHWC | wc.IO_OUT_BF16)
+    assert rc == wc.OK and nchw.io == 0
+    assert (outonly.fsh, outonly.fpc, outonly.fnu, outonly.form) == (nchw.fsh, nchw.fpc, nchw.fnu, nchw.form)
+
+
+def test_nhwc_input_without_whole_channel_groups_is_staged(wc):
+    """The fused form's NHWC loads need C % 8 == 0 and a 16-byte aligned input; otherwise the
+    layer runs staged (the input transform reads any C and alignment), and bits beyond the
+    three io flags are refused."""
+    for c, addr in ((36, 256), (256, 264)):
+        pb = wc.make_problem(32, c, 56, 56, 256, ptrs=dict(FAKE, in_=addr))
+        rc, ki, prm = _ki_io(wc, pb, wc.IO_IN_NHWC)
+        assert rc == wc.OK and prm.form == wc.FORM_STAGED and prm.io == wc.IO_IN_NHWC, c
+        assert ki.slot(3)["grid"][0] > 1
+        rc, _, prm = _ki_io(wc, pb, wc.IO_NCHW)
+        assert rc == wc.OK and prm.form == wc.FORM_FUSED
+    pb = wc.make_problem(32, 256, 56, 56, 256, ptrs=FAKE)
+    for bad in (8, 16, -1):
+        assert _ki_io(wc, pb, bad)[0] == wc.ERR_INVALID
+    L = wc.load_library()
+    assert L.conv2d_forward_io(ctypes.byref(pb), -1, 8, None) == wc.ERR_INVALID
 
 
 @pytest.mark.parametrize("w,band", [(224, 1), (512, 1), (629, 1), (631, 0), (1500, 0)])
