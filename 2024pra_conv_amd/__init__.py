@@ -22,7 +22,7 @@ DEMO_PATH = os.path.join(PKG_DIR, "lib", "conv2dfp16demo")
 OK, ERR_NULL, ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_WORKSPACE, ERR_HIP, ERR_INVALID, ERR_IO, ERR_MISMATCH = range(9)
 METHOD_WINOGRAD, METHOD_DIRECT = 0, 1
 ACT_NONE, ACT_RELU = 0, 1
-FORM_STAGED, FORM_FUSED = 0, 1
+FORM_STAGED, FORM_FUSED, FORM_F43 = 0, 1, 2
 IO_NCHW, IO_IN_NHWC, IO_OUT_NHWC, IO_OUT_BF16 = 0, 1, 2, 4
 ERROR_NAMES = {
     ERR_NULL: "CONV2D_ERR_NULL",
@@ -100,6 +100,16 @@ _SIGS = {
     "conv2d_gemm": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_forward": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_fused_gemm": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_form_workspace_size": ([_P(Problem), ctypes.c_int, _P(ctypes.c_size_t), _P(ctypes.c_size_t),
+                                    _P(ctypes.c_size_t)], ctypes.c_int),
+    "conv2d_f43_prepare_weights": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_f43_input_transform": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_f43_gemm": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_f43_output_transform": ([_P(Problem), ctypes.c_void_p], ctypes.c_int),
+    "conv2d_f43_forward": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "conv2d_f43_num_tile_configs": ([], ctypes.c_int),
+    "conv2d_f43_tile_config": ([ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int)], ctypes.c_int),
+    "conv2d_f43_select_tile_config": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
     "conv2d_forward_static": ([_P(Problem), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "conv2d_static_scratch_bytes": ([], ctypes.c_size_t),
     "conv2d_form": ([_P(Problem), _P(ctypes.c_int)], ctypes.c_int),
@@ -198,6 +208,17 @@ def workspace_size(pb: Problem):
     a, b, c = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
     check(L.conv2d_workspace_size(ctypes.byref(pb), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
           "conv2d_workspace_size")
+    return a.value, b.value, c.value
+
+
+def form_workspace_size(pb: Problem, form: int):
+    """(V, U, M) bytes of one form, or None where that form does not apply."""
+    L = load_library()
+    a, b, c = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+    rc = L.conv2d_form_workspace_size(ctypes.byref(pb), form, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    if rc == ERR_UNSUPPORTED:
+        return None
+    check(rc, "conv2d_form_workspace_size")
     return a.value, b.value, c.value
 
 

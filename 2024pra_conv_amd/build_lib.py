@@ -32,6 +32,7 @@ LIB_SOURCES = [
     "u_cache.hip",
     "mlp.hip",
     "static_scratch.hip",
+    "wino43.hip",
 ]
 
 # Packed FP32 VALU instructions are kept out of all device code: the compiler would otherwise

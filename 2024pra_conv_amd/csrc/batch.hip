@@ -212,11 +212,15 @@ int conv2d_batch_destroy(conv2d_batch* b) {
     return CONV2D_OK;
 }
 
-// Filter transforms of every layer (once per weight set).
+// Filter transforms of every layer (once per weight set): the F(2x2) U, which both batched
+// forms use (a shape tuned to the F(4x4) form still runs fused or staged here).
 int conv2d_batch_prepare_weights(conv2d_batch* b, void* stream) {
     if (!b) return CONV2D_ERR_NULL;
     for (const problem_t& pb : b->layers) {
-        int rc = conv2d_prepare_weights(&pb, stream);
+        if (!pb.weight || !pb.transform_filter) return CONV2D_ERR_NULL;
+        wino::Geometry g;
+        int rc = wino::derive_geometry(&pb, &g);
+        if (!rc) rc = wino::prepare_weights_form(&pb, g, CONV2D_FORM_STAGED, (hipStream_t)stream);
         if (rc) return rc;
     }
     return CONV2D_OK;

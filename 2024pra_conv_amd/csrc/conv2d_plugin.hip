@@ -110,8 +110,31 @@ int derive_geometry_io(const problem_t* pb, int io, Geometry* g) {
                       g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
                       out_img_bytes < (1LL << 31) && (long long)g->u_bytes < (1LL << 31);
     }
+    g->f43_ok = false;
+    if (g->winograd_ok) {
+        // F(4x4,3x3) form: 4x4 tiles; the first of 4/2/1 tile rows whose input staging fits in
+        // 64 KiB, else one tile row in up to 160 KiB
+        g->t4h = (g->Oh + 3) / 4;
+        g->t4w = (g->Ow + 3) / 4;
+        g->T4 = (long long)g->t4h * g->t4w;
+        g->NT4 = g->T4 * pb->n;
+        g->G32_4 = (g->NT4 + 31) / 32;
+        g->band4 = 0;
+        const int c4[3] = {4, 2, 1};
+        for (int i = 0; i < 3 && !g->band4; ++i) {
+            const int b = c4[i];
+            if (b > 1 && b > g->t4h) continue;
+            if (input43_lds_bytes(b, g->t4w, (int)pb->w, (int)pb->q) <= 65536) g->band4 = b;
+        }
+        if (!g->band4 && input43_lds_bytes(1, g->t4w, (int)pb->w, (int)pb->q) <= 163840) g->band4 = 1;
+        g->v43_bytes = (size_t)36 * 2 * g->CB * (size_t)g->G32_4 * kFragHalves * 2;
+        g->u43_bytes = (size_t)36 * 2 * g->CB * (size_t)g->KB32 * kFragHalves * 2;
+        g->m43_bytes = (size_t)36 * g->KB32 * 32 * (size_t)g->G32_4 * 32 * 2;
+        g->f43_ok = g->band4 > 0 && g->NT4 <= (1LL << 30);
+    }
     // the Winograd path needs the staged form's input-transform band or the fused form
     g->direct = !(g->winograd_ok && (g->band || g->fused_ok));
+    if (g->direct) g->f43_ok = false;
     g->staged_ok = !g->direct && g->band > 0;
     if (g->direct) {
         // general-geometry path: GEMM columns are output pixels, steps run over 32 values
@@ -209,7 +232,8 @@ bool use_fused(const problem_t* pb, const Geometry& g) {
     if (g.direct || !g.fused_ok) return false;
     if (!g.staged_ok) return true;  // too wide for the input transform's LDS staging
     const int tuned = tuned_form_lookup(pb);
-    if (tuned >= 0) return tuned == CONV2D_FORM_FUSED;  // measured by the autotuner
+    if (tuned == CONV2D_FORM_STAGED || tuned == CONV2D_FORM_FUSED)
+        return tuned == CONV2D_FORM_FUSED;  // measured by the autotuner
     // Measured on 45 shapes (both forms timed after each had run 50 ms, so at sustained
     // clocks): with K <= 384 the fused form was faster on all but two (at most 6% slower,
     // up to 3.2x faster); with K = 512 the staged GEMM's 128-row tiles win from
@@ -268,6 +292,99 @@ void fill_param(const problem_t* pb, const Geometry& g, int cfg, mykernelParamTy
         a->flags = 4;
     else
         a->flags = 0;
+}
+
+int select_form(const problem_t* pb, const Geometry& g) {
+    if (g.direct) return CONV2D_FORM_STAGED;
+    if (g.f43_ok && pb->mid_result && tuned_form_lookup(pb) == CONV2D_FORM_F43) return CONV2D_FORM_F43;
+    return use_fused(pb, g) ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED;
+}
+
+void fill_param43(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {
+    fill_param(pb, g, cfg, a);
+    a->tiles_h = g.t4h;
+    a->tiles_w = g.t4w;
+    a->T = (int)g.T4;
+    a->NT = (int)g.NT4;
+    a->G32 = (int)g.G32_4;
+    a->band = g.band4;
+    a->nsteps = g.CB;
+    a->form = CONV2D_FORM_F43;
+}
+
+// F(4x4) GEMM heuristic (used when the tuned table has no entry): the 8-stage 128x128 tile for
+// deep reductions (CB >= 8) with two workgroups per compute unit, the 4-stage 128x128 tile
+// (64x64 waves) when it still gives two, 64x128 when it gives one, else 32x64.
+int select_config43(const problem_t* pb, const Geometry& g) {
+    if (tuned_form_lookup(pb) == CONV2D_FORM_F43) {
+        const int cfg = tuned_config_lookup(pb);
+        if (cfg >= 0 && cfg < kNumPosConfigs) return cfg;
+    }
+    const long long cus = compute_units();
+    auto items = [&](int bk, int bt) {
+        return 36LL * ((g.KB32 + bk / 32 - 1) / (bk / 32)) * ((g.G32_4 + bt / 32 - 1) / (bt / 32));
+    };
+    if (g.CB >= 8 && items(128, 128) >= 2 * cus) return 0;
+    if (items(128, 128) >= 2 * cus) return 5;
+    if (items(64, 128) >= cus) return 4;
+    return 3;
+}
+
+Launch slot_filter43(const Geometry& g) {
+    Launch L;
+    L.kernel = (const void*)&wino43_filter_transform;
+    L.grid = dim3((unsigned)((g.KB32 * 2 * g.CB + 3) / 4), 1, 1);  // one wave per fragment column
+    L.block = dim3(256, 1, 1);
+    L.lds = 0;
+    return L;
+}
+
+Launch slot_input43(const problem_t* pb, const Geometry& g) {
+    Launch L;
+    L.kernel = (const void*)&wino43_input_transform;
+    L.grid = dim3((unsigned)((g.t4h + g.band4 - 1) / g.band4), (unsigned)g.CB, pb->n);
+    L.block = dim3(256, 1, 1);
+    L.lds = (unsigned)input43_lds_bytes(g.band4, g.t4w, (int)pb->w, (int)pb->q);
+    return L;
+}
+
+Launch slot_gemm43(const Geometry& g, int cfg) {
+    const GemmConfig& c = kPosConfigs[cfg];
+    Launch L;
+    L.kernel = c.kernel;
+    const int vf = c.bt / 32, uf = c.bk / 32;
+    const unsigned ng = (unsigned)((g.G32_4 + vf - 1) / vf), nk = (unsigned)((g.KB32 + uf - 1) / uf);
+    L.grid = dim3(ng * nk * 36u, 1, 1);
+    L.block = dim3((unsigned)c.threads, 1, 1);
+    L.lds = 0;
+    return L;
+}
+
+Launch slot_output43(const Geometry& g, int k) {
+    Launch L;
+    L.kernel = (const void*)&wino43_output_transform;
+    L.grid = dim3((unsigned)((g.NT4 + 255) / 256), (unsigned)((k + 3) / 4), 1);
+    L.block = dim3(256, 1, 1);
+    L.lds = 0;
+    return L;
+}
+
+int prepare_weights_form(const problem_t* pb, const Geometry& g, int form, hipStream_t st) {
+    mykernelParamType a;
+    if (form == CONV2D_FORM_F43) {
+        fill_param43(pb, g, 0, &a);
+        return launch(slot_filter43(g), &a, st);
+    }
+    fill_param(pb, g, 0, &a);
+    return launch(slot_filter(g), &a, st);
+}
+
+// S2 + S3 + S4 of the F(4x4) form with the epilogue in a (a filled by fill_param43).
+static int forward43(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a, hipStream_t st) {
+    int rc;
+    if ((rc = launch(slot_input43(pb, g), a, st))) return rc;
+    if ((rc = launch(slot_gemm43(g, cfg), a, st))) return rc;
+    return launch(slot_output43(g, (int)pb->k), a, st);
 }
 
 Launch slot_filter(const Geometry& g) {
@@ -381,8 +498,22 @@ int getkernelInfo_io(problem_t* problem, int io, kernelInfo_t* ki, void* param) 
     Geometry g;
     int rc = derive_geometry_io(problem, io, &g);
     if (rc) return rc;
+    const int form = select_form(problem, g);
+    if (form == CONV2D_FORM_F43) {
+        const int cfg = select_config43(problem, g);
+        fill_param43(problem, g, cfg, (mykernelParamType*)param);
+        put_slot(slot_filter43(g), &ki->blockx1, &ki->blocky1, &ki->blockz1, &ki->threadx1, &ki->thready1,
+                 &ki->threadz1, &ki->dynmicLdsSize1, &ki->kernelPtr1);
+        put_slot(slot_input43(problem, g), &ki->blockx2, &ki->blocky2, &ki->blockz2, &ki->threadx2, &ki->thready2,
+                 &ki->threadz2, &ki->dynmicLdsSize2, &ki->kernelPtr2);
+        put_slot(slot_gemm43(g, cfg), &ki->blockx3, &ki->blocky3, &ki->blockz3, &ki->threadx3, &ki->thready3,
+                 &ki->threadz3, &ki->dynmicLdsSize3, &ki->kernelPtr3);
+        put_slot(slot_output43(g, (int)problem->k), &ki->blockx4, &ki->blocky4, &ki->blockz4, &ki->threadx4,
+                 &ki->thready4, &ki->threadz4, &ki->dynmicLdsSize4, &ki->kernelPtr4);
+        return CONV2D_OK;
+    }
     const int cfg = select_config(problem, g);
-    const bool fused = use_fused(problem, g);
+    const bool fused = form == CONV2D_FORM_FUSED;
     fill_param(problem, g, cfg, (mykernelParamType*)param);
     ((mykernelParamType*)param)->form = fused ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED;
     put_slot(slot_filter(g), &ki->blockx1, &ki->blocky1, &ki->blockz1, &ki->threadx1,
@@ -402,9 +533,27 @@ int conv2d_workspace_size(const problem_t* problem, size_t* pin_bytes, size_t* f
     Geometry g;
     int rc = derive_geometry(problem, &g);
     if (rc) return rc;
-    if (pin_bytes) *pin_bytes = g.v_bytes;
-    if (filter_bytes) *filter_bytes = g.u_bytes;
-    if (mid_bytes) *mid_bytes = 0;
+    // every form the library may choose for the shape: the F(4x4) form's U and V where it
+    // applies (V is smaller than F(2x2)'s, U 2.25x larger) and its M in mid_result
+    if (pin_bytes) *pin_bytes = g.f43_ok && g.v43_bytes > g.v_bytes ? g.v43_bytes : g.v_bytes;
+    if (filter_bytes) *filter_bytes = g.f43_ok && g.u43_bytes > g.u_bytes ? g.u43_bytes : g.u_bytes;
+    if (mid_bytes) *mid_bytes = g.f43_ok ? g.m43_bytes : 0;
+    return CONV2D_OK;
+}
+
+int conv2d_form_workspace_size(const problem_t* problem, int form, size_t* pin_bytes, size_t* filter_bytes,
+                               size_t* mid_bytes) {
+    if (!problem) return CONV2D_ERR_NULL;
+    if (form != CONV2D_FORM_STAGED && form != CONV2D_FORM_FUSED && form != CONV2D_FORM_F43)
+        return CONV2D_ERR_INVALID;
+    Geometry g;
+    int rc = derive_geometry(problem, &g);
+    if (rc) return rc;
+    const bool f43 = form == CONV2D_FORM_F43;
+    if (f43 && !g.f43_ok) return CONV2D_ERR_UNSUPPORTED;
+    if (pin_bytes) *pin_bytes = f43 ? g.v43_bytes : g.v_bytes;
+    if (filter_bytes) *filter_bytes = f43 ? g.u43_bytes : g.u_bytes;
+    if (mid_bytes) *mid_bytes = f43 ? g.m43_bytes : 0;
     return CONV2D_OK;
 }
 
@@ -447,9 +596,7 @@ int conv2d_prepare_weights(const problem_t* pb, void* stream) {
     if (rc) return rc;
     Geometry g;
     if ((rc = derive_geometry(pb, &g))) return rc;
-    mykernelParamType a;
-    fill_param(pb, g, 0, &a);
-    return launch(slot_filter(g), &a, (hipStream_t)stream);
+    return prepare_weights_form(pb, g, select_form(pb, g), (hipStream_t)stream);
 }
 
 int conv2d_input_transform(const problem_t* pb, void* stream) {
@@ -483,7 +630,14 @@ int conv2d_forward_io(const problem_t* pb, int cfg, int io, void* stream) {
     if (rc) return rc;
     Geometry g;
     if ((rc = derive_geometry_io(pb, io, &g))) return rc;
-    if (cfg < 0 && use_fused(pb, g)) {
+    const int form = cfg < 0 ? select_form(pb, g) : CONV2D_FORM_STAGED;
+    if (form == CONV2D_FORM_F43) {
+        mykernelParamType a;
+        const int c43 = select_config43(pb, g);
+        fill_param43(pb, g, c43, &a);
+        return forward43(pb, g, c43, &a, (hipStream_t)stream);
+    }
+    if (form == CONV2D_FORM_FUSED) {
         mykernelParamType a;
         fill_param(pb, g, select_config(pb, g), &a);
         a.form = CONV2D_FORM_FUSED;
@@ -513,7 +667,16 @@ int conv2d_forward_epilogue(const problem_t* pb, int cfg, const void* bias, int 
     if (act != CONV2D_ACT_NONE && act != CONV2D_ACT_RELU) return CONV2D_ERR_INVALID;
     Geometry g;
     if ((rc = derive_geometry(pb, &g))) return rc;
-    if (cfg < 0 && use_fused(pb, g)) {
+    const int form = cfg < 0 ? select_form(pb, g) : CONV2D_FORM_STAGED;
+    if (form == CONV2D_FORM_F43) {
+        mykernelParamType a;
+        const int c43 = select_config43(pb, g);
+        fill_param43(pb, g, c43, &a);
+        a.bias = bias;
+        a.act = act;
+        return forward43(pb, g, c43, &a, (hipStream_t)stream);
+    }
+    if (form == CONV2D_FORM_FUSED) {
         mykernelParamType a;
         fill_param(pb, g, select_config(pb, g), &a);
         a.form = CONV2D_FORM_FUSED;
@@ -537,7 +700,7 @@ int conv2d_form(const problem_t* pb, int* form) {
     Geometry g;
     int rc = derive_geometry(pb, &g);
     if (rc) return rc;
-    *form = use_fused(pb, g) ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED;
+    *form = select_form(pb, g);
     return CONV2D_OK;
 }
 
@@ -607,6 +770,89 @@ int conv2d_direct_f64_io(const problem_t* pb, int io, double* out_f64, void* str
     hipLaunchKernelGGL(conv_direct_f64_kernel, dim3((unsigned)blocks), dim3(256), 0,
                        (hipStream_t)stream, a, out_f64);
     return hipGetLastError() == hipSuccess ? CONV2D_OK : CONV2D_ERR_HIP;
+}
+
+// ---- F(4x4,3x3) form stage entry points ----------------------------------------------------
+
+static int f43_prepare(const problem_t* pb, Geometry* g) {
+    if (!pb) return CONV2D_ERR_NULL;
+    int rc = derive_geometry(pb, g);
+    if (rc) return rc;
+    return g->f43_ok ? CONV2D_OK : CONV2D_ERR_UNSUPPORTED;
+}
+
+int conv2d_f43_prepare_weights(const problem_t* pb, void* stream) {
+    int rc = check_ptrs(pb, false, true, false, false, true);
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = f43_prepare(pb, &g))) return rc;
+    return prepare_weights_form(pb, g, CONV2D_FORM_F43, (hipStream_t)stream);
+}
+
+int conv2d_f43_input_transform(const problem_t* pb, void* stream) {
+    int rc = check_ptrs(pb, true, false, false, true, false);
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = f43_prepare(pb, &g))) return rc;
+    mykernelParamType a;
+    fill_param43(pb, g, 0, &a);
+    return launch(slot_input43(pb, g), &a, (hipStream_t)stream);
+}
+
+int conv2d_f43_gemm(const problem_t* pb, int cfg, void* stream) {
+    int rc = check_ptrs(pb, false, false, false, true, true);
+    if (rc) return rc;
+    if (!pb->mid_result) return CONV2D_ERR_NULL;
+    Geometry g;
+    if ((rc = f43_prepare(pb, &g))) return rc;
+    if (cfg < 0) cfg = select_config43(pb, g);
+    if (cfg >= kNumPosConfigs) return CONV2D_ERR_INVALID;
+    mykernelParamType a;
+    fill_param43(pb, g, cfg, &a);
+    return launch(slot_gemm43(g, cfg), &a, (hipStream_t)stream);
+}
+
+int conv2d_f43_output_transform(const problem_t* pb, void* stream) {
+    int rc = check_ptrs(pb, false, false, true, false, false);
+    if (rc) return rc;
+    if (!pb->mid_result) return CONV2D_ERR_NULL;
+    Geometry g;
+    if ((rc = f43_prepare(pb, &g))) return rc;
+    mykernelParamType a;
+    fill_param43(pb, g, 0, &a);
+    return launch(slot_output43(g, (int)pb->k), &a, (hipStream_t)stream);
+}
+
+int conv2d_f43_forward(const problem_t* pb, int cfg, void* stream) {
+    int rc = check_ptrs(pb, true, false, true, true, true);
+    if (rc) return rc;
+    if (!pb->mid_result) return CONV2D_ERR_NULL;
+    Geometry g;
+    if ((rc = f43_prepare(pb, &g))) return rc;
+    if (cfg < 0) cfg = select_config43(pb, g);
+    if (cfg >= kNumPosConfigs) return CONV2D_ERR_INVALID;
+    mykernelParamType a;
+    fill_param43(pb, g, cfg, &a);
+    return forward43(pb, g, cfg, &a, (hipStream_t)stream);
+}
+
+int conv2d_f43_num_tile_configs(void) { return kNumPosConfigs; }
+
+int conv2d_f43_tile_config(int cfg, int* bk, int* bt, int* threads) {
+    if (cfg < 0 || cfg >= kNumPosConfigs) return CONV2D_ERR_INVALID;
+    if (bk) *bk = kPosConfigs[cfg].bk;
+    if (bt) *bt = kPosConfigs[cfg].bt;
+    if (threads) *threads = kPosConfigs[cfg].threads;
+    return CONV2D_OK;
+}
+
+int conv2d_f43_select_tile_config(const problem_t* pb, int* cfg) {
+    if (!cfg) return CONV2D_ERR_NULL;
+    Geometry g;
+    int rc = f43_prepare(pb, &g);
+    if (rc) return rc;
+    *cfg = select_config43(pb, g);
+    return CONV2D_OK;
 }
 
 // ---- FP8 compensated form ----------------------------------------------------------------

@@ -26,6 +26,12 @@ struct Geometry {
     int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
     long long fitems;
     int io;  // CONV2D_IO_* tensor layouts and output type
+    // F(4x4,3x3) form: 4x4 output tiles (t4h x t4w per image), input-transform band of band4
+    // tile rows (4*band4 + 2 input rows in LDS); f43_ok: the band fits
+    bool f43_ok;
+    int t4h, t4w, band4;
+    long long T4, NT4, G32_4;
+    size_t v43_bytes, u43_bytes, m43_bytes;
 };
 
 struct Launch {
@@ -65,6 +71,18 @@ int autotune_subset(const problem_t* pb, int reps, void* stream, const int* cfgs
                     float* times_us);
 // Whether getkernelInfo runs this Winograd-path problem in the fused form.
 bool use_fused(const problem_t* pb, const Geometry& g);
+// The form getkernelInfo and conv2d_forward use: CONV2D_FORM_F43 for a shape the tuned table
+// records as such (when the F(4x4) form applies and mid_result is set), else fused or staged.
+int select_form(const problem_t* pb, const Geometry& g);
+// F(4x4,3x3) form: kernel argument, configuration choice (kPosConfigs) and launch slots.
+void fill_param43(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a);
+int select_config43(const problem_t* pb, const Geometry& g);
+Launch slot_filter43(const Geometry& g);
+Launch slot_input43(const problem_t* pb, const Geometry& g);
+Launch slot_gemm43(const Geometry& g, int cfg);
+Launch slot_output43(const Geometry& g, int k);
+// The filter transform of `form` (F(2x2) for staged and fused, F(4x4) for CONV2D_FORM_F43).
+int prepare_weights_form(const problem_t* pb, const Geometry& g, int form, hipStream_t st);
 // Fastest of reps launches of L with argument a on stream st, in microseconds (one warm-up).
 int time_launch_us(const Launch& L, mykernelParamType* a, int reps, hipStream_t st, float* us);
 

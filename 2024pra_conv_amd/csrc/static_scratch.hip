@@ -1,7 +1,7 @@
 // Allocation-free forward pass: conv2d_forward with the transformed input (V on the Winograd
-// path's staged form, the im2col input on the general-geometry path) in a library-owned
-// device array instead of the caller's transform_pin, for callers that cannot allocate
-// (code under graph capture, tight serving loops). No counterpart in the reference, whose
+// path's staged and F(4x4) forms, the im2col input on the general-geometry path) in a
+// library-owned device array instead of the caller's transform_pin, for callers that cannot
+// allocate (code under graph capture, tight serving loops). No counterpart in the reference, whose
 // harness allocates every workspace itself (main.cpp:33-38).
 //
 // The array `scratch` is a file-scope __device__ object: every GPU the library runs on holds
@@ -58,11 +58,25 @@ int conv2d_forward_static(const problem_t* pb, int cfg, void* stream) {
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     mykernelParamType a;
-    if (cfg < 0 && wino::use_fused(pb, g)) {
+    const int form = cfg < 0 ? wino::select_form(pb, g) : CONV2D_FORM_STAGED;
+    if (form == CONV2D_FORM_FUSED) {
         // the fused form keeps V on chip: nothing beyond the inputs and outputs
         wino::fill_param(pb, g, wino::select_config(pb, g), &a);
         a.form = CONV2D_FORM_FUSED;
         return wino::launch(wino::slot_fused(g), &a, st);
+    }
+    if (form == CONV2D_FORM_F43) {
+        // chosen only when the caller passes mid_result, which keeps M; V goes to the array
+        if (g.v43_bytes > kStaticScratchBytes) return CONV2D_ERR_WORKSPACE;
+        void* v = nullptr;
+        if ((rc = scratch_address(&v))) return rc;
+        problem_t q = *pb;
+        q.transform_pin = (conv2d_half_t*)v;
+        const int c43 = wino::select_config43(pb, g);
+        wino::fill_param43(&q, g, c43, &a);
+        if ((rc = wino::launch(wino::slot_input43(&q, g), &a, st))) return rc;
+        if ((rc = wino::launch(wino::slot_gemm43(g, c43), &a, st))) return rc;
+        return wino::launch(wino::slot_output43(g, (int)pb->k), &a, st);
     }
     if (!g.direct && !g.staged_ok) return CONV2D_ERR_UNSUPPORTED;
     if (cfg < 0) cfg = wino::select_config(pb, g);

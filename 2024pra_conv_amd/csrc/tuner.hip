@@ -16,7 +16,7 @@ namespace {
 struct TunedEntry {
     unsigned n, c, h, w, k, p, q;
     int cfg;
-    int form;  // CONV2D_FORM_STAGED or CONV2D_FORM_FUSED
+    int form;  // CONV2D_FORM_STAGED, CONV2D_FORM_FUSED or CONV2D_FORM_F43 (cfg then indexes its table)
 };
 
 std::mutex g_mu;
@@ -49,11 +49,16 @@ int tuned_form_lookup(const problem_t* pb) {
 
 extern "C" {
 
-// Records (or replaces) the form and staged configuration for one exact shape.
+// Whether cfg indexes the GEMM table of `form` (the F(4x4) form has its own).
+static bool valid_entry(int cfg, int form) {
+    if (form == CONV2D_FORM_F43) return cfg >= 0 && cfg < kNumPosConfigs;
+    return (form == CONV2D_FORM_STAGED || form == CONV2D_FORM_FUSED) && cfg >= 0 && cfg < kNumGemmConfigs;
+}
+
+// Records (or replaces) the form and its GEMM configuration for one exact shape.
 int conv2d_tuned_set_form(const problem_t* pb, int cfg, int form) {
     if (!pb) return CONV2D_ERR_NULL;
-    if (cfg < 0 || cfg >= kNumGemmConfigs) return CONV2D_ERR_INVALID;
-    if (form != CONV2D_FORM_STAGED && form != CONV2D_FORM_FUSED) return CONV2D_ERR_INVALID;
+    if (!valid_entry(cfg, form)) return CONV2D_ERR_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto& e : g_table)
         if (same_shape(e, pb)) {
@@ -80,7 +85,7 @@ int conv2d_tuned_count(void) {
     return (int)g_table.size();
 }
 
-// Text format, one shape per line: "n c h w k p q cfg form" (form 0 staged, 1 fused; files
+// Text format, one shape per line: "n c h w k p q cfg form" (form 0 staged, 1 fused, 2 F(4x4); files
 // without the form column load as staged). Lines starting with '#' are comments.
 int conv2d_tuned_save(const char* path) {
     if (!path) return CONV2D_ERR_NULL;
@@ -107,8 +112,7 @@ int conv2d_tuned_load(const char* path) {
         e.form = CONV2D_FORM_STAGED;
         const int got = sscanf(line, "%u %u %u %u %u %u %u %d %d", &e.n, &e.c, &e.h, &e.w, &e.k, &e.p,
                                &e.q, &e.cfg, &e.form);
-        if ((got != 8 && got != 9) || e.cfg < 0 || e.cfg >= kNumGemmConfigs ||
-            (e.form != CONV2D_FORM_STAGED && e.form != CONV2D_FORM_FUSED)) {
+        if ((got != 8 && got != 9) || !valid_entry(e.cfg, e.form)) {
             rc = CONV2D_ERR_INVALID;
             break;
         }
@@ -264,25 +268,17 @@ int autotune_subset(const problem_t* pb, int reps, void* stream, const int* cfgs
 
 }  // namespace wino
 
-// Times the fused kernel on the problem (the output it writes is gated against the staged
-// output in `golden`: max |fused - staged| <= 1e-2 * max |staged|, the library's accuracy bound,
-// since the fused form's FP16 input transform rounds differently and is not bit-identical).
-// *us = best of reps, or -1 when the fused form does not apply or fails the gate.
-static int time_fused(const problem_t* pb, const wino::Geometry& g, int reps, hipStream_t st,
-                      const uint16_t* golden, float* us) {
-    *us = -1.f;
-    if (!g.fused_ok) return CONV2D_OK;
-    mykernelParamType a;
-    wino::fill_param(pb, g, 0, &a);
-    a.form = CONV2D_FORM_FUSED;
-    float t = 0.f;
-    int rc = wino::time_launch_us(wino::slot_fused(g), &a, reps, st, &t);
-    if (rc) return rc;
+// Whether the output the problem's out holds is within the library's accuracy bound of the
+// staged output in `golden`: max |out - staged| <= 1e-2 * max |staged| (the fused and F(4x4)
+// forms round differently from the staged form and are not bit-identical to it).
+static int gate_against(const problem_t* pb, const wino::Geometry& g, hipStream_t st, const uint16_t* golden,
+                        bool* pass) {
+    *pass = false;
     const size_t nout = (size_t)pb->n * pb->k * g.Oh * g.Ow;
     unsigned* d2 = nullptr;
     unsigned h2[2] = {0u, 0u};
     if (hipMalloc(&d2, 2 * sizeof(unsigned)) != hipSuccess) return CONV2D_ERR_HIP;
-    rc = CONV2D_OK;
+    int rc = CONV2D_OK;
     if (hipMemsetAsync(d2, 0, 2 * sizeof(unsigned), st) != hipSuccess) rc = CONV2D_ERR_HIP;
     if (!rc) {
         hipLaunchKernelGGL(tuner_max_abs_diff, dim3(1024), dim3(256), 0, st, (const _Float16*)pb->out,
@@ -296,7 +292,88 @@ static int time_fused(const problem_t* pb, const wino::Geometry& g, int reps, hi
     float dmax, ymax;
     memcpy(&dmax, &h2[0], 4);
     memcpy(&ymax, &h2[1], 4);
-    if (dmax <= 1e-2f * ymax) *us = t;
+    *pass = dmax <= 1e-2f * ymax;
+    return CONV2D_OK;
+}
+
+// Times the fused kernel on the problem, gated against the staged output in `golden`.
+// *us = best of reps, or -1 when the fused form does not apply or fails the gate.
+static int time_fused(const problem_t* pb, const wino::Geometry& g, int reps, hipStream_t st,
+                      const uint16_t* golden, float* us) {
+    *us = -1.f;
+    if (!g.fused_ok) return CONV2D_OK;
+    mykernelParamType a;
+    wino::fill_param(pb, g, 0, &a);
+    a.form = CONV2D_FORM_FUSED;
+    float t = 0.f;
+    int rc = wino::time_launch_us(wino::slot_fused(g), &a, reps, st, &t);
+    if (rc) return rc;
+    bool pass = false;
+    if ((rc = gate_against(pb, g, st, golden, &pass))) return rc;
+    if (pass) *us = t;
+    return CONV2D_OK;
+}
+
+// Times the F(4x4) form: its filter transform (into transform_filter), input transform, every
+// GEMM configuration (each bit-identical to the first, as all configurations must be) and
+// output transform; the output is gated against the staged output in `golden`. *us = input
+// transform + best GEMM + output transform, or -1 when the form does not apply (or mid_result
+// is not set) or fails the gate; *best_cfg its fastest GEMM configuration.
+static int time_f43(const problem_t* pb, const wino::Geometry& g, int reps, hipStream_t st, const uint16_t* golden,
+                    float* us, int* best_cfg) {
+    *us = -1.f;
+    *best_cfg = -1;
+    if (!g.f43_ok || !pb->mid_result) return CONV2D_OK;
+    mykernelParamType a;
+    wino::fill_param43(pb, g, 0, &a);
+    int rc = wino::prepare_weights_form(pb, g, CONV2D_FORM_F43, st);
+    float t_in = 0.f, t_out = 0.f;
+    if (!rc) rc = wino::time_launch_us(wino::slot_input43(pb, g), &a, reps, st, &t_in);
+    if (rc) return rc;
+    const size_t mhalves = g.m43_bytes / 2;
+    uint16_t* first = nullptr;
+    unsigned* count = nullptr;
+    if (hipMalloc(&first, g.m43_bytes) != hipSuccess || hipMalloc(&count, sizeof(unsigned)) != hipSuccess) {
+        (void)hipFree(first);
+        return CONV2D_ERR_HIP;
+    }
+    float best_t = -1.f;
+    for (int cfg = 0; cfg < kNumPosConfigs && !rc; ++cfg) {
+        wino::fill_param43(pb, g, cfg, &a);
+        float t = 0.f;
+        if ((rc = wino::time_launch_us(wino::slot_gemm43(g, cfg), &a, reps, st, &t))) break;
+        bool same = true;
+        if (cfg == 0) {
+            if (hipMemcpyAsync(first, pb->mid_result, g.m43_bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+                rc = CONV2D_ERR_HIP;
+        } else {
+            unsigned h = 0;
+            if (hipMemsetAsync(count, 0, sizeof(unsigned), st) != hipSuccess) rc = CONV2D_ERR_HIP;
+            if (!rc) {
+                hipLaunchKernelGGL(tuner_count_mismatch, dim3(1024), dim3(256), 0, st, (const uint16_t*)first,
+                                   (const uint16_t*)pb->mid_result, mhalves, count);
+                if (hipMemcpyAsync(&h, count, sizeof h, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess)
+                    rc = CONV2D_ERR_HIP;
+            }
+            same = h == 0;
+        }
+        if (!rc && same && (best_t < 0.f || t < best_t)) {
+            best_t = t;
+            *best_cfg = cfg;
+        }
+    }
+    (void)hipFree(first);
+    (void)hipFree(count);
+    if (rc) return rc;
+    if (*best_cfg < 0) return CONV2D_OK;
+    // M of the best configuration, then the output transform (timed) and the gate
+    wino::fill_param43(pb, g, *best_cfg, &a);
+    if ((rc = wino::launch(wino::slot_gemm43(g, *best_cfg), &a, st))) return rc;
+    if ((rc = wino::time_launch_us(wino::slot_output43(g, (int)pb->k), &a, reps, st, &t_out))) return rc;
+    bool pass = false;
+    if ((rc = gate_against(pb, g, st, golden, &pass))) return rc;
+    if (pass) *us = t_in + best_t + t_out;
     return CONV2D_OK;
 }
 
@@ -317,11 +394,13 @@ extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int*
     if (hipMalloc(&golden, nout * 2) != hipSuccess) return CONV2D_ERR_HIP;
     mykernelParamType a;
     wino::fill_param(pb, g, best, &a);
-    float t_s2 = -1.f, t_fused = -1.f;
+    float t_s2 = -1.f, t_fused = -1.f, t_f43 = -1.f;
+    int cfg43 = -1;
     rc = wino::launch(wino::slot_gemm(g, best), &a, st);
     if (!rc && hipMemcpyAsync(golden, pb->out, nout * 2, hipMemcpyDeviceToDevice, st) != hipSuccess) rc = CONV2D_ERR_HIP;
     if (!rc) rc = wino::time_launch_us(wino::slot_input(pb, g), &a, reps, st, &t_s2);
     if (!rc) rc = time_fused(pb, g, reps, st, golden, &t_fused);
+    if (!rc) rc = time_f43(pb, g, reps, st, golden, &t_f43, &cfg43);
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = CONV2D_ERR_HIP;
     (void)hipFree(golden);
     if (rc) return rc;
@@ -329,9 +408,25 @@ extern "C" int conv2d_autotune(const problem_t* pb, int reps, void* stream, int*
         for (int i = 0; i < ntimes && i < kNumGemmConfigs; ++i) times_us[i] = t[i];
         if (ntimes > kNumGemmConfigs) times_us[kNumGemmConfigs] = t_s2;
         if (ntimes > kNumGemmConfigs + 1) times_us[kNumGemmConfigs + 1] = t_fused;
+        if (ntimes > kNumGemmConfigs + 2) times_us[kNumGemmConfigs + 2] = t_f43;
     }
     if (best_cfg) *best_cfg = best;
-    // the fused form replaces the input transform and the GEMM
-    const bool fused = t_fused > 0.f && t_fused < t_s2 + t[best];
-    return conv2d_tuned_set_form(pb, best, fused ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED);
+    // the fastest of: input transform + best GEMM (staged), the fused kernel, the F(4x4) form
+    int form = CONV2D_FORM_STAGED, cfg = best;
+    float t_best = t_s2 + t[best];
+    if (t_fused > 0.f && t_fused < t_best) {
+        form = CONV2D_FORM_FUSED;
+        t_best = t_fused;
+    }
+    if (t_f43 > 0.f && t_f43 < t_best) {
+        form = CONV2D_FORM_F43;
+        cfg = cfg43;
+    }
+    if ((rc = conv2d_tuned_set_form(pb, cfg, form))) return rc;
+    // transform_filter holds the recorded form's U (the F(4x4) timing overwrote it)
+    if (g.f43_ok && pb->mid_result) {
+        if ((rc = wino::prepare_weights_form(pb, g, form, st))) return rc;
+        if (hipStreamSynchronize(st) != hipSuccess) return CONV2D_ERR_HIP;
+    }
+    return CONV2D_OK;
 }

@@ -8,7 +8,8 @@
 // File layout (little-endian, 64-byte header, then the payload):
 //   0  char[8]  magic "WINOU16\0"
 //   8  u32      layout version (CONV2D_U_LAYOUT_VERSION)
-//  12  u32      method (CONV2D_METHOD_WINOGRAD or CONV2D_METHOD_DIRECT: different layouts)
+//  12  u32      U layout: 0 Winograd F(2x2) (CONV2D_METHOD_WINOGRAD), 1 direct path
+//               (CONV2D_METHOD_DIRECT), 2 Winograd F(4x4) (the problem's form is CONV2D_FORM_F43)
 //  16  u32 x4   k, c, r, s
 //  32  u64      FNV-1a 64 of the K*C*R*S FP16 weights
 //  40  u64      payload bytes (the problem's U size)
@@ -26,6 +27,7 @@
 namespace {
 
 constexpr char kMagic[8] = {'W', 'I', 'N', 'O', 'U', '1', '6', '\0'};
+constexpr uint32_t kLayoutF43 = 2;  // header layout value of the F(4x4) U
 
 struct Header {
     char magic[8];
@@ -60,13 +62,14 @@ int problem_header(const problem_t* pb, Header* h, size_t* ubytes) {
     memset(h, 0, sizeof(*h));
     memcpy(h->magic, kMagic, sizeof(kMagic));
     h->version = CONV2D_U_LAYOUT_VERSION;
-    h->method = g.direct ? CONV2D_METHOD_DIRECT : CONV2D_METHOD_WINOGRAD;
+    const bool f43 = wino::select_form(pb, g) == CONV2D_FORM_F43;
+    h->method = g.direct ? CONV2D_METHOD_DIRECT : (f43 ? kLayoutF43 : CONV2D_METHOD_WINOGRAD);
     h->k = pb->k;
     h->c = pb->c;
     h->r = pb->r;
     h->s = pb->s;
-    h->payload_bytes = g.u_bytes;
-    *ubytes = g.u_bytes;
+    h->payload_bytes = f43 ? g.u43_bytes : g.u_bytes;
+    *ubytes = h->payload_bytes;
     return CONV2D_OK;
 }
 

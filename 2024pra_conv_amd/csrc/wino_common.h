@@ -30,6 +30,25 @@ __device__ inline uint32_t pack_bf16x2(float a, float b) {
 __device__ inline uint32_t pack_out2(bool bf16, float a, float b) {
     return bf16 ? pack_bf16x2(a, b) : pack_f16x2(a, b);
 }
+// FP64 -> FP16 rounded once to nearest even: FP32 rounded to odd (an inexact result gets an
+// odd last bit, moving away from the FP64 value's side it was rounded past), then the FP32 ->
+// FP16 conversion; FP32 keeps 13 more bits than FP16, so the two steps equal one correct
+// rounding. A zero comes out as +0 whatever its sign (d + 0.0), so exact transforms that sum
+// to zero do not depend on the operation order.
+__device__ inline half_t f64_to_f16(double d) {
+    d = d + 0.0;
+    float r = (float)d;
+    const double back = (double)r;
+    if (back != d) {
+        uint32_t b = __builtin_bit_cast(uint32_t, r);
+        if ((b & 1u) == 0u) {
+            const bool away = (d > back) == !(b >> 31);  // |d| > |r|
+            b = away ? b + 1u : b - 1u;
+            r = __builtin_bit_cast(float, b);
+        }
+    }
+    return (half_t)r;
+}
 // A compile-time flag passed to generic lambdas (one instantiation per output type).
 template <bool B>
 struct BoolTag {
@@ -49,6 +68,10 @@ __host__ __device__ inline size_t frag_offset(int pos, int cb, int blk, int CB, 
 __global__ void wino_filter_transform(mykernelParamType prm);
 __global__ void wino_input_transform(mykernelParamType prm);
 __global__ void wino_noop(mykernelParamType prm);
+// F(4x4,3x3) form: filter, input and output transforms (S3: wino_pos_*kernel)
+__global__ void wino43_filter_transform(mykernelParamType prm);
+__global__ void wino43_input_transform(mykernelParamType prm);
+__global__ void wino43_output_transform(mykernelParamType prm);
 __global__ void conv_direct_f64_kernel(mykernelParamType prm, double* out);
 // general-geometry path: packing kernels of the implicit-GEMM direct convolution
 __global__ void direct_pack_weights(mykernelParamType prm);
@@ -103,6 +126,18 @@ __host__ __device__ inline int input_lds_pitch(int tiles_w, int W, int q) {
     return (m + 1) & ~1;
 }
 __host__ __device__ inline int input_lds_rows(int band) { return 2 * band + 2; }
+// F(4x4,3x3) input staging: tile tw's 6-column window starts at LDS column 4*tw + 8, a band of
+// `band` tile rows needs 4*band + 2 input rows.
+__host__ __device__ inline int input43_lds_pitch(int tiles_w, int W, int q) {
+    int a = 4 * tiles_w + 10;
+    int b = W + q + 8;
+    int m = a > b ? a : b;
+    return (m + 1) & ~1;
+}
+__host__ __device__ inline int input43_lds_rows(int band) { return 4 * band + 2; }
+__host__ __device__ inline size_t input43_lds_bytes(int band, int tiles_w, int W, int q) {
+    return (size_t)32 * input43_lds_rows(band) * input43_lds_pitch(tiles_w, W, q) * sizeof(uint16_t);
+}
 __host__ __device__ inline size_t input_lds_bytes(int band, int tiles_w, int W, int q) {
     return (size_t)32 * input_lds_rows(band) * input_lds_pitch(tiles_w, W, q) * sizeof(uint16_t);
 }

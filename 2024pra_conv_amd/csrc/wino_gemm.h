@@ -13,6 +13,11 @@ struct GemmConfig {
 extern const GemmConfig kGemmConfigs[];
 extern const int kNumGemmConfigs;
 
+// Per-position GEMM of the F(4x4,3x3) form (36 positions, M to mid_result): same fields, no
+// grouped kernels.
+extern const GemmConfig kPosConfigs[];
+extern const int kNumPosConfigs;
+
 // Implicit-GEMM direct convolution (general-geometry path): same fields, no grouped kernels.
 extern const GemmConfig kDirectConfigs[];
 extern const int kNumDirectConfigs;

@@ -36,6 +36,13 @@ namespace {
 // KS 16-channel substeps per pipeline step (one v_mfma_f32_32x32x16_f16 k-step each).
 constexpr int KS = 2;
 
+// What a GEMM body computes:
+//   kGemmWino   the F(2x2) GEMM: 16 positions, fold per position, 2x2-tile epilogue;
+//   kGemmDirect the implicit-GEMM direct convolution of the general-geometry path;
+//   kGemmPos    one position `fpos` of the F(4x4) form (36 positions): CB steps of its packed
+//               U and V, M stored in FP16 to mid_result (store_m16).
+enum { kGemmWino = 0, kGemmDirect = 1, kGemmPos = 2 };
+
 template <int BK, int BT, int WK, int WT>
 struct GemmShape {
     static constexpr int kThreads = WK * WT * 64;
@@ -56,12 +63,13 @@ struct GemmShape {
 // One GEMM work item (k-block kblk x column block gblk). smem holds 2 stages. Ends after
 // the epilogue stores; callers reusing smem for another item must __syncthreads() first.
 // Stage layout: substep kk = 0..KS-1, each [UF U fragments][VF V fragments].
-// DIRECT = false: the Winograd GEMM (16 positions, fold per position, 2x2-tile epilogue).
-// DIRECT = true: the implicit-GEMM direct convolution of the general-geometry path: the
+// MODE kGemmWino: the Winograd GEMM (16 positions, fold per position, 2x2-tile epilogue).
+// MODE kGemmDirect: the implicit-GEMM direct convolution of the general-geometry path: the
 // steps run over prm.nsteps chunks of 32 values of the flattened (filter tap, channel)
 // reduction index into one accumulator, whose elements are the outputs themselves.
-template <int BK, int BT, int WK, int WT, int D, bool DIRECT = false>
-__device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
+// MODE kGemmPos: position fpos of the F(4x4) form (CB steps), M stored by store_m16.
+template <int BK, int BT, int WK, int WT, int D, int MODE = kGemmWino>
+__device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem, int fpos = 0) {
     using S = GemmShape<BK, BT, WK, WT>;
     constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF, NL = S::kLoads;
     static_assert(8 % D == 0 && D % 2 == 0, "ring depth: even, dividing the step count (a multiple of 8)");
@@ -74,9 +82,15 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
     const int gb0 = gblk * VF;  // first 32-column block of V
     const int kb0 = kblk * UF;  // first 32-row block of U
-    const int nsteps = DIRECT ? prm.nsteps : 16 * CB;  // 32-value reduction chunks
+    // 32-value reduction chunks (a multiple of 8 except per F(4x4) position, whose steps past
+    // CB issue no MFMAs)
+    const int nsteps = MODE == kGemmDirect ? prm.nsteps : (MODE == kGemmPos ? CB : 16 * CB);
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
+    if constexpr (MODE == kGemmPos) {
+        U += (size_t)fpos * 2 * CB * KB * kFragHalves;
+        V += (size_t)fpos * 2 * CB * G32 * kFragHalves;
+    }
 
     // copy slot i of this thread: piece pc = tid + i * threads of every step. Step s covers
     // 16-channel blocks 2s and 2s+1 of the [pos][c16] sequence (CB 32-channel chunks per
@@ -161,19 +175,21 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
         for (int d = 0; d < D; ++d) {
             const int s = s0 + d;
             const int cur = d & 1;  // D is even, so s & 1 == d & 1
+            if (MODE != kGemmPos || s < nsteps) {
 #pragma unroll
-            for (int kk = 0; kk < KS; ++kk)
+                for (int kk = 0; kk < KS; ++kk)
 #pragma unroll
-                for (int mi = 0; mi < MI; ++mi)
+                    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-                    for (int ni = 0; ni < NI; ++ni)
-                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[cur][kk][mi], rb[cur][kk][ni],
-                                                                             acc[mi][ni], 0, 0, 0);
+                        for (int ni = 0; ni < NI; ++ni)
+                            acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[cur][kk][mi], rb[cur][kk][ni],
+                                                                                 acc[mi][ni], 0, 0, 0);
+            }
             lread(ra[cur ^ 1], rb[cur ^ 1], cur ^ 1);
             lstore(ring[(d + 2) % D], cur);
             gload(ring[(d + 2) % D], s + 2 + D);
             __syncthreads();
-            if constexpr (!DIRECT) {
+            if constexpr (MODE == kGemmWino) {
                 if (++cb == CB) {
                     fold_position(pos, acc, Y);
 #pragma unroll
@@ -189,8 +205,10 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
         }
     }
 
-    if constexpr (DIRECT)
+    if constexpr (MODE == kGemmDirect)
         store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
+    else if constexpr (MODE == kGemmPos)
+        store_m16<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane, fpos);
     else
         store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
@@ -226,8 +244,8 @@ struct DmaShape {
 
 }  // namespace
 
-template <int BK, int BT, int WK, int WT, int S, bool DIRECT = false>
-__device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
+template <int BK, int BT, int WK, int WT, int S, int MODE = kGemmWino>
+__device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem, int fpos = 0) {
     using Sh = DmaShape<BK, BT, WK, WT, S>;
     constexpr int MI = Sh::MI, NI = Sh::NI, UF = Sh::UF, VF = Sh::VF, FPW = Sh::kFPW;
     const int tid = threadIdx.x;
@@ -238,9 +256,13 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
     const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
     const int gb0 = gblk * VF;
     const int kb0 = kblk * UF;
-    const int nsteps = DIRECT ? prm.nsteps : 16 * CB;
+    const int nsteps = MODE == kGemmDirect ? prm.nsteps : (MODE == kGemmPos ? CB : 16 * CB);
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
+    if constexpr (MODE == kGemmPos) {
+        U += (size_t)fpos * 2 * CB * KB * kFragHalves;
+        V += (size_t)fpos * 2 * CB * G32 * kFragHalves;
+    }
 
     // this wave's DMA slots: fragments wave*FPW .. wave*FPW+FPW-1 of the stage layout
     // [kk][UF U fragments][VF V fragments]; beyond-the-matrix fragments read fragment 0
@@ -318,15 +340,17 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
             __syncthreads();
             dma(s + S - 1, (d + S - 1) % S);
             lread(ra[cur ^ 1], rb[cur ^ 1], (d + 1) % S);
+            if (MODE != kGemmPos || s < nsteps) {
 #pragma unroll
-            for (int kk = 0; kk < KS; ++kk)
+                for (int kk = 0; kk < KS; ++kk)
 #pragma unroll
-                for (int mi = 0; mi < MI; ++mi)
+                    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-                    for (int ni = 0; ni < NI; ++ni)
-                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[cur][kk][mi], rb[cur][kk][ni],
-                                                                             acc[mi][ni], 0, 0, 0);
-            if constexpr (!DIRECT) {
+                        for (int ni = 0; ni < NI; ++ni)
+                            acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[cur][kk][mi], rb[cur][kk][ni],
+                                                                                 acc[mi][ni], 0, 0, 0);
+            }
+            if constexpr (MODE == kGemmWino) {
                 if (++cb == CB) {
                     fold_position(pos, acc, Y);
 #pragma unroll
@@ -343,8 +367,10 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
     }
     wait_vm<0>();  // the trailing (re-read) DMAs land before the workgroup's LDS is released
 
-    if constexpr (DIRECT)
+    if constexpr (MODE == kGemmDirect)
         store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
+    else if constexpr (MODE == kGemmPos)
+        store_m16<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane, fpos);
     else
         store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
@@ -457,7 +483,7 @@ const GemmConfig kGemmConfigs[] = {
 const int kNumGemmConfigs = (int)(sizeof(kGemmConfigs) / sizeof(kGemmConfigs[0]));
 
 // ---- direct (implicit-GEMM) form of the general-geometry path --------------------------------
-// The same two pipeline bodies with DIRECT = true: U holds the packed filter taps, V the
+// The same two pipeline bodies in kGemmDirect mode: U holds the packed filter taps, V the
 // packed im2col input, and the steps run over (tap, channel chunk) pairs.
 
 template <int BK, int BT, int WK, int WT, int D>
@@ -469,7 +495,7 @@ wino_direct_kernel(mykernelParamType prm) {
     const int ng = (prm.G32 + S::VF - 1) / S::VF;
     int kblk, gblk;
     xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
-    gemm_body<BK, BT, WK, WT, D, true>(prm, kblk, gblk, smem);
+    gemm_body<BK, BT, WK, WT, D, kGemmDirect>(prm, kblk, gblk, smem);
 }
 
 template <int BK, int BT, int WK, int WT, int S>
@@ -481,7 +507,7 @@ wino_direct_dma_kernel(mykernelParamType prm) {
     const int ng = (prm.G32 + Sh::VF - 1) / Sh::VF;
     int kblk, gblk;
     xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
-    gemm_dma_body<BK, BT, WK, WT, S, true>(prm, kblk, gblk, smem);
+    gemm_dma_body<BK, BT, WK, WT, S, kGemmDirect>(prm, kblk, gblk, smem);
 }
 
 #define DIRECT_CFG(BK, BT, WK, WT, D) \
@@ -496,3 +522,49 @@ const GemmConfig kDirectConfigs[] = {
     DIRECT_DMA_CFG(128, 64, 2, 2, 4),   // 3: LDS-DMA, 4-stage ring, wave 64x32, 4 waves
 };
 const int kNumDirectConfigs = (int)(sizeof(kDirectConfigs) / sizeof(kDirectConfigs[0]));
+
+// ---- per-position GEMM of the F(4x4,3x3) form -----------------------------------------------
+// One workgroup per (k-block, column block, position): the position's CB steps, M to
+// mid_result. Without the fold's output accumulators a wave holds only its MFMA accumulators,
+// so 64 x 64 wave tiles keep two waves per SIMD.
+
+template <int BK, int BT, int WK, int WT, int D>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(2)))
+wino_pos_kernel(mykernelParamType prm) {
+    using S = GemmShape<BK, BT, WK, WT>;
+    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
+    const int nk = (prm.KB32 + S::UF - 1) / S::UF;
+    const int ng = (prm.G32 + S::VF - 1) / S::VF;
+    int kblk, rest;
+    xcd_remap(blockIdx.x, nk * ng * 36, nk, &kblk, &rest);
+    gemm_body<BK, BT, WK, WT, D, kGemmPos>(prm, kblk, rest % ng, smem, rest / ng);
+}
+
+template <int BK, int BT, int WK, int WT, int S>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(2)))
+wino_pos_dma_kernel(mykernelParamType prm) {
+    using Sh = DmaShape<BK, BT, WK, WT, S>;
+    __shared__ __attribute__((aligned(16))) half_t smem[Sh::kSmemHalves];
+    const int nk = (prm.KB32 + Sh::UF - 1) / Sh::UF;
+    const int ng = (prm.G32 + Sh::VF - 1) / Sh::VF;
+    int kblk, rest;
+    xcd_remap(blockIdx.x, nk * ng * 36, nk, &kblk, &rest);
+    gemm_dma_body<BK, BT, WK, WT, S, kGemmPos>(prm, kblk, rest % ng, smem, rest / ng);
+}
+
+#define POS_CFG(BK, BT, WK, WT, D) \
+    { BK, BT, WK * WT * 64, D, (const void*)&wino_pos_kernel<BK, BT, WK, WT, D>, nullptr }
+#define POS_DMA_CFG(BK, BT, WK, WT, S) \
+    { BK, BT, WK * WT * 64, S, (const void*)&wino_pos_dma_kernel<BK, BT, WK, WT, S>, nullptr }
+
+const GemmConfig kPosConfigs[] = {
+    POS_DMA_CFG(128, 128, 2, 4, 8),  // 0: LDS-DMA, 8-stage ring, wave 64x32, 8 waves
+    POS_DMA_CFG(128, 64, 2, 2, 4),   // 1: LDS-DMA, 4-stage ring, wave 64x32, 4 waves
+    POS_CFG(64, 128, 1, 4, 2),       // 2: wave 64x32, 4 waves
+    POS_CFG(32, 64, 1, 2, 2),        // 3: wave 32x32, 2 waves (small problems)
+    POS_DMA_CFG(64, 128, 1, 4, 4),   // 4: LDS-DMA, 4-stage ring, wave 64x32, 4 waves
+    POS_DMA_CFG(128, 128, 2, 2, 4),  // 5: LDS-DMA, 4-stage ring, wave 64x64, 4 waves
+    POS_DMA_CFG(256, 128, 4, 2, 4),  // 6: LDS-DMA, 4-stage ring, wave 64x64, 8 waves
+    POS_DMA_CFG(128, 256, 2, 4, 4),  // 7: LDS-DMA, 4-stage ring, wave 64x64, 8 waves
+};
+const int kNumPosConfigs = (int)(sizeof(kPosConfigs) / sizeof(kPosConfigs[0]));

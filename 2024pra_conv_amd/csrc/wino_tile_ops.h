@@ -88,16 +88,14 @@ __device__ inline void stage_rows_nhwc(const half_t* in_n, int cb, int C, int H,
     }
 }
 
-// Stages the input rows of tile-row band bi (image n, 32-channel block cb) in LDS: zero fill
+// Stages LR input rows from row ih0 (image n, 32-channel block cb) in LDS as
+// [channel][row][column], input column iw at LDS column q + 8 + iw (pitch LWp): zero fill
 // (padding halo, rows outside the image, channels >= C), then coalesced row loads. Ends with
 // a barrier, so the staging is visible to every wave.
-__device__ inline void stage_input_band(const mykernelParamType& prm, int n, int cb, int bi, half_t* lds) {
-    const int TH = prm.band;
-    const int th0 = bi * TH;
+__device__ inline void stage_band(const mykernelParamType& prm, int n, int cb, int ih0, int LR, int LWp,
+                                  half_t* lds) {
     const int H = prm.h, W = prm.w, C = prm.c;
     const int qq = prm.q;
-    const int LR = input_lds_rows(TH);
-    const int LWp = input_lds_pitch(prm.tiles_w, W, qq);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -110,7 +108,6 @@ __device__ inline void stage_input_band(const mykernelParamType& prm, int n, int
     }
     __syncthreads();
     const half_t* in_n = (const half_t*)prm.pin + (size_t)n * C * H * W;
-    const int ih0 = 2 * th0 - prm.p;
     const int njobs = 32 * LR;
     if (prm.io & CONV2D_IO_IN_NHWC) {
         stage_rows_nhwc(in_n, cb, C, H, W, ih0, LR, LWp, qq, lds);
@@ -138,6 +135,13 @@ __device__ inline void stage_input_band(const mykernelParamType& prm, int n, int
         stage_rows_flat<1>(in_n, cb, C, H, W, ih0, LR, LWp, qq, lds);
     }
     __syncthreads();
+}
+
+// The input rows of 2x2-tile row band bi (tile rows bi*band ..): rows 2*th0 - p .. + 2*band + 1.
+__device__ inline void stage_input_band(const mykernelParamType& prm, int n, int cb, int bi, half_t* lds) {
+    const int th0 = bi * prm.band;
+    stage_band(prm, n, cb, 2 * th0 - prm.p, input_lds_rows(prm.band), input_lds_pitch(prm.tiles_w, prm.w, prm.q),
+               lds);
 }
 
 // ---- GEMM epilogue pieces -----------------------------------------------------------------
@@ -308,6 +312,35 @@ __device__ inline void store_plain32(const mykernelParamType& prm, const floatx1
                 }
                 o[(size_t)k * prm.T] = (half_t)y;
             }
+    }
+}
+
+// Epilogue of the F(4x4) per-position GEMM (position pos): M / 4 rounded to FP16 into
+// mid_result as [pos][k / 4][column][k % 4] (rows padded to KB32*32, columns to G32*32). The
+// four consecutive rows a lane holds per accumulator quarter are one 8-byte store, and lanes
+// 0-31 (consecutive columns) write 256 contiguous bytes.
+template <int MI, int NI>
+__device__ inline void store_m16(const mykernelParamType& prm, const floatx16 (&acc)[MI][NI], int gbw, int kbw,
+                                 int lane, int pos) {
+    typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) u32x2_t global_u32x2_t;
+    const size_t KQp = (size_t)prm.KB32 * 8, NTp = (size_t)prm.G32 * 32;
+    global_u32x2_t* M = (global_u32x2_t*)prm.mid_result;
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+        if (gbw + ni >= prm.G32) continue;
+        const size_t t = (size_t)(gbw + ni) * 32 + (lane & 31);
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi) {
+            if (kbw + mi >= prm.KB32) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const size_t kq = (size_t)(kbw + mi) * 8 + 2 * i + (lane >> 5);
+                const u32x2_t pk = {pack_f16x2(0.25f * acc[mi][ni][4 * i], 0.25f * acc[mi][ni][4 * i + 1]),
+                                    pack_f16x2(0.25f * acc[mi][ni][4 * i + 2], 0.25f * acc[mi][ni][4 * i + 3])};
+                M[((size_t)pos * KQp + kq) * NTp + t] = pk;
+            }
+        }
     }
 }
 

@@ -8,7 +8,8 @@ import ctypes
 
 import torch
 
-from . import KernelInfo, ParamType, check, fp8_workspace_size, load_library, make_problem, workspace_size
+from . import (FORM_F43, FORM_STAGED, KernelInfo, ParamType, check, form_workspace_size, fp8_workspace_size,
+               load_library, make_problem, workspace_size)
 
 
 def _stream_handle(stream: torch.cuda.Stream | None):
@@ -44,20 +45,26 @@ class WinogradConv:
         self.oh = (h - r + 2 * pad[0]) // stride[0] + 1
         self.ow = (wd - s + 2 * pad[1]) // stride[1] + 1
         probe = make_problem(*self.shape)
-        vb, ub, _ = workspace_size(probe)
-        self.v_bytes, self.u_bytes = vb, ub
+        vb, ub, mb = workspace_size(probe)  # enough for every form
+        # the F(2x2) (or direct-path) operand sizes, and the F(4x4) form's where it applies
+        self.v_bytes, self.u_bytes, _ = form_workspace_size(probe, FORM_STAGED)
+        f43 = form_workspace_size(probe, FORM_F43)
+        self.v43_bytes, self.u43_bytes, self.m43_bytes = f43 if f43 else (0, 0, 0)
         dev = x.device
         oshape = (n, self.oh, self.ow, k) if io & 2 else (n, k, self.oh, self.ow)
         odt = torch.bfloat16 if io & 4 else torch.float16
         self.out = out if out is not None else torch.empty(oshape, dtype=odt, device=dev)
         if workspace is not None:
-            self.V, self.U = workspace
+            self.V, self.U = workspace[0], workspace[1]
+            self.M = workspace[2] if len(workspace) > 2 else None
         else:
             self.V = torch.empty(max(vb, 2), dtype=torch.uint8, device=dev)
             self.U = torch.empty(max(ub, 2), dtype=torch.uint8, device=dev)
+            self.M = torch.empty(mb, dtype=torch.uint8, device=dev) if mb else None
         self.pb = make_problem(*self.shape, ptrs={
             "in_": x.data_ptr(), "weight": w.data_ptr(), "out": self.out.data_ptr(),
-            "transform_pin": self.V.data_ptr(), "transform_filter": self.U.data_ptr(), "mid_result": None})
+            "transform_pin": self.V.data_ptr(), "transform_filter": self.U.data_ptr(),
+            "mid_result": self.M.data_ptr() if self.M is not None else None})
         self._ki = None
         self._param = None
 
@@ -130,15 +137,18 @@ class WinogradConv:
         """Fused form: input transform inside the GEMM (needs prepare_weights; V unused)."""
         check(self.L.conv2d_fused_gemm(ctypes.byref(self.pb), _stream_handle(stream)), "conv2d_fused_gemm")
 
-    def autotune(self, reps: int = 5, stream=None, forms: bool = False):
+    def autotune(self, reps: int = 5, stream=None, forms: bool = False, f43: bool = False):
         """Runs the library's autotuner; returns (best_cfg, per-config best times in us, -1 = gated out)
-        and, with forms=True, also (input-transform us, fused-kernel us or -1)."""
+        and, with forms=True, also (input-transform us, fused-kernel us or -1), with f43=True also
+        the F(4x4) form's us (input transform + best GEMM + output transform) or -1."""
         n = self.L.conv2d_num_tile_configs()
-        times = (ctypes.c_float * (n + 2))()
+        times = (ctypes.c_float * (n + 3))()
         best = ctypes.c_int(-1)
         check(self.L.conv2d_autotune(ctypes.byref(self.pb), reps, _stream_handle(stream), ctypes.byref(best),
-                                     times, n + 2), "conv2d_autotune")
+                                     times, n + 3), "conv2d_autotune")
         t = list(times)
+        if f43:
+            return best.value, t[:n], t[n], t[n + 1], t[n + 2]
         return (best.value, t[:n], t[n], t[n + 1]) if forms else (best.value, t[:n])
 
     def selected_config(self) -> int:
@@ -163,12 +173,50 @@ class WinogradConv:
         """The output as N,K,Oh,Ow (a permuted copy for NHWC output)."""
         return self.out.permute(0, 3, 1, 2).contiguous() if self.io & 2 else self.out
 
+    # --- F(4x4,3x3) form stages (CONV2D_FORM_F43) -----------------------------------------
+    def f43_prepare_weights(self, stream=None) -> None:
+        check(self.L.conv2d_f43_prepare_weights(ctypes.byref(self.pb), _stream_handle(stream)),
+              "conv2d_f43_prepare_weights")
+
+    def f43_input_transform(self, stream=None) -> None:
+        check(self.L.conv2d_f43_input_transform(ctypes.byref(self.pb), _stream_handle(stream)),
+              "conv2d_f43_input_transform")
+
+    def f43_gemm(self, cfg: int = -1, stream=None) -> None:
+        check(self.L.conv2d_f43_gemm(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_f43_gemm")
+
+    def f43_output_transform(self, stream=None) -> None:
+        check(self.L.conv2d_f43_output_transform(ctypes.byref(self.pb), _stream_handle(stream)),
+              "conv2d_f43_output_transform")
+
+    def f43_forward(self, cfg: int = -1, stream=None) -> None:
+        check(self.L.conv2d_f43_forward(ctypes.byref(self.pb), cfg, _stream_handle(stream)), "conv2d_f43_forward")
+
+    def f43_selected_config(self) -> int:
+        c = ctypes.c_int()
+        check(self.L.conv2d_f43_select_tile_config(ctypes.byref(self.pb), ctypes.byref(c)),
+              "conv2d_f43_select_tile_config")
+        return c.value
+
+    def set_form(self, form: int, cfg: int) -> None:
+        """Records (form, cfg) for this shape in the library's tuned table."""
+        check(self.L.conv2d_tuned_set_form(ctypes.byref(self.pb), cfg, form), "conv2d_tuned_set_form")
+
     # --- raw workspace views --------------------------------------------------------------
     def u_halves(self) -> torch.Tensor:
         return self.U[: self.u_bytes].view(torch.float16)
 
     def v_halves(self) -> torch.Tensor:
         return self.V[: self.v_bytes].view(torch.float16)
+
+    def u43_halves(self) -> torch.Tensor:
+        return self.U[: self.u43_bytes].view(torch.float16)
+
+    def v43_halves(self) -> torch.Tensor:
+        return self.V[: self.v43_bytes].view(torch.float16)
+
+    def m43_halves(self) -> torch.Tensor:
+        return self.M[: self.m43_bytes].view(torch.float16)
 
 
 class WinogradConvFP8:

@@ -62,8 +62,10 @@ enum {
  * workspaces are caller-owned device buffers:
  *   transform_pin    : transformed input V (size from conv2d_workspace_size)
  *   transform_filter : transformed filter U (size from conv2d_workspace_size)
- *   mid_result       : unused by this implementation (the output transform is fused into
- *                      the GEMM, so the per-position products M never reach HBM); may be NULL. */
+ *   mid_result       : the F(4x4,3x3) form's per-position products M (size from
+ *                      conv2d_workspace_size); the F(2x2) forms fuse the output transform into
+ *                      the GEMM and never use it. May be NULL (the F(4x4) form is then never
+ *                      chosen). */
 typedef struct {
     conv2d_half_t* in;
     conv2d_half_t* weight;
@@ -141,12 +143,20 @@ int getkernelInfo(__in__ problem_t* problem, __out__ kernelInfo_t* kernelInfo,
 
 /* ---- Additive entry points (no counterpart in the reference) ------------------------- */
 
-/* Workspace sizes in bytes for transform_pin (V), transform_filter (U) and mid_result
- * (always 0). The reference harness sized them as 4*h*h*n*c, 16*k*c and 4*k*h*w*n halves
- * (main.cpp:33-38); those sizes suffice when h == w is even, c % 32 == 0, k % 32 == 0 and
- * n*(h/2)*(w/2) % 32 == 0, and are too small otherwise (ragged or padded shapes). */
+/* Workspace sizes in bytes for transform_pin (V), transform_filter (U) and mid_result: enough
+ * for every form the library may run the problem in. mid_result is 0 bytes unless the
+ * F(4x4,3x3) form (CONV2D_FORM_F43) applies; it is then that form's M, and transform_filter
+ * its U (2.25x the F(2x2) U). The reference harness sized them as 4*h*h*n*c, 16*k*c and
+ * 4*k*h*w*n halves (main.cpp:33-38); for the F(2x2) forms those sizes suffice when h == w is
+ * even, c % 32 == 0, k % 32 == 0 and n*(h/2)*(w/2) % 32 == 0, and are too small otherwise
+ * (ragged or padded shapes). */
 int conv2d_workspace_size(const problem_t* problem, size_t* pin_bytes,
                           size_t* filter_bytes, size_t* mid_bytes);
+/* The bytes one form uses (CONV2D_FORM_STAGED and CONV2D_FORM_FUSED: the F(2x2) V and U, or the
+ * general-geometry path's packed operands; CONV2D_FORM_F43: its V, U and M;
+ * CONV2D_ERR_UNSUPPORTED where that form does not apply). */
+int conv2d_form_workspace_size(const problem_t* problem, int form, size_t* pin_bytes, size_t* filter_bytes,
+                               size_t* mid_bytes);
 
 /* Launches the four slots in order on a HIP stream (NULL = null stream) with
  * hipExtLaunchKernel, as the reference harness does (main.cpp:105-108). */
@@ -234,14 +244,41 @@ int conv2d_weights_load(const problem_t* problem, const char* path, void* stream
 int conv2d_weight_hash(const void* host_bytes, size_t nbytes, unsigned long long* hash);
 
 /* Which form getkernelInfo and conv2d_forward(-1) use for a Winograd-path problem: the form
- * recorded for the shape in the tuned table (conv2d_autotune records the faster one it
- * measured), else CONV2D_FORM_FUSED when the fused form applies and K <= 384, or
- * N*T*K*C < 8e8, or N*T*K*C < 1.2e9 with C >= K (T = 2x2 tiles per image; where the fused
- * form measured faster on MI355X, by up to 3.2x), else CONV2D_FORM_STAGED (also for the
- * direct path). */
+ * recorded for the shape in the tuned table (conv2d_autotune records the fastest one it
+ * measured; CONV2D_FORM_F43 only where that form applies and problem->mid_result is set),
+ * else CONV2D_FORM_FUSED when the fused form applies and K <= 384, or N*T*K*C < 8e8, or
+ * N*T*K*C < 1.2e9 with C >= K (T = 2x2 tiles per image; where the fused form measured faster
+ * on MI355X, by up to 3.2x), else CONV2D_FORM_STAGED (also for the direct path). The shape
+ * heuristic never picks CONV2D_FORM_F43, whose U does not fit the reference harness's
+ * transform_filter (16*k*c halves, main.cpp:38). */
 #define CONV2D_FORM_STAGED 0
 #define CONV2D_FORM_FUSED 1
+#define CONV2D_FORM_F43 2
 int conv2d_form(const problem_t* problem, int* form);
+
+/* ---- Winograd F(4x4,3x3) form: no counterpart in the reference, which implements F(2x2,3x3)
+ * only. 36 positions per 4x4 output tile (2.25 multiplications per output instead of 4) on the
+ * interpolation points {0, 1, -1, 2, -1/2} and infinity; four slots as the reference's four
+ * stages: S1 U = G' g G'^T and S2 V = B^T d B exact in FP64 and rounded once to FP16, S3 the
+ * 36 per-position MFMA GEMMs storing FP16(M / 4) in mid_result, S4 the output transform in FP32
+ * with the epilogue and io options. Workspaces from conv2d_workspace_size (which covers this
+ * form wherever it applies); CONV2D_ERR_UNSUPPORTED for problems it does not take (the direct
+ * path, input rows too wide for its staging). Normwise error against the exact convolution
+ * modelled at 3.4e-3 (uniform data) and 3.9e-3 (normal data) on a C = 256 layer, against 4.1e-4
+ * and 5.9e-4 for the F(2x2) forms on the same data. ---- */
+/* S1 into transform_filter (what conv2d_prepare_weights runs when this form is selected). */
+int conv2d_f43_prepare_weights(const problem_t* problem, void* stream);
+/* S2 into transform_pin. */
+int conv2d_f43_input_transform(const problem_t* problem, void* stream);
+/* S3 with GEMM configuration cfg (-1: the tuned or heuristic one) into mid_result. */
+int conv2d_f43_gemm(const problem_t* problem, int cfg, void* stream);
+/* S4 from mid_result into out (N,K,Oh,Ow FP16). */
+int conv2d_f43_output_transform(const problem_t* problem, void* stream);
+/* S2 + S3 + S4 (weights prepared once). */
+int conv2d_f43_forward(const problem_t* problem, int cfg, void* stream);
+int conv2d_f43_num_tile_configs(void);
+int conv2d_f43_tile_config(int cfg, int* bk, int* bt, int* threads);
+int conv2d_f43_select_tile_config(const problem_t* problem, int* cfg);
 
 /* Tile configurations of the GEMM stage. */
 int conv2d_num_tile_configs(void);
@@ -291,10 +328,12 @@ int conv2d_fp8_select_tile_config(const problem_t* problem, int* cfg);
 
 /* ---- Tuned tile-configuration table (replaces the reference's hard-coded (n, c) dispatch,
  * conv2d.cpp:273-494). getkernelInfo consults it before the shape heuristic. Each entry holds
- * the staged GEMM configuration and the form (CONV2D_FORM_STAGED or CONV2D_FORM_FUSED). ---- */
+ * the form (CONV2D_FORM_STAGED, CONV2D_FORM_FUSED or CONV2D_FORM_F43) and its GEMM
+ * configuration. ---- */
 /* Staged form with configuration cfg. */
 int conv2d_tuned_set(const problem_t* problem, int cfg);
-/* Form `form` and staged configuration cfg. */
+/* Form `form` with GEMM configuration cfg (an index into conv2d_tile_config's table, or into
+ * conv2d_f43_tile_config's for CONV2D_FORM_F43). */
 int conv2d_tuned_set_form(const problem_t* problem, int cfg, int form);
 int conv2d_tuned_clear(void);
 int conv2d_tuned_count(void);
@@ -308,12 +347,18 @@ int conv2d_tuned_load(const char* path);
  * problem->out is overwritten) with a deterministic best-of-`reps` protocol on `stream`,
  * gates each configuration on its output being bit-identical to configuration 0's, then times
  * the input transform and, where the fused form applies, the fused kernel (gated on
- * max |fused - staged| <= 1e-2 max |staged|: its FP16 input transform rounds differently), and
- * records the fastest passing configuration and the faster form (fused when it beats input
- * transform + GEMM) in the tuned table. times_us[i] (i < ntimes) receives: for i < 16 the best
- * time of configuration i in microseconds, or -1 if it failed the gate; times_us[16] the input
+ * max |fused - staged| <= 1e-2 max |staged|: its FP16 input transform rounds differently), and,
+ * where the F(4x4,3x3) form applies and problem->mid_result is set, that form (its filter and
+ * input transforms, every one of its GEMM configurations, each gated on bit-identity with the
+ * first, and its output transform; the output gated as the fused form's), then records the
+ * fastest form with its fastest configuration in the tuned table. With mid_result set, every
+ * workspace must have conv2d_workspace_size's size. On return transform_filter holds the U of
+ * the recorded form. times_us[i] (i < ntimes) receives: for i < 16 the best time of
+ * configuration i in microseconds, or -1 if it failed the gate; times_us[16] the input
  * transform; times_us[17] the fused kernel (-1 when the fused form does not apply or failed
- * its gate). Allocates temporary output-sized buffers. */
+ * its gate); times_us[18] the F(4x4) form's input transform + best GEMM + output transform
+ * (-1 when it was not timed or failed its gate). Allocates temporary buffers of the output's
+ * and M's size. */
 int conv2d_autotune(const problem_t* problem, int reps, void* stream, int* best_cfg,
                     float* times_us, int ntimes);
 

@@ -133,7 +133,7 @@ def test_weights_load_refuses_foreign_files_before_device_access(wc, tmp_path):
     L = wc.load_library()
     n, c, h, w, k = 2, 40, 10, 10, 24
     pb = wc.make_problem(n, c, h, w, k, ptrs=FAKE)
-    _, ub, _ = wc.workspace_size(pb)
+    _, ub, _ = wc.form_workspace_size(pb, wc.FORM_STAGED)  # the U of the problem's (F(2x2)) form
 
     def header(magic=b"WINOU16\0", version=1, method=wc.METHOD_WINOGRAD, kk=k, cc=c, r=3, s_=3, ubytes=ub, zero=0):
         return struct.pack("<8sII4IQQQQ", magic, version, method, kk, cc, r, s_, 0, ubytes, 0, zero)
@@ -148,6 +148,7 @@ def test_weights_load_refuses_foreign_files_before_device_access(wc, tmp_path):
     assert load(header(magic=b"WINOU08\0")) == wc.ERR_MISMATCH
     assert load(header(version=2)) == wc.ERR_MISMATCH
     assert load(header(method=wc.METHOD_DIRECT)) == wc.ERR_MISMATCH
+    assert load(header(method=2)) == wc.ERR_MISMATCH                  # an F(4x4) U
     assert load(header(kk=k + 1)) == wc.ERR_MISMATCH
     assert load(header(ubytes=ub - 1024)) == wc.ERR_MISMATCH
     assert load(header(zero=1)) == wc.ERR_MISMATCH
@@ -374,11 +375,19 @@ def test_workspace_sizes_and_legacy_sizing(wc):
     for (n, c, h, k) in [(32, 256, 56, 256), (1, 64, 56, 64), (64, 512, 28, 512), (32, 96, 55, 96),
                          (1, 3, 13, 32), (16, 128, 64, 27), (2, 40, 10, 20)]:
         pb = wc.make_problem(n, c, h, h, k)
-        vb, ub, mb = wc.workspace_size(pb)
+        vb, ub, mb = wc.form_workspace_size(pb, wc.FORM_STAGED)
         tiles = (h + 1) // 2
         g32 = -(-(n * tiles * tiles) // 32)
         cb, kb = -(-c // 32), -(-k // 32)
         assert vb == 16 * 2 * cb * g32 * 512 * 2 and ub == 16 * 2 * cb * kb * 512 * 2 and mb == 0
+        assert wc.form_workspace_size(pb, wc.FORM_FUSED) == (vb, ub, 0)
+        # the F(4x4) form: 36 positions over 4x4 tiles; M as FP16 with rows and columns padded to 32
+        t4 = (h + 3) // 4
+        g4 = -(-(n * t4 * t4) // 32)
+        v4, u4, m4 = wc.form_workspace_size(pb, wc.FORM_F43)
+        assert (v4, u4, m4) == (36 * 2 * cb * g4 * 1024, 36 * 2 * cb * kb * 1024, 36 * kb * 32 * g4 * 32 * 2)
+        # conv2d_workspace_size covers every form
+        assert wc.workspace_size(pb) == (max(vb, v4), max(ub, u4), m4)
         legacy_ok = vb <= 4 * h * h * n * c * 2 and ub <= 16 * k * c * 2
         predicted = h % 2 == 0 and c % 32 == 0 and k % 32 == 0 and (n * tiles * tiles) % 32 == 0
         assert legacy_ok == predicted, (n, c, h, k)
@@ -470,3 +479,61 @@ def test_tuned_table_records_the_form(wc, tmp_path):
 def test_missing_library_fails_loudly(wc, tmp_path):
     with pytest.raises(RuntimeError):
         wc.load_library(str(tmp_path / "nope.so"))
+
+
+def test_f43_form_slots_from_the_tuned_table(wc, tmp_path):
+    """A shape recorded as CONV2D_FORM_F43 runs its four slots (filter, input transform, the
+    36-position GEMM, output transform) when mid_result is set; without mid_result the shape
+    falls back to the heuristic form. The blob carries the 4x4-tile geometry."""
+    L = wc.load_library()
+    f = ctypes.c_int()
+    ptrs = dict(FAKE, mid_result=256)
+    pb = wc.make_problem(32, 256, 56, 56, 256, ptrs=ptrs)
+    L.conv2d_tuned_clear()
+    try:
+        assert L.conv2d_tuned_set_form(ctypes.byref(pb), 5, wc.FORM_F43) == wc.OK
+        assert L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK and f.value == wc.FORM_F43
+        rc, ki, prm = _ki(wc, pb)
+        assert rc == wc.OK and prm.form == wc.FORM_F43 and prm.cfg == 5
+        assert (prm.tiles_h, prm.tiles_w, prm.T, prm.NT, prm.G32, prm.CB, prm.KB32) == (14, 14, 196, 6272, 196, 8, 8)
+        # band: 4 tile rows need 18 x 66 x 64 B > 64 KiB of LDS, 2 rows 42,240 B
+        assert prm.band == 2 and ki.slot(2)["lds"] == 32 * 10 * 66 * 2
+        assert ki.slot(1)["grid"] == (8 * 16 // 4, 1, 1) and ki.slot(1)["block"] == (256, 1, 1)
+        assert ki.slot(2)["grid"] == (7, 8, 32)
+        bk, bt, threads = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        assert L.conv2d_f43_tile_config(5, ctypes.byref(bk), ctypes.byref(bt), ctypes.byref(threads)) == wc.OK
+        assert ki.slot(3)["grid"] == (-(-8 // (bk.value // 32)) * -(-196 // (bt.value // 32)) * 36, 1, 1)
+        assert ki.slot(3)["block"] == (threads.value, 1, 1)
+        assert ki.slot(4)["grid"] == (-(-6272 // 256), 64, 1) and ki.slot(4)["block"] == (256, 1, 1)
+        # the tuned file keeps the form
+        path = str(tmp_path / "t.txt").encode()
+        assert L.conv2d_tuned_save(path) == wc.OK and L.conv2d_tuned_clear() == wc.OK
+        assert L.conv2d_tuned_load(path) == wc.OK and L.conv2d_tuned_count() == 1
+        assert L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK and f.value == wc.FORM_F43
+        # no mid_result: the heuristic form (fused for K <= 384)
+        nomid = wc.make_problem(32, 256, 56, 56, 256, ptrs=FAKE)
+        assert L.conv2d_form(ctypes.byref(nomid), ctypes.byref(f)) == wc.OK and f.value == wc.FORM_FUSED
+        rc, _, prm = _ki(wc, nomid)
+        assert rc == wc.OK and prm.form == wc.FORM_FUSED
+        # an F(4x4) entry indexes the F(4x4) GEMM table
+        n43 = L.conv2d_f43_num_tile_configs()
+        assert L.conv2d_tuned_set_form(ctypes.byref(pb), n43, wc.FORM_F43) == wc.ERR_INVALID
+        assert L.conv2d_tuned_set_form(ctypes.byref(pb), n43, wc.FORM_STAGED) == wc.OK
+    finally:
+        L.conv2d_tuned_clear()
+
+
+def test_f43_entry_points_refuse(wc):
+    L = wc.load_library()
+    pb = wc.make_problem(2, 16, 12, 12, 16, ptrs=FAKE)  # no mid_result
+    assert L.conv2d_f43_gemm(ctypes.byref(pb), -1, None) == wc.ERR_NULL
+    assert L.conv2d_f43_output_transform(ctypes.byref(pb), None) == wc.ERR_NULL
+    assert L.conv2d_f43_forward(ctypes.byref(pb), -1, None) == wc.ERR_NULL
+    withmid = wc.make_problem(2, 16, 12, 12, 16, ptrs=dict(FAKE, mid_result=256))
+    assert L.conv2d_f43_gemm(ctypes.byref(withmid), L.conv2d_f43_num_tile_configs(), None) == wc.ERR_INVALID
+    direct = wc.make_problem(2, 16, 12, 12, 16, r=5, s=5, p=2, q=2, ptrs=dict(FAKE, mid_result=256))
+    for fn in (L.conv2d_f43_prepare_weights, L.conv2d_f43_input_transform, L.conv2d_f43_output_transform):
+        assert fn(ctypes.byref(direct), None) == wc.ERR_UNSUPPORTED
+    assert wc.form_workspace_size(direct, wc.FORM_F43) is None
+    assert L.conv2d_tuned_set_form(ctypes.byref(pb), 0, 3) == wc.ERR_INVALID
+    assert L.conv2d_f43_tile_config(-1, None, None, None) == wc.ERR_INVALID
