@@ -80,7 +80,7 @@ class ParamType(ctypes.Structure):
                [(f, ctypes.c_int) for f in
                 ("tiles_h", "tiles_w", "T", "NT", "G32", "CB", "KB32", "band", "cfg", "flags", "nsteps",
                  "fbh", "fbw", "fnbh", "fnbw", "fpr", "fpc", "fsh", "fnu", "act", "form")] + \
-               [("bias", ctypes.c_void_p), ("io", ctypes.c_int), ("io_pad_", ctypes.c_int)]
+               [("bias", ctypes.c_void_p), ("io", ctypes.c_int), ("f43_npos", ctypes.c_int)]
 
 
 _lib = None

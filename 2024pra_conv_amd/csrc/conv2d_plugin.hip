@@ -300,8 +300,21 @@ int select_form(const problem_t* pb, const Geometry& g) {
     return use_fused(pb, g) ? CONV2D_FORM_FUSED : CONV2D_FORM_STAGED;
 }
 
+// Positions per F(4x4) GEMM workgroup: the most (a divisor of 36) that still gives two
+// workgroups per compute unit, so each pipeline runs over several positions' steps instead of
+// filling and draining its ring once per position.
+static int f43_positions_per_item(const Geometry& g, int cfg) {
+    const GemmConfig& c = kPosConfigs[cfg];
+    const long long base = (long long)((g.KB32 + c.bk / 32 - 1) / (c.bk / 32)) * ((g.G32_4 + c.bt / 32 - 1) / (c.bt / 32));
+    const int divs[9] = {36, 18, 12, 9, 6, 4, 3, 2, 1};
+    for (int i = 0; i < 9; ++i)
+        if (base * (36 / divs[i]) >= 2LL * compute_units()) return divs[i];
+    return 1;
+}
+
 void fill_param43(const problem_t* pb, const Geometry& g, int cfg, mykernelParamType* a) {
     fill_param(pb, g, cfg, a);
+    a->f43_npos = f43_positions_per_item(g, cfg);
     a->tiles_h = g.t4h;
     a->tiles_w = g.t4w;
     a->T = (int)g.T4;
@@ -312,9 +325,9 @@ void fill_param43(const problem_t* pb, const Geometry& g, int cfg, mykernelParam
     a->form = CONV2D_FORM_F43;
 }
 
-// F(4x4) GEMM heuristic (used when the tuned table has no entry): the 8-stage 128x128 tile for
-// deep reductions (CB >= 8) with two workgroups per compute unit, the 4-stage 128x128 tile
-// (64x64 waves) when it still gives two, 64x128 when it gives one, else 32x64.
+// F(4x4) GEMM heuristic (used when the tuned table has no entry), from measurements on MI355X:
+// the 128x128 tile of 64x64 waves for deep reductions (CB >= 8) with four 128x128 items per
+// compute unit, else the 128x64 tile when it gives one item per unit, else 32x64.
 int select_config43(const problem_t* pb, const Geometry& g) {
     if (tuned_form_lookup(pb) == CONV2D_FORM_F43) {
         const int cfg = tuned_config_lookup(pb);
@@ -324,16 +337,16 @@ int select_config43(const problem_t* pb, const Geometry& g) {
     auto items = [&](int bk, int bt) {
         return 36LL * ((g.KB32 + bk / 32 - 1) / (bk / 32)) * ((g.G32_4 + bt / 32 - 1) / (bt / 32));
     };
-    if (g.CB >= 8 && items(128, 128) >= 2 * cus) return 0;
-    if (items(128, 128) >= 2 * cus) return 5;
-    if (items(64, 128) >= cus) return 4;
+    if (g.CB >= 8 && items(128, 128) >= 4 * cus) return 5;
+    if (items(128, 64) >= cus) return 1;
     return 3;
 }
 
 Launch slot_filter43(const Geometry& g) {
     Launch L;
     L.kernel = (const void*)&wino43_filter_transform;
-    L.grid = dim3((unsigned)((g.KB32 * 2 * g.CB + 3) / 4), 1, 1);  // one wave per fragment column
+    // one thread per element of one position's packed U
+    L.grid = dim3((unsigned)(((long long)g.KB32 * 2 * g.CB * kFragHalves + 255) / 256), 1, 1);
     L.block = dim3(256, 1, 1);
     L.lds = 0;
     return L;
@@ -354,7 +367,7 @@ Launch slot_gemm43(const Geometry& g, int cfg) {
     L.kernel = c.kernel;
     const int vf = c.bt / 32, uf = c.bk / 32;
     const unsigned ng = (unsigned)((g.G32_4 + vf - 1) / vf), nk = (unsigned)((g.KB32 + uf - 1) / uf);
-    L.grid = dim3(ng * nk * 36u, 1, 1);
+    L.grid = dim3(ng * nk * (unsigned)(36 / f43_positions_per_item(g, cfg)), 1, 1);
     L.block = dim3((unsigned)c.threads, 1, 1);
     L.lds = 0;
     return L;

@@ -11,23 +11,25 @@
 //   G'  = [[1, 0, 0], [-1/4, -1/4, -1/4], [1/4, -1/4, 1/4], [1/4, 1/2, 1], [-1, 1/2, -1/4], [0, 0, 1]]
 //   A'^T = [[1, 4/3, 4/3, 4/15, 16/15, 0], [0, 4/3, -4/3, 8/15, -8/15, 0],
 //           [0, 4/3, 4/3, 16/15, 4/15, 0], [0, 4/3, -4/3, 32/15, -2/15, 1]]
-// S1 and S2 compute in FP64, where every sum of FP16 values times these dyadic constants is exact
-// (fewer than 53 significant bits for any FP16 input), so U and V are the exact transforms
-// rounded once to FP16, whatever the operation order. S3 stores FP16(M / 4); S4 undoes the
-// scale inside its FP32 coefficients (2 A'^T applied on both sides).
+// S1 computes in FP64, where every sum of FP16 values times these dyadic constants is exact
+// (fewer than 53 significant bits for any FP16 input), so U is the exact transform rounded once
+// to FP16, whatever the operation order. S2 computes in FP32 in a fixed operation order (no
+// contraction; the oracle restates it) and rounds once to FP16. S3 stores FP16(M / 4); S4 undoes
+// the scale inside its FP32 coefficients (2 A'^T applied on both sides).
 #include "wino_tile_ops.h"
 
 namespace {
 
-// One 6-point B^T transform (exact in FP64).
-__device__ inline void bt6(const double d[6], double o[6]) {
-    const double a = d[1] - d[3], e = d[4] - d[2];
-    o[0] = __builtin_fma(1.5, a, __builtin_fma(-2.0, d[2], d[0] + d[4]));
-    o[1] = __builtin_fma(-2.5, d[2], __builtin_fma(-0.5, d[3], d[4] - d[1]));
-    o[2] = __builtin_fma(0.5, d[2], __builtin_fma(-2.5, d[3], d[4] + d[1]));
-    o[3] = __builtin_fma(-0.5, a, e);
-    o[4] = __builtin_fma(2.0, a, e);
-    o[5] = __builtin_fma(-1.5, e, __builtin_fma(-2.0, d[3], d[1] + d[5]));
+// One 6-point B^T transform in FP32, every product and sum rounded once in this order.
+__device__ inline void bt6f(const float d[6], float o[6]) {
+#pragma clang fp contract(off)
+    const float a = d[1] - d[3], e = d[4] - d[2];
+    o[0] = 1.5f * a + (-2.0f * d[2] + (d[0] + d[4]));
+    o[1] = -2.5f * d[2] + (-0.5f * d[3] + (d[4] - d[1]));
+    o[2] = 0.5f * d[2] + (-2.5f * d[3] + (d[4] + d[1]));
+    o[3] = -0.5f * a + e;
+    o[4] = 2.0f * a + e;
+    o[5] = -1.5f * e + (-2.0f * d[3] + (d[1] + d[5]));
 }
 
 // One 3-point G' transform (exact in FP64).
@@ -54,57 +56,47 @@ __device__ inline void at4(const float m[6], float y[4]) {
 
 }  // namespace
 
-// S1: U = G' g G'^T for 8 channels x one 32-row block per lane pair, packed as
-// U[pos][c16][kb][lane][8] (lane = (c % 16 / 8) * 32 + k % 32). One wave per fragment column
-// (c16, kb); zeros for k >= K or c >= C.
+// S1: U = G' g G'^T, one thread per element (k, c) of the packed U[pos][c16][kb][lane][8]
+// (lane = (c % 16 / 8) * 32 + k % 32, element c % 8): consecutive threads write consecutive
+// halves of a fragment, so each of the 36 stores of a wave is 128 contiguous bytes. Zeros for
+// k >= K or c >= C.
 __global__ void __launch_bounds__(256) wino43_filter_transform(mykernelParamType prm) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int KB = prm.KB32, C16 = 2 * prm.CB;
-    const int col = blockIdx.x * 4 + wave;
-    if (col >= C16 * KB) return;
-    const int c16 = col / KB, kb = col - c16 * KB;
-    const int K = prm.k, C = prm.c;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= C16 * KB * kFragHalves) return;
+    const int frag = idx / kFragHalves, r = idx - frag * kFragHalves;
+    const int c16 = frag / KB, kb = frag - c16 * KB;
+    const int lane = r >> 3, e = r & 7;
     const int k = kb * 32 + (lane & 31);
-    const int c0 = c16 * 16 + 8 * (lane >> 5);
-    const half_t* __restrict__ wgt = (const half_t*)prm.pweight;
-    half_t* dst = (half_t*)prm.transform_filter + ((size_t)c16 * KB + kb) * kFragHalves + (size_t)lane * 8;
-    const size_t pos_stride = (size_t)C16 * KB * kFragHalves;
-    // t[e][i][s] = (G' g)[i][s] for channel c0 + e
-    double t[8][6][3];
+    const int c = c16 * 16 + 8 * (lane >> 5) + e;
+    const bool ok = k < prm.k && c < prm.c;
+    const half_t* wp = (const half_t*)prm.pweight + ((size_t)(ok ? k : 0) * prm.c + (ok ? c : 0)) * 9;
+    double t[6][3];  // (G' g)[i][s]
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int c = c0 + e;
-        const bool ok = k < K && c < C;
-        const half_t* wp = wgt + ((size_t)(ok ? k : 0) * C + (ok ? c : 0)) * 9;
+    for (int s = 0; s < 3; ++s) {
+        double g[3], o[6];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            double g[3], o[6];
+        for (int rr = 0; rr < 3; ++rr) g[rr] = ok ? (double)wp[rr * 3 + s] : 0.0;
+        g6(g, o);
 #pragma unroll
-            for (int r = 0; r < 3; ++r) g[r] = ok ? (double)wp[r * 3 + s] : 0.0;
-            g6(g, o);
-#pragma unroll
-            for (int i = 0; i < 6; ++i) t[e][i][s] = o[i];
-        }
+        for (int i = 0; i < 6; ++i) t[i][s] = o[i];
     }
+    half_t* dst = (half_t*)prm.transform_filter + (size_t)idx;
+    const size_t pos_stride = (size_t)C16 * KB * kFragHalves;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-        half8 outv[6];
+        double o[6];
+        g6(t[i], o);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            double o[6];
-            g6(t[e][i], o);
-#pragma unroll
-            for (int j = 0; j < 6; ++j) outv[j][e] = f64_to_f16(o[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < 6; ++j) *(half8*)(dst + (size_t)(6 * i + j) * pos_stride) = outv[j];
+        for (int j = 0; j < 6; ++j) dst[(size_t)(6 * i + j) * pos_stride] = f64_to_f16(o[j]);
     }
 }
 
 // S2: V = B^T d B of the 6x6 window of every 4x4 tile in tile-row band bi (image n, 32-channel
-// block cb), staged in LDS like the F(2x2) input transform; each lane takes one tile x 8
-// channels (4 per pass) of a 16-channel half and writes its slot of the 36 position fragments
-// V[pos][c16][g32][lane][8]. The last image's padding columns (g >= NT) are written as zeros.
+// block cb), staged in LDS like the F(2x2) input transform. One thread per (tile of the band,
+// 4-channel quad of the block): it writes its 8 bytes of the lane slot of each of the 36
+// position fragments V[pos][c16][g32][lane][8] (lane = (c % 16 / 8) * 32 + g % 32). The last
+// band of the last image also writes the padding columns g in [NT, G32*32) as zeros.
 __global__ void __launch_bounds__(256) wino43_input_transform(mykernelParamType prm) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
     const int n = blockIdx.z, cb = blockIdx.y, bi = blockIdx.x;
@@ -115,75 +107,67 @@ __global__ void __launch_bounds__(256) wino43_input_transform(mykernelParamType 
     const int LR = input43_lds_rows(TH);
     const int LWp = input43_lds_pitch(tiles_w, prm.w, prm.q);
     stage_band(prm, n, cb, 4 * th0 - prm.p, LR, LWp, lds);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     const int T = prm.T, NT = prm.NT, G32 = prm.G32, CB16 = 2 * prm.CB;
     const int gs = n * T + th0 * tiles_w;
     const int ge = n * T + th1 * tiles_w;
-    const int kg = lane >> 5;
+    // the last band of the last image also covers the padding columns of the last 32-column block
+    const int gend = ge == NT ? G32 * 32 : ge;
     const size_t pos_stride = (size_t)CB16 * G32 * kFragHalves;
-    half_t* V = (half_t*)prm.transform_pin + (size_t)lane * 8;
-    const int grp0 = gs >> 5;
-    const int nunits = 2 * (((ge - 1) >> 5) - grp0 + 1);
-    for (int u = wave; u < nunits; u += nwaves) {
-        const int grp = grp0 + (u >> 1), h = u & 1;
-        const int g = grp * 32 + (lane & 31);
-        const bool valid = g >= gs && g < ge;
-        const bool tail = (ge == NT) && g >= NT && g < G32 * 32;
-        if (!valid && !tail) continue;
-        half_t* dst = V + ((size_t)(2 * cb + h) * G32 + grp) * kFragHalves;
-        int th = 0, tw = 0;
-        if (valid) {
+    typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+    const int items = (gend - gs) * 8;
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        const int quad = it & 7;  // channels 4*quad .. +3 of the staged 32
+        const int g = gs + (it >> 3);
+        const int c16 = 2 * cb + (quad >> 2);
+        const int lane = (g & 31) + 32 * ((quad >> 1) & 1);
+        half_t* dst = (half_t*)prm.transform_pin + ((size_t)c16 * G32 + (g >> 5)) * kFragHalves + lane * 8 +
+                      4 * (quad & 1);
+        uint32_t outv[36][2];  // [pos][channel pair]
+#pragma unroll
+        for (int pos = 0; pos < 36; ++pos) outv[pos][0] = outv[pos][1] = 0u;
+        if (g < ge) {
             const int t = g - n * T;
-            th = t / tiles_w;
-            tw = t - th * tiles_w;
-        }
+            const int th = t / tiles_w;
+            const int tw = t - th * tiles_w;
 #pragma unroll
-        for (int e0 = 0; e0 < 8; e0 += 4) {
-            uint32_t outv[36][2];  // [pos][channel pair] of this pass's 4 channels
+            for (int e = 0; e < 4; ++e) {
+                const int ch = 4 * quad + e;
+                const half_t* base = lds + (ch * LR + 4 * (th - th0)) * LWp + 4 * tw + 8;
+                float tt[6][6];  // tt[i][j] = (B^T d)[i][j]
 #pragma unroll
-            for (int pos = 0; pos < 36; ++pos) outv[pos][0] = outv[pos][1] = 0u;
-            if (valid) {
+                for (int j2 = 0; j2 < 3; ++j2) {
+                    float c0[6], c1[6];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int ch = h * 16 + kg * 8 + e0 + e;
-                    const half_t* base = lds + (ch * LR + 4 * (th - th0)) * LWp + 4 * tw + 8;
-                    double t[6][6];  // t[i][j] = (B^T d)[i][j]
-#pragma unroll
-                    for (int j2 = 0; j2 < 3; ++j2) {
-                        double c0[6], c1[6];
-#pragma unroll
-                        for (int r = 0; r < 6; ++r) {
-                            const half2_t pr = *(const half2_t*)(base + r * LWp + 2 * j2);
-                            c0[r] = (double)pr[0];
-                            c1[r] = (double)pr[1];
-                        }
-                        double o0[6], o1[6];
-                        bt6(c0, o0);
-                        bt6(c1, o1);
-#pragma unroll
-                        for (int i = 0; i < 6; ++i) {
-                            t[i][2 * j2] = o0[i];
-                            t[i][2 * j2 + 1] = o1[i];
-                        }
+                    for (int r = 0; r < 6; ++r) {
+                        const half2_t pr = *(const half2_t*)(base + r * LWp + 2 * j2);
+                        c0[r] = (float)pr[0];
+                        c1[r] = (float)pr[1];
                     }
+                    float o0[6], o1[6];
+                    bt6f(c0, o0);
+                    bt6f(c1, o1);
 #pragma unroll
                     for (int i = 0; i < 6; ++i) {
-                        double o[6];
-                        bt6(t[i], o);
+                        tt[i][2 * j2] = o0[i];
+                        tt[i][2 * j2 + 1] = o1[i];
+                    }
+                }
 #pragma unroll
-                        for (int j = 0; j < 6; ++j) {
-                            const uint32_t hb = __builtin_bit_cast(uint16_t, f64_to_f16(o[j]));
-                            outv[6 * i + j][e >> 1] |= hb << (16 * (e & 1));
-                        }
+                for (int i = 0; i < 6; ++i) {
+                    float o[6];
+                    bt6f(tt[i], o);
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        const uint32_t hb = __builtin_bit_cast(uint16_t, (half_t)o[j]);
+                        outv[6 * i + j][e >> 1] |= hb << (16 * (e & 1));
                     }
                 }
             }
+        }
 #pragma unroll
-            for (int pos = 0; pos < 36; ++pos) {
-                typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
-                const u32x2_t pk = {outv[pos][0], outv[pos][1]};
-                *(u32x2_t*)(dst + (size_t)pos * pos_stride + e0) = pk;
-            }
+        for (int pos = 0; pos < 36; ++pos) {
+            const u32x2_t pk = {outv[pos][0], outv[pos][1]};
+            *(u32x2_t*)(dst + (size_t)pos * pos_stride) = pk;
         }
     }
 }

@@ -39,8 +39,8 @@ constexpr int KS = 2;
 // What a GEMM body computes:
 //   kGemmWino   the F(2x2) GEMM: 16 positions, fold per position, 2x2-tile epilogue;
 //   kGemmDirect the implicit-GEMM direct convolution of the general-geometry path;
-//   kGemmPos    one position `fpos` of the F(4x4) form (36 positions): CB steps of its packed
-//               U and V, M stored in FP16 to mid_result (store_m16).
+//   kGemmPos    positions fpos .. fpos + f43_npos - 1 of the F(4x4) form (36 positions): their
+//               packed U and V in sequence, each M stored in FP16 to mid_result (store_m16).
 enum { kGemmWino = 0, kGemmDirect = 1, kGemmPos = 2 };
 
 template <int BK, int BT, int WK, int WT>
@@ -67,7 +67,8 @@ struct GemmShape {
 // MODE kGemmDirect: the implicit-GEMM direct convolution of the general-geometry path: the
 // steps run over prm.nsteps chunks of 32 values of the flattened (filter tap, channel)
 // reduction index into one accumulator, whose elements are the outputs themselves.
-// MODE kGemmPos: position fpos of the F(4x4) form (CB steps), M stored by store_m16.
+// MODE kGemmPos: positions fpos .. fpos + prm.f43_npos - 1 of the F(4x4) form in one pipeline
+// (npos * CB steps), each position's M stored by store_m16 at its last step.
 template <int BK, int BT, int WK, int WT, int D, int MODE = kGemmWino>
 __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem, int fpos = 0) {
     using S = GemmShape<BK, BT, WK, WT>;
@@ -82,9 +83,9 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
     const int gb0 = gblk * VF;  // first 32-column block of V
     const int kb0 = kblk * UF;  // first 32-row block of U
-    // 32-value reduction chunks (a multiple of 8 except per F(4x4) position, whose steps past
-    // CB issue no MFMAs)
-    const int nsteps = MODE == kGemmDirect ? prm.nsteps : (MODE == kGemmPos ? CB : 16 * CB);
+    // 32-value reduction chunks (a multiple of 8 except for F(4x4) position groups, whose steps
+    // past npos * CB issue no MFMAs)
+    const int nsteps = MODE == kGemmDirect ? prm.nsteps : (MODE == kGemmPos ? prm.f43_npos * CB : 16 * CB);
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
     if constexpr (MODE == kGemmPos) {
@@ -201,16 +202,28 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
                     cb = 0;
                     ++pos;
                 }
+            } else if constexpr (MODE == kGemmPos) {
+                // at the end of each position of the group: M to mid_result, then the next one
+                if (s < nsteps && ++cb == CB) {
+                    store_m16<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane, fpos + pos);
+#pragma unroll
+                    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+                            for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+                    cb = 0;
+                    ++pos;
+                }
             }
         }
     }
 
     if constexpr (MODE == kGemmDirect)
         store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
-    else if constexpr (MODE == kGemmPos)
-        store_m16<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane, fpos);
-    else
+    else if constexpr (MODE == kGemmWino)
         store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
+    // kGemmPos: every position of the group was stored at its last step
 }
 
 // ---------------------------------------------------------------------------------------
@@ -256,7 +269,7 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
     const int KB = prm.KB32, G32 = prm.G32, CB = prm.CB;
     const int gb0 = gblk * VF;
     const int kb0 = kblk * UF;
-    const int nsteps = MODE == kGemmDirect ? prm.nsteps : (MODE == kGemmPos ? CB : 16 * CB);
+    const int nsteps = MODE == kGemmDirect ? prm.nsteps : (MODE == kGemmPos ? prm.f43_npos * CB : 16 * CB);
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
     if constexpr (MODE == kGemmPos) {
@@ -362,6 +375,19 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
                     cb = 0;
                     ++pos;
                 }
+            } else if constexpr (MODE == kGemmPos) {
+                // at the end of each position of the group: M to mid_result, then the next one
+                if (s < nsteps && ++cb == CB) {
+                    store_m16<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane, fpos + pos);
+#pragma unroll
+                    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+                            for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+                    cb = 0;
+                    ++pos;
+                }
             }
         }
     }
@@ -369,10 +395,9 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
 
     if constexpr (MODE == kGemmDirect)
         store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
-    else if constexpr (MODE == kGemmPos)
-        store_m16<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane, fpos);
-    else
+    else if constexpr (MODE == kGemmWino)
         store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
+    // kGemmPos: every position of the group was stored at its last step
 }
 
 // Waves per SIMD the register allocation must allow: two for 64x32 wave tiles (so two waves
@@ -524,9 +549,9 @@ const GemmConfig kDirectConfigs[] = {
 const int kNumDirectConfigs = (int)(sizeof(kDirectConfigs) / sizeof(kDirectConfigs[0]));
 
 // ---- per-position GEMM of the F(4x4,3x3) form -----------------------------------------------
-// One workgroup per (k-block, column block, position): the position's CB steps, M to
-// mid_result. Without the fold's output accumulators a wave holds only its MFMA accumulators,
-// so 64 x 64 wave tiles keep two waves per SIMD.
+// One workgroup per (k-block, column block, group of f43_npos positions): the group's steps in
+// one pipeline, each position's M to mid_result. Without the fold's output accumulators a wave
+// holds only its MFMA accumulators, so 64 x 64 wave tiles keep two waves per SIMD.
 
 template <int BK, int BT, int WK, int WT, int D>
 __global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(2)))
@@ -536,8 +561,8 @@ wino_pos_kernel(mykernelParamType prm) {
     const int nk = (prm.KB32 + S::UF - 1) / S::UF;
     const int ng = (prm.G32 + S::VF - 1) / S::VF;
     int kblk, rest;
-    xcd_remap(blockIdx.x, nk * ng * 36, nk, &kblk, &rest);
-    gemm_body<BK, BT, WK, WT, D, kGemmPos>(prm, kblk, rest % ng, smem, rest / ng);
+    xcd_remap(blockIdx.x, nk * ng * (36 / prm.f43_npos), nk, &kblk, &rest);
+    gemm_body<BK, BT, WK, WT, D, kGemmPos>(prm, kblk, rest % ng, smem, (rest / ng) * prm.f43_npos);
 }
 
 template <int BK, int BT, int WK, int WT, int S>
@@ -548,8 +573,8 @@ wino_pos_dma_kernel(mykernelParamType prm) {
     const int nk = (prm.KB32 + Sh::UF - 1) / Sh::UF;
     const int ng = (prm.G32 + Sh::VF - 1) / Sh::VF;
     int kblk, rest;
-    xcd_remap(blockIdx.x, nk * ng * 36, nk, &kblk, &rest);
-    gemm_dma_body<BK, BT, WK, WT, S, kGemmPos>(prm, kblk, rest % ng, smem, rest / ng);
+    xcd_remap(blockIdx.x, nk * ng * (36 / prm.f43_npos), nk, &kblk, &rest);
+    gemm_dma_body<BK, BT, WK, WT, S, kGemmPos>(prm, kblk, rest % ng, smem, (rest / ng) * prm.f43_npos);
 }
 
 #define POS_CFG(BK, BT, WK, WT, D) \

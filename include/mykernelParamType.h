@@ -47,12 +47,14 @@ typedef struct mykernelParamType {
     /* output epilogue, applied in FP32 before the one FP16 rounding of every output:
        out = act(y + bias[k]); bias: K FP16 values or NULL, act: 0 none, 1 ReLU */
     int act;
-    int form;  /* Winograd path: 0 staged (slots: S1, S2, S3, empty), 1 fused (S1, fused, empty x2) */
+    int form;  /* Winograd path: 0 staged (slots: S1, S2, S3, empty), 1 fused (S1, fused, empty x2),
+                  2 F(4x4,3x3) (S1, S2, the per-position GEMM, output transform) */
     const void* bias;
     /* tensor layouts and output type (CONV2D_IO_* bits of conv2d.h): bit 0 input N,H,W,C (else
        N,C,H,W), bit 1 output N,Oh,Ow,K (else N,K,Oh,Ow), bit 2 bfloat16 output (else FP16) */
     int io;
-    int io_pad_;
+    /* F(4x4,3x3) form: Winograd positions each GEMM workgroup runs in sequence (divides 36) */
+    int f43_npos;
 } mykernelParamType;
 
 /*

