@@ -7,8 +7,9 @@
 // Changes:
 //   * inputs come from a seeded SplitMix64 generator instead of unseeded rand()
 //     (main.cpp:40-48); --dist positive reproduces the reference's (rand()%255)/255 range;
-//   * workspaces are sized from conv2d_workspace_size, not from 4*h*h*n*c etc.
-//     (main.cpp:33-38), which is too small for ragged shapes;
+//   * workspaces (mid_result included, for the F(4x4) form) are sized from
+//     conv2d_workspace_size, not from 4*h*h*n*c etc. (main.cpp:33-38), which is too small for
+//     ragged shapes;
 //   * every HIP call is checked; the plugin's return codes are checked;
 //   * timing runs on one created stream: the reference protocol (all four slots, filter
 //     transform included, main.cpp:112-131) and the steady state (S2 + S3, weights
@@ -104,19 +105,20 @@ int main(int argc, char** argv) {
         h_w[i] = (_Float16)(positive ? (double)(splitmix64(seed + 1, i) % 255) / 255.0
                                      : 2.0 * unit(seed + 1, i) - 1.0);
 
-    _Float16 *d_in, *d_w, *d_out, *d_v, *d_u;
+    _Float16 *d_in, *d_w, *d_out, *d_v, *d_u, *d_m = nullptr;
     double* d_ref;
     HIPCHECK(hipMalloc(&d_in, n_in * 2));
     HIPCHECK(hipMalloc(&d_w, n_w * 2));
     HIPCHECK(hipMalloc(&d_out, n_out * 2));
     HIPCHECK(hipMalloc(&d_v, v_bytes));
     HIPCHECK(hipMalloc(&d_u, u_bytes));
+    if (m_bytes) HIPCHECK(hipMalloc(&d_m, m_bytes));  // the F(4x4) form's M, where that form applies
     HIPCHECK(hipMalloc(&d_ref, n_out * sizeof(double)));
     HIPCHECK(hipMemcpy(d_in, h_in.data(), n_in * 2, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(d_w, h_w.data(), n_w * 2, hipMemcpyHostToDevice));
     HIPCHECK(hipMemset(d_out, 0, n_out * 2));
     pb.in = d_in; pb.weight = d_w; pb.out = d_out;
-    pb.transform_pin = d_v; pb.transform_filter = d_u; pb.mid_result = nullptr;
+    pb.transform_pin = d_v; pb.transform_filter = d_u; pb.mid_result = d_m;
 
     int psize = 0;
     if ((rc = getParamsize(&pb, &psize))) return 3;
@@ -185,5 +187,6 @@ int main(int argc, char** argv) {
     free(param);
     HIPCHECK(hipFree(d_in)); HIPCHECK(hipFree(d_w)); HIPCHECK(hipFree(d_out));
     HIPCHECK(hipFree(d_v)); HIPCHECK(hipFree(d_u)); HIPCHECK(hipFree(d_ref));
+    if (d_m) HIPCHECK(hipFree(d_m));
     return 0;
 }

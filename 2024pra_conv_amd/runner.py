@@ -238,7 +238,7 @@ class WinogradConvFP8:
         self.ow = (wd - s + 2 * pad[1]) + 1
         probe = make_problem(*self.shape)
         self.v_bytes, self.u_bytes = fp8_workspace_size(probe)
-        self.u16_bytes = workspace_size(probe)[1]
+        self.u16_bytes = form_workspace_size(probe, FORM_STAGED)[1]  # the FP16 (F(2x2)) U before the records
         dev = x.device
         self.out = out if out is not None else torch.empty((n, k, self.oh, self.ow), dtype=torch.float16, device=dev)
         self.V = torch.empty(max(self.v_bytes, 2), dtype=torch.uint8, device=dev)

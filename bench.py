@@ -213,6 +213,71 @@ def fp8_section(runner, dev, steps: int) -> dict:
     }
 
 
+def f43_section(runner, dev, steps: int) -> dict:
+    """The F(4x4,3x3) form (chosen through the tuned table) on the workload layer and on the
+    compute-bound N=64 C=512 28x28 K=512 layer: the four-slot step as the reference harness times
+    it (its filter transform, input transform, per-position GEMM and output transform), the stage
+    times, the dominant kernel's roofline and the error against FP64, next to the F(2x2) forms on
+    the same data."""
+    import torch
+    wc = importlib.import_module("2024pra_conv_amd")
+    out = {}
+    for name, (n, c, h, k, seed) in {"N32_C256_56_K256": (32, 256, 56, 256, 20240),
+                                      "N64_C512_28_K512": (64, 512, 28, 512, 20243)}.items():
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        x = torch.randn((n, c, h, h), generator=gen, device=dev, dtype=torch.float32).half()
+        wt = (torch.randn((k, c, 3, 3), generator=gen, device=dev, dtype=torch.float32) * (2.0 / (9 * c)) ** 0.5).half()
+        conv = runner.WinogradConv(x, wt)
+        L = conv.L
+        L.conv2d_tuned_clear()
+        # the F(2x2) forms: staged (input transform + GEMM, library configuration) and fused
+        cfg16 = conv.selected_config()
+        conv.prepare_weights()
+        clock_warmup(lambda: conv.forward(cfg16), min_ms=30.0)
+        t_staged = _events_time(lambda: conv.forward(cfg16), steps)
+        t_fused = _events_time(conv.fused_gemm, steps)
+        # the F(4x4) form through the tuned table: four slots, then its stages
+        cfg = conv.f43_selected_config()
+        conv.set_form(wc.FORM_F43, cfg)
+        ki, param = conv.kernel_info()
+        slots = SlotLauncher(ki, param, torch.cuda.current_stream())
+        clock_warmup(slots.step, min_ms=30.0)
+        t_four = _events_time(slots.step, steps)
+        t_slot = [_events_time(lambda j=j: slots.launch(j), steps) for j in range(4)]
+        t_steady = _events_time(lambda: conv.forward(-1), steps)
+        ref = conv.direct_f64()
+        slots.step()
+        torch.cuda.synchronize()
+        err = _err(conv.out, ref)
+        L.conv2d_tuned_clear()
+        direct = 2.0 * n * k * c * 9 * h * h
+        t4 = ((h + 3) // 4) ** 2
+        gemm_flops = 2.0 * 36 * n * k * c * t4
+        m_bytes = 2.0 * 36 * k * n * t4
+        v_bytes = 2.0 * 36 * c * n * t4
+        t3 = t_slot[2]
+        k_bytes = v_bytes + 2.0 * 36 * k * c + m_bytes
+        out[name] = {
+            "four_slot_us": round(t_four * 1e3, 2), "four_slot_tflops": round(direct / (t_four * 1e-3) / 1e12, 2),
+            "steady_us": round(t_steady * 1e3, 2), "steady_tflops": round(direct / (t_steady * 1e-3) / 1e12, 2),
+            "slots_us": [round(t * 1e3, 2) for t in t_slot], "gemm_config": cfg,
+            "gemm_roofline": {"kernel": "wino_pos_*kernel (36 per-position GEMMs, M to mid_result)",
+                              "mfma_frac": round(gemm_flops / (t3 * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 4),
+                              "hbm_frac": round(k_bytes / (t3 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                              "bound": "hbm" if k_bytes / (PEAK_HBM_GBS * 1e9) > gemm_flops / (PEAK_FP16_TFLOPS * 1e12)
+                              else "mfma", "algorithmic_bytes": k_bytes, "algorithmic_flops": gemm_flops},
+            "s2_hbm_frac": round((2.0 * n * c * h * h + v_bytes) / (t_slot[1] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "s4_hbm_frac": round((m_bytes + 2.0 * n * k * h * h) / (t_slot[3] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "error_vs_fp64": err,
+            "f2x2_staged_steady_us": round(t_staged * 1e3, 2), "f2x2_fused_us": round(t_fused * 1e3, 2),
+            "f2x2_staged_tile_config": cfg16,
+        }
+        del conv, slots
+        torch.cuda.empty_cache()
+    return out
+
+
 def batched_section(runner, dev, steps: int) -> dict:
     """The batching form: 24 seeded layers (N in {1,4,16}, C,K in {3..512}, H=W in {7..112},
     uniform[-1,1]) as one captured persistent graph vs 24 separate plain launches."""
@@ -514,6 +579,7 @@ def main() -> int:
     if not args.no_forms:
         del conv
         torch.cuda.empty_cache()
+        line["f43_form"] = f43_section(runner, dev, max(5, args.steps // 5))
         line["fp8_form"] = fp8_section(runner, dev, max(5, args.steps // 5))
         line["batched_form"] = batched_section(runner, dev, max(5, args.steps // 5))
         line["direct_form"] = direct_section(runner, dev, max(5, args.steps // 5))
