@@ -171,11 +171,12 @@ _SIGS = {
 
 
 def load_library(path: str | None = None):
-    """Loads libwinoconv.so (raises if it is not built: there is no fallback)."""
+    """Loads libwinoconv.so (raises if it is not built: there is no fallback). WINOCONV_LIB
+    names another build of the same library (the sanitizer build of build_lib.py)."""
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("WINOCONV_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError(f"{p} is not built; run __graft_entry__.build() (no CPU fallback exists)")
     L = ctypes.CDLL(p)

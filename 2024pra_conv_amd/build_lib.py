@@ -2,6 +2,15 @@
 
 Usage: python 2024pra_conv_amd/build_lib.py   (or __graft_entry__.build()).
 Compiles every time it is called.
+
+       python 2024pra_conv_amd/build_lib.py --sanitize
+builds a second copy of the library, lib/sanitize/libwinoconv.so, whose host code (plugin
+dispatch, tuned-table and U-cache file parsing) carries AddressSanitizer and
+UndefinedBehaviorSanitizer checks (device code is built as usual). The CPU tests run against it
+with the sanitizer runtime preloaded:
+       LD_PRELOAD=$(ls /opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so) \
+       ASAN_OPTIONS=detect_leaks=0 WINOCONV_LIB=2024pra_conv_amd/lib/sanitize/libwinoconv.so \
+       python -m pytest tests -m "not gpu"
 """
 from __future__ import annotations
 
@@ -60,10 +69,31 @@ def _run(cmd: list[str]) -> None:
         sys.stderr.write("\n".join(noise) + "\n")
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-    _run([HIPCC, *COMMON, *DEVICE_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+# host-only sanitizers: each -fsanitize= applies to the host pass alone
+SANITIZE_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def _compile(src: str, objdir: str = OBJDIR, extra: tuple = ()) -> str:
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+    _run([HIPCC, *COMMON, *DEVICE_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
     return obj
+
+
+def build_sanitized(verbose: bool = False) -> str:
+    """The library with host-side ASan + UBSan, in lib/sanitize/ (the product library is untouched)."""
+    objdir = os.path.join(OBJDIR, "sanitize")
+    outdir = os.path.join(LIBDIR, "sanitize")
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(outdir, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, len(LIB_SOURCES))) as ex:
+        objs = list(ex.map(lambda src: _compile(src, objdir, tuple(SANITIZE_HOST)), LIB_SOURCES))
+    lib = os.path.join(outdir, "libwinoconv.so")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fsanitize=address", "-fsanitize=undefined",
+          "-shared-libasan", *objs, "-o", lib])
+    if verbose:
+        print(f"built {lib}")
+    return lib
 
 
 def build(verbose: bool = False) -> str:
@@ -85,4 +115,7 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    if "--sanitize" in sys.argv[1:]:
+        build_sanitized(verbose=True)
+    else:
+        build(verbose=True)
