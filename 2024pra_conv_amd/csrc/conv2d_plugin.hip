@@ -105,7 +105,10 @@ int derive_geometry_io(const problem_t* pb, int io, Geometry* g) {
         // under 2^31 bytes
         const long long img_bytes = ((long long)pb->c + 32) * pb->h * pb->w * 2;
         const long long out_img_bytes = (long long)pb->k * g->Oh * g->Ow * 2;
+        // (a one-pixel-wide image has no fused form: its straddling pixel pair would load the
+        // dword before the image's first pixel)
         g->fused_ok = (!nhwc_in || nhwc_ok) && g->fnu <= (nhwc_in ? kFusedNhwcUnits : 2) &&
+                      (g->flw != kFusedLoad2 || pb->w != 1) &&
                       fused_kernel_ptr(g->flw, fused_shape_index(g->fbh, g->fbw), g->fsh) &&
                       g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
                       out_img_bytes < (1LL << 31) && (long long)g->u_bytes < (1LL << 31);

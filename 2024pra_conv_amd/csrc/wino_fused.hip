@@ -47,20 +47,24 @@ __device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t 
 }
 
 // Staging of one thread: up to NU units, a unit = (channel group g, patch row, pixel pair) =
-// 8 channels x 2 adjacent pixels. With W even (D16 = false) a pair starts on an even input
-// column, so it is entirely inside or outside the image and its offset is 4-byte aligned: one
-// 4-byte buffer load per channel. With W odd or a base that is not 4-byte aligned (D16) each
-// pixel is a 2-byte load with its own offset. Loads are never predicated per lane (a whole
-// wave skips a unit set none of its lanes holds): a pixel outside the image, or of a thread
-// slot without a unit, gets a byte offset of 2^31 or more, and a channel >= C lands at or
-// beyond the image's last byte; both are past the resource's range (the image,
-// C*H*W*2 < 2^31 bytes), where buffer loads return zero without accessing memory.
+// 8 channels x 2 adjacent pixels, one 4-byte buffer load per channel. A pair starts on an even
+// patch column. With W even and a 4-byte aligned base (D16 = false) it is entirely inside or
+// outside the image and its offset is 4-byte aligned. With W odd or a base that is not 4-byte
+// aligned (D16) the offset is only 2-byte aligned (the kernel driver runs compute queues in
+// the unaligned access mode, so such a load returns the two pixels), and with W odd the pair
+// at column W - 1 has its second pixel past the right edge: that unit loads the dword ending
+// with its first pixel (offset - 2, never before the image for W >= 3) and its byte selectors
+// take the high half and a zero. Loads are never predicated per lane (a whole wave skips a unit
+// set none of its lanes holds): a pixel outside the image, or of a thread slot without a
+// unit, gets a byte offset of 2^31 or more, and a channel >= C lands at or beyond the image's
+// last byte; both are past the resource's range (the image, C*H*W*2 < 2^31 bytes), where
+// buffer loads return zero without accessing memory.
 template <int NU>
 struct Staging {
-    uint32_t off[NU];   // byte offset of the unit's first pixel in channel g*8 of chunk 0
-    uint32_t off1[NU];  // D16: byte offset of its second pixel
-    int lds[NU];        // LDS offset (halves) of the unit's first pixel
-    uint32_t w[NU][8];  // loaded pairs (first pixel in the low half)
+    uint32_t off[NU];             // byte offset of the unit's load in channel g*8 of chunk 0
+    uint32_t sel0[NU], sel1[NU];  // D16: v_perm selectors of the pair's first and second pixel
+    int lds[NU];                  // LDS offset (halves) of the unit's first pixel
+    uint32_t w[NU][8];            // loaded pairs (first pixel in the low half)
 };
 
 // FP16 x - y as fma(y, m1, x) with m1 = -1: y * -1 is exact, so the single rounding of the
@@ -89,8 +93,11 @@ __device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw
         const bool pix = row && iw >= 0 && iw < W;
         st.off[i] = pix ? (uint32_t)(((g * 8 * H + ih) * W + iw) * 2) : 0x80000000u;
         if (D16) {
-            const bool pix1 = row && iw + 1 >= 0 && iw + 1 < W;
-            st.off1[i] = pix1 ? (uint32_t)(((g * 8 * H + ih) * W + iw + 1) * 2) : 0x80000000u;
+            // iw is even, so only the pair at column W - 1 (W odd) straddles the image edge
+            const bool straddle = pix && iw + 1 >= W;
+            if (straddle) st.off[i] -= 2u;
+            st.sel0[i] = straddle ? 0x07060302u : 0x05040100u;
+            st.sel1[i] = straddle ? 0x0C0C0C0Cu : 0x07060302u;  // selector 0x0C: a zero byte
         }
         // pair m: pixel 2m at column parity 0, pixel 2m+1 at parity 1, both at column m; a
         // thread slot without a unit stores to a dummy pixel pair past both patch buffers
@@ -99,41 +106,29 @@ __device__ inline void stage_setup(const mykernelParamType& prm, int ih0, int iw
 }
 
 // Loads units first .. first + N - 1 of the thread's staging.
-template <int N, bool D16, int NU>
+template <int N, int NU>
 __device__ inline void stage_load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes,
                                   Staging<NU>& st, int first) {
 #pragma unroll
     for (int i = first; i < first + N; ++i) {
         const uint32_t base = st.off[i] + chunk_bytes;
-        if (D16) {
-            const uint32_t base1 = st.off1[i] + chunk_bytes;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b16(in, (int)(base + e * chan_bytes), 0, 0);
-                const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b16(in, (int)(base1 + e * chan_bytes), 0, 0);
-                st.w[i][e] = lo | (hi << 16);
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                st.w[i][e] = __builtin_amdgcn_raw_buffer_load_b32(in, (int)(base + e * chan_bytes), 0, 0);
-        }
+        for (int e = 0; e < 8; ++e)
+            st.w[i][e] = __builtin_amdgcn_raw_buffer_load_b32(in, (int)(base + e * chan_bytes), 0, 0);
     }
 }
 
-template <int NU>
+template <int NU, bool D16>
 __device__ inline void stage_store(const Staging<NU>& st, half_t* buf, int half_pc) {
 #pragma unroll
     for (int i = 0; i < NU; ++i) {
         u32x4 p0, p1;  // pixel 2m and 2m+1, 8 channels each
-        p0[0] = pack_lo(st.w[i][0], st.w[i][1]);
-        p0[1] = pack_lo(st.w[i][2], st.w[i][3]);
-        p0[2] = pack_lo(st.w[i][4], st.w[i][5]);
-        p0[3] = pack_lo(st.w[i][6], st.w[i][7]);
-        p1[0] = pack_hi(st.w[i][0], st.w[i][1]);
-        p1[1] = pack_hi(st.w[i][2], st.w[i][3]);
-        p1[2] = pack_hi(st.w[i][4], st.w[i][5]);
-        p1[3] = pack_hi(st.w[i][6], st.w[i][7]);
+        const uint32_t s0 = D16 ? st.sel0[i] : 0x05040100u, s1 = D16 ? st.sel1[i] : 0x07060302u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            p0[k] = __builtin_amdgcn_perm(st.w[i][2 * k + 1], st.w[i][2 * k], s0);
+            p1[k] = __builtin_amdgcn_perm(st.w[i][2 * k + 1], st.w[i][2 * k], s1);
+        }
         *(u32x4*)(buf + st.lds[i]) = p0;
         *(u32x4*)(buf + st.lds[i] + half_pc * 8) = p1;
     }
@@ -274,15 +269,15 @@ struct PatchStager {
             // 648-816 pair units: the second unit set (threads + 512) ends in wave 2-4; the
             // later waves skip its loads (wave-uniform branch; their stores go to the dummy pair)
             constexpr int nunits = 4 * PR * (PC / 2);
-            stage_load<1, d16>(in, chunk_bytes, chan_bytes, p, 0);
+            stage_load<1>(in, chunk_bytes, chan_bytes, p, 0);
             if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 + 512 < nunits)
-                stage_load<1, d16>(in, chunk_bytes, chan_bytes, p, 1);
+                stage_load<1>(in, chunk_bytes, chan_bytes, p, 1);
         }
     }
     __device__ void store(half_t* buf) const {
         if constexpr (nhwc) nhwc_store(n, buf);
         else if constexpr (quad) quad_store<PC>(q, buf);
-        else stage_store<NU>(p, buf, PC / 2);
+        else stage_store<NU, d16>(p, buf, PC / 2);
     }
 };
 
@@ -359,15 +354,9 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     half8 m1 = {-1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16};
     asm volatile("" : "+s"(m1));
 
+    // written first by chunk 0's MFMAs, whose accumulator input is the constant zero (no
+    // zeroing instructions per item)
     floatx16 acc[2][2][2];
-#pragma unroll
-    for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-        for (int m = 0; m < 2; ++m)
-#pragma unroll
-            for (int tb = 0; tb < 2; ++tb)
-#pragma unroll
-                for (int e = 0; e < 16; ++e) acc[bb][m][tb][e] = 0.f;
 
     // the next chunk's patch loads are in flight during this chunk's MFMAs; U has one
     // register set (two waves per SIMD hide its L2 latency), reloaded after its last use
@@ -383,7 +372,8 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     st.store(lds);
     __syncthreads();
 
-    auto chunk = [&](int ch) {
+    auto chunk = [&](int ch, auto first_tag) {
+        constexpr bool FIRST = decltype(first_tag)::value;
         const int buf_ofs = (ch & 1) * PB;
         const bool more = ch + 1 < CB;
         // Wait for this chunk's U here, before the next chunk's patch loads are issued: the
@@ -422,8 +412,8 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
                 for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
                     for (int m = 0; m < 2; ++m)
-                        acc[bb][m][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ucur[(ks * 2 + bb) * 2 + m], v[bb],
-                                                                               acc[bb][m][tb], 0, 0, 0);
+                        acc[bb][m][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                            ucur[(ks * 2 + bb) * 2 + m], v[bb], (FIRST && ks == 0) ? floatx16{} : acc[bb][m][tb], 0, 0, 0);
                 __builtin_amdgcn_s_setprio(0);
             }
         }
@@ -433,7 +423,8 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         }
         __syncthreads();
     };
-    for (int ch = 0; ch < CB; ++ch) chunk(ch);
+    chunk(0, BoolTag<true>{});
+    for (int ch = 1; ch < CB; ++ch) chunk(ch, BoolTag<false>{});
 
     // ---- epilogue; LDS Z layout (floats): [a][j][block = m*2 + tb][e/4][lane][4] ----
     // Row a's fold Z_0 = (M0 + M1) + M2, Z_1 = (M1 - M2) - M3 needs both halves of the row. For

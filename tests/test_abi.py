@@ -159,9 +159,9 @@ def test_weights_load_refuses_foreign_files_before_device_access(wc, tmp_path):
 
 def test_form_choice_and_fused_slots(wc):
     """Layers with K <= 384, N*T*K*C < 8e8, or N*T*K*C < 1.2e9 with C >= K take the fused
-    form (slot 2 = the fused kernel, slots 3 and 4 empty), odd widths included; K >= 416
-    beyond that (28x28 C = K = 512 at N = 64, 56x56 C = 512 K = 385, C < K = 512 at 8.2e8)
-    and the direct path stay staged."""
+    form (slot 2 = the fused kernel, slots 3 and 4 empty), odd widths included (but not a
+    one-pixel width); K >= 416 beyond that (28x28 C = K = 512 at N = 64, 56x56 C = 512
+    K = 385, C < K = 512 at 8.2e8) and the direct path stay staged."""
     L = wc.load_library()
     f = ctypes.c_int()
     cases = [((32, 256, 56, 56, 256), {}, wc.FORM_FUSED), ((8, 256, 14, 14, 256), {}, wc.FORM_FUSED),
@@ -172,7 +172,9 @@ def test_form_choice_and_fused_slots(wc):
              ((16, 512, 28, 28, 512), {}, wc.FORM_FUSED), ((8, 512, 28, 28, 512), {}, wc.FORM_FUSED),
              ((32, 512, 28, 28, 512), {}, wc.FORM_STAGED), ((32, 256, 28, 28, 512), {}, wc.FORM_STAGED),
              ((32, 384, 56, 56, 384), {}, wc.FORM_FUSED), ((64, 256, 56, 56, 257), {}, wc.FORM_FUSED),
-             ((2, 16, 9, 9, 16), dict(u=2, v=2), wc.FORM_STAGED)]
+             ((2, 16, 9, 9, 16), dict(u=2, v=2), wc.FORM_STAGED),
+             # a one-pixel-wide image: its 4-byte pair loads would start before the image
+             ((4, 16, 9, 1, 16), {}, wc.FORM_STAGED), ((4, 16, 9, 3, 16), {}, wc.FORM_FUSED)]
     for (n, c, h, w, k), extra, want in cases:
         pb = wc.make_problem(n, c, h, w, k, **extra, ptrs=FAKE)
         assert L.conv2d_form(ctypes.byref(pb), ctypes.byref(f)) == wc.OK and f.value == want, (n, c, h, k)
