@@ -413,10 +413,10 @@ Launch slot_filter(const Geometry& g) {
         L.block = dim3(256, 1, 1);
         return L;
     }
-    // one 4-wave workgroup per U fragment column (32 rows x 16 channels, 16 positions)
+    // one one-wave workgroup per (U fragment column of 32 rows x 16 channels, transform row)
     L.kernel = (const void*)&wino_filter_transform;
-    L.grid = dim3((unsigned)(g.KB32 * 2 * g.CB), 1, 1);
-    L.block = dim3(256, 1, 1);
+    L.grid = dim3((unsigned)(g.KB32 * 2 * g.CB * 4), 1, 1);
+    L.block = dim3(64, 1, 1);
     return L;
 }
 

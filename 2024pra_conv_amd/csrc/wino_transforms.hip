@@ -11,11 +11,12 @@
 // ---------------------------------------------------------------------------------------
 // S1: filter transform.
 //   G = [[1,0,0],[1/2,1/2,1/2],[1/2,-1/2,1/2],[0,0,1]]   (reference preprocessing.cuh:77-81)
-// Workgroup = one fragment column (32 rows k x 16 channels, all 16 positions), 4 waves: the
-// 32 x 16 filters (32 rows of 144 contiguous halves when C % 16 == 0) are staged once in LDS
-// with coalesced 16-byte loads, then wave a produces positions 4a .. 4a+3 of every
-// (k, 8-channel group): each of a lane's 4 stores is one full 16-byte lane slot, so a wave
-// stores whole 1 KiB fragments. KB32 * 2CB workgroups (128 for K = C = 256).
+// One wave per (fragment column, transform row a): lane l takes filter k = kb*32 + l%32 and
+// channel group cb*16 + 8*(l/32) .. +7, reads its 8 filters (72 contiguous halves of KCRS:
+// nine 16-byte loads when C % 8 == 0 and the weights are 16-byte aligned) straight into
+// registers, and stores row a of U for its 8 channels: positions 4a .. 4a+3, one full 16-byte
+// lane slot each, so a wave stores four whole 1 KiB fragments. KB32 * 2CB * 4 one-wave
+// workgroups (512 for K = C = 256): no LDS staging, no barrier, one memory round trip.
 // ---------------------------------------------------------------------------------------
 // Row a of U = G g G^T for one 3x3 filter g (row-major), FP32: t = row a of G g, then
 // u[b] = (t G^T)[b]. The same expressions, in the same order, as every other producer of U.
@@ -32,45 +33,42 @@ __device__ inline void filter_row_fp32(const float g[9], int a, float u[4]) {
     u[3] = t[2];
 }
 
-__global__ void __launch_bounds__(256) wino_filter_transform(mykernelParamType prm) {
-    __shared__ __attribute__((aligned(16))) half_t wl[32 * 144];
-    const int frag = blockIdx.x;
+__global__ void __launch_bounds__(64) wino_filter_transform(mykernelParamType prm) {
     const int KB = prm.KB32, CB16 = 2 * prm.CB;
+    const int a = blockIdx.x & 3;
+    const int frag = blockIdx.x >> 2;
     const int kb = frag % KB;
     const int cb = frag / KB;  // 16-channel block
     const int K = prm.k, C = prm.c;
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x;
+    const int k = kb * 32 + (lane & 31);
+    const int c0 = cb * 16 + (lane >> 5) * 8;
     const half_t* wgt = (const half_t*)prm.pweight;
-    // staging: row r = filter k = kb*32 + r, its channels cb*16 .. +15 as 144 halves (channel
-    // major, 9 taps each), zero outside K and C
+    // taps of channels c0 .. c0+7 of filter k: halves (k*C + c0)*9 .. +71
+    half_t gh[72];
     if ((C & 7) == 0 && (((uintptr_t)wgt) & 15) == 0) {
-        // 18 16-byte vectors per row; vector j holds taps of channel group j / 9 (8 channels);
-        // with C % 8 == 0 a group is entirely inside or outside C
-        for (int i = tid; i < 32 * 18; i += 256) {
-            const int r = i / 18, j = i - (i / 18) * 18;
-            const int k = kb * 32 + r;
-            const int cg = cb * 16 + (j / 9) * 8;
-            half8 v = {};
-            if (k < K && cg < C) v = *(const half8*)(wgt + ((size_t)k * C + cb * 16) * 9 + 8 * j);
-            *(half8*)(wl + r * 144 + 8 * j) = v;
+        // with C % 8 == 0 a channel group is entirely inside or outside C
+        const bool in = k < K && c0 < C;
+        const half8* src = (const half8*)(wgt + ((size_t)(in ? k : 0) * C + (in ? c0 : 0)) * 9);
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const half8 v = in ? src[j] : half8{};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) gh[8 * j + e] = v[e];
         }
     } else {
-        for (int i = tid; i < 32 * 144; i += 256) {
-            const int r = i / 144, e = i - (i / 144) * 144;
-            const int k = kb * 32 + r, c = cb * 16 + e / 9;
-            wl[i] = (k < K && c < C) ? wgt[((size_t)k * C + c) * 9 + e % 9] : (half_t)0.0f;
-        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                gh[9 * e + i] = (k < K && c0 + e < C) ? wgt[((size_t)k * C + c0 + e) * 9 + i] : (half_t)0.0f;
     }
-    __syncthreads();
-    const int a = tid >> 6;
-    const int lane = tid & 63;
-    const half_t* g8 = wl + (lane & 31) * 144 + (lane >> 5) * 72;
     half8 outv[4];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         float g[9], u[4];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] = (float)g8[9 * e + i];
+        for (int i = 0; i < 9; ++i) g[i] = (float)gh[9 * e + i];
         filter_row_fp32(g, a, u);
 #pragma unroll
         for (int b = 0; b < 4; ++b) outv[b][e] = (half_t)u[b];

@@ -94,7 +94,7 @@ def test_getkernelinfo_fills_slots_and_param(wc):
         assert s["kernel"] and all(s["grid"]) and all(s["block"])
     # this layer (K = 256 <= 320) takes the fused form: filter transform, fused kernel, two empty
     assert prm.form == wc.FORM_FUSED
-    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 1, 1) and ki.slot(1)["block"] == (256, 1, 1)
+    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB * 4, 1, 1) and ki.slot(1)["block"] == (64, 1, 1)
     assert ki.slot(2)["grid"] == (32 * prm.fnbh * prm.fnbw * 4, 1, 1) and ki.slot(2)["block"] == (512, 1, 1)
     assert ki.slot(3)["grid"] == (1, 1, 1) and ki.slot(4)["grid"] == (1, 1, 1)
 
@@ -111,7 +111,7 @@ def test_getkernelinfo_staged_slots(wc):
     # one-dimensional grid over (column block, k-block) pairs, remapped per XCD in the kernel
     assert ki.slot(3)["grid"] == (-(-prm.G32 // (cfg["bt"] // 32)) * -(-prm.KB32 // (cfg["bk"] // 32)), 1, 1)
     assert ki.slot(2)["grid"] == (-(-14 // prm.band), 16, 64)
-    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB, 1, 1) and ki.slot(1)["block"] == (256, 1, 1)
+    assert ki.slot(1)["grid"] == (prm.KB32 * 2 * prm.CB * 4, 1, 1) and ki.slot(1)["block"] == (64, 1, 1)
 
 
 def test_weight_hash_is_fnv1a64():
