@@ -873,13 +873,16 @@ int conv2d_f43_select_tile_config(const problem_t* pb, int* cfg) {
 
 // ---- FP8 compensated form ----------------------------------------------------------------
 
-// FP8 tile heuristic: on MI355X the 32-row tiles measured fastest (more workgroups per
-// launch); 32x128 once it gives two workgroups per compute unit, else 32x64.
+// FP8 tile heuristic (LDS-DMA pipeline, measured on MI355X): the 128x128 tile of eight waves
+// once it gives every compute unit a workgroup (N=64 C=K=512 28x28: 184 us against 238-347 us
+// for the smaller tiles), else 64x64 with one workgroup per unit, else 32x64.
 static int fp8_heuristic_config(const problem_t* pb) {
     Geometry g;
-    if (derive_geometry(pb, &g)) return 3;
-    const long long items = (long long)g.KB32 * ((g.G32 + 3) / 4);
-    return items >= 2LL * compute_units() ? 3 : 6;
+    if (derive_geometry(pb, &g)) return 0;
+    const long long cus = compute_units();
+    auto items = [&](int bk, int bt) { return (long long)((g.KB32 + bk / 32 - 1) / (bk / 32)) * ((g.G32 + bt / 32 - 1) / (bt / 32)); };
+    if (items(128, 128) >= cus) return 0;
+    return items(64, 64) >= cus ? 4 : 5;
 }
 
 static int fp8_prepare(const problem_t* pb, int cfg, Geometry* g, mykernelParamType* a, int* out_cfg) {

@@ -95,15 +95,17 @@ __global__ void __launch_bounds__(256) wino_filter_fp8(mykernelParamType prm) {
     const int Ehi = block_exponent(m), Elo = lo_exponent(Ehi);
     uint32_t hi[8], lo[8];
     split_hi_lo<32>(v, Ehi, Elo, hi, lo);
-    uint8_t* rec = (uint8_t*)prm.transform_filter + fp8_u8_offset(CB, KB) +
-                   ((size_t)(pos * CQ + (cb >> 1)) * KB + (r >> 5)) * kFp8RecBytes;
+    uint8_t* U8 = (uint8_t*)prm.transform_filter + fp8_u8_offset(CB, KB);
+    const size_t ri = (size_t)(pos * CQ + (cb >> 1)) * KB + (r >> 5);  // record index
+    uint8_t* rec = U8 + ri * kFp8CodeBytes;
+    uint8_t* sc = U8 + fp8_scale_offset(prm.c, KB) + ri * kFp8ScaleBytes;
     uint8_t* part = rec + (cb & 1) * 1024;
     const int l0 = r & 31, l1 = 32 + (r & 31);  // lanes holding channels 0-15 / 16-31
     *(u32x4*)(part + l0 * 16) = u32x4{hi[0], hi[1], hi[2], hi[3]};
     *(u32x4*)(part + l1 * 16) = u32x4{hi[4], hi[5], hi[6], hi[7]};
     *(u32x4*)(part + 2048 + l0 * 16) = u32x4{lo[0], lo[1], lo[2], lo[3]};
     *(u32x4*)(part + 2048 + l1 * 16) = u32x4{lo[4], lo[5], lo[6], lo[7]};
-    *(uint16_t*)(rec + 4096 + ((cb & 1) * 32 + l0) * 2) = (uint16_t)((Ehi + 127) | ((Elo + 127) << 8));
+    *(uint16_t*)(sc + ((cb & 1) * 32 + l0) * 2) = (uint16_t)((Ehi + 127) | ((Elo + 127) << 8));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -173,16 +175,18 @@ __global__ void __launch_bounds__(256) wino_input_transform_fp8(mykernelParamTyp
             uint32_t hi[2], lo[2];
             split_hi_lo<8>(v[pos], Ehi, Elo, hi, lo);
             if (store) {
-                uint8_t* rec = V8 + ((size_t)(pos * CQ + (cb >> 1)) * G32 + (g >> 5)) * kFp8RecBytes;
+                const size_t ri = (size_t)(pos * CQ + (cb >> 1)) * G32 + (g >> 5);  // record index
+                uint8_t* rec = V8 + ri * kFp8CodeBytes;
+                uint8_t* sc = V8 + fp8_scale_offset(prm.c, G32) + ri * kFp8ScaleBytes;
                 uint8_t* dst = rec + part * 1024 + lcode * 16 + (q & 1) * 8;
                 *(uint2*)dst = make_uint2(hi[0], hi[1]);
                 *(uint2*)(dst + 2048) = make_uint2(lo[0], lo[1]);
-                if (q == 0) *(uint16_t*)(rec + 4096 + lscale * 2) = (uint16_t)((Ehi + 127) | ((Elo + 127) << 8));
+                if (q == 0) *(uint16_t*)(sc + lscale * 2) = (uint16_t)((Ehi + 127) | ((Elo + 127) << 8));
                 if (pad_half) {
                     uint8_t* z = rec + 1024 + lcode * 16 + (q & 1) * 8;
                     *(uint2*)z = make_uint2(0u, 0u);
                     *(uint2*)(z + 2048) = make_uint2(0u, 0u);
-                    if (q == 0) *(uint16_t*)(rec + 4096 + (32 + (g & 31)) * 2) = (uint16_t)0;
+                    if (q == 0) *(uint16_t*)(sc + (32 + (g & 31)) * 2) = (uint16_t)0;
                 }
             }
         }
@@ -190,26 +194,38 @@ __global__ void __launch_bounds__(256) wino_input_transform_fp8(mykernelParamTyp
 }
 
 // ---------------------------------------------------------------------------------------
-// Three-pass FP8 GEMM with the fused output transform. Same structure as the FP16 GEMM
-// (workgroup tile BK x BT, waves WK x WT with 32x32 sub-tiles, branch-free copies with a
-// register ring D steps deep, two LDS stages, register double-buffered operands); one
-// pipeline step = one 64-channel block of one position = UF + VF records.
+// Three-pass FP8 GEMM with the fused output transform, on the LDS-DMA pipeline of the FP16
+// GEMM (workgroup tile BK x BT, waves WK x WT with 32x32 sub-tiles). One step = one 64-channel
+// block of one position = UF + VF records. A step's record codes (four 1 KiB units each) go
+// global -> LDS with global_load_lds_dwordx4, every wave moving the same number of whole units,
+// and its (UF + VF) x 128 scale bytes with one global_load_lds_dword per wave (64 lanes x 4
+// bytes, each lane fetching its own word of the step's scale list); S LDS stages, S - 2 steps
+// in flight, the waits counted by hand (every wave issues exactly CPW + 1 such loads per step
+// and no other vector-memory loads inside the loop).
 // ---------------------------------------------------------------------------------------
 namespace {
 
-template <int BK, int BT, int WK, int WT>
+template <int BK, int BT, int WK, int WT, int S>
 struct Fp8Shape {
-    static constexpr int kThreads = WK * WT * 64;
-    static constexpr int MI = BK / WK / 32;
-    static constexpr int NI = BT / WT / 32;
-    static constexpr int UF = BK / 32;
-    static constexpr int VF = BT / 32;
-    static constexpr int kRecPieces = kFp8RecBytes / 16;
-    static constexpr int kPieces = (UF + VF) * kRecPieces;
-    static constexpr int kLoads = (kPieces + kThreads - 1) / kThreads;
-    static constexpr int kStageBytes = kLoads * kThreads * 16;
-    static_assert(MI >= 1 && NI >= 1, "wave tile must be at least 32x32");
-    static_assert(BK % (WK * 32) == 0 && BT % (WT * 32) == 0, "tile must divide evenly");
+    static constexpr int kWaves = WK * WT;
+    static constexpr int kThreads = kWaves * 64;
+    static constexpr int MI = BK / WK / 32, NI = BT / WT / 32;
+    static constexpr int UF = BK / 32, VF = BT / 32;
+    static constexpr int kRecs = UF + VF;                    // records per step: U first, then V
+    static constexpr int kCPW = 4 * kRecs / kWaves;          // 1 KiB code units per wave and step
+    static constexpr int kVPW = kCPW + 1;                    // vector-memory loads per wave and step
+    static constexpr int kScaleOfs = kRecs * kFp8CodeBytes;  // a stage: codes, then the scale words
+    static constexpr int kStageBytes = kScaleOfs + kWaves * 256;
+    static constexpr int kSmemBytes = S * kStageBytes;
+    // operand registers double-buffered (the next step's LDS reads overlap this step's MFMAs)
+    // when they fit beside the accumulators and the fold's four output sums
+    static constexpr bool kDB = (MI + NI) * 2 * 17 + MI * NI * 80 <= 200 || kWaves <= 4;
+    static_assert((4 * kRecs) % kWaves == 0, "code units must split evenly over the waves");
+    static_assert(kRecs * kFp8ScaleBytes <= kWaves * 256, "one 256-byte scale load per wave covers the step");
+    static_assert(S >= 4 && 16 % S == 0, "stage count: at least 4, dividing the step count (a multiple of 16)");
+    static_assert((S - 2) * kVPW <= 63, "vmcnt range");
+    static_assert(MI >= 1 && NI >= 1 && BK % (WK * 32) == 0 && BT % (WT * 32) == 0, "tile shape");
+    static_assert(kSmemBytes <= 160 * 1024, "LDS of one compute unit");
 };
 
 struct Fp8Frag {
@@ -217,7 +233,8 @@ struct Fp8Frag {
     int sc;  // byte 0: E_hi + 127, byte 1: E_lo + 127
 };
 
-__device__ inline Fp8Frag read_frag(const uint8_t* rec, int lane) {
+// rec: the record's codes in LDS; sc: its 128 scale bytes
+__device__ inline Fp8Frag read_frag(const uint8_t* rec, const uint8_t* sc, int lane) {
     Fp8Frag f;
     const int4v h0 = *(const int4v*)(rec + lane * 16);
     const int4v h1 = *(const int4v*)(rec + 1024 + lane * 16);
@@ -225,18 +242,28 @@ __device__ inline Fp8Frag read_frag(const uint8_t* rec, int lane) {
     const int4v l1 = *(const int4v*)(rec + 3072 + lane * 16);
     f.hi = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
     f.lo = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
-    f.sc = *(const uint16_t*)(rec + 4096 + lane * 2);
+    f.sc = *(const uint16_t*)(sc + lane * 2);
     return f;
 }
 
+// One global_load_lds_dword: lane i's 4 bytes at gptr land in LDS at lds_byte_addr + 4 i.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ inline void lds_dma4(const void* gptr, uint32_t lds_byte_addr) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off"
+                 :
+                 : "s"(lds_byte_addr), "v"(gptr)
+                 : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
 }  // namespace
 
-template <int BK, int BT, int WK, int WT, int D>
+template <int BK, int BT, int WK, int WT, int S>
 __device__ inline void gemm_fp8_body(const mykernelParamType& prm, int kblk, int gblk, uint8_t* smem) {
-    using S = Fp8Shape<BK, BT, WK, WT>;
-    constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF, NL = S::kLoads;
-    constexpr int RP = S::kRecPieces;
-    static_assert(16 % D == 0 && D % 2 == 0, "ring depth: even, dividing the step count (a multiple of 16)");
+    using Sh = Fp8Shape<BK, BT, WK, WT, S>;
+    constexpr int MI = Sh::MI, NI = Sh::NI, UF = Sh::UF, VF = Sh::VF, CPW = Sh::kCPW, VPW = Sh::kVPW;
+    constexpr bool DB = Sh::kDB;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -249,37 +276,52 @@ __device__ inline void gemm_fp8_body(const mykernelParamType& prm, int kblk, int
     const int nsteps = 16 * CQ;  // (position, 64-channel block) pairs
     const uint8_t* __restrict__ U8 = (const uint8_t*)prm.transform_filter + fp8_u8_offset(prm.CB, KB);
     const uint8_t* __restrict__ V8 = (const uint8_t*)prm.transform_pin;
+    const uint8_t* __restrict__ U8s = U8 + fp8_scale_offset(prm.c, KB);
+    const uint8_t* __restrict__ V8s = V8 + fp8_scale_offset(prm.c, G32);
 
-    // copy slot i: piece pc = tid + i * threads of every step; record (pos, cq, rb) of step
-    // s = pos * CQ + cq lies at s * RB + rb records, so addresses advance by RB records per step
-    const uint8_t* src[NL];
-    size_t stride[NL];
+    // code units of this wave: unit u = wave * CPW + j = record u / 4, 1 KiB part u % 4; record
+    // (pos, cq, rb) of step s = pos * CQ + cq is record s * RB + rb, so addresses advance by RB
+    // records per step; a record beyond the matrix reads record 0 (its rows or columns are never
+    // stored)
+    const uint8_t* csrc[CPW];
+    size_t cstride[CPW];
 #pragma unroll
-    for (int i = 0; i < NL; ++i) {
-        const int pc = tid + i * S::kThreads;
-        const int p = pc < S::kPieces ? pc : 0;
-        const int rec = p / RP, within = p % RP;
-        if (rec < UF) {
-            const bool ok = pc < S::kPieces && kb0 + rec < KB;
-            src[i] = U8 + (ok ? (size_t)(kb0 + rec) * kFp8RecBytes : 0) + (size_t)within * 16;
-            stride[i] = (size_t)KB * kFp8RecBytes;
+    for (int j = 0; j < CPW; ++j) {
+        const int u = wave * CPW + j;
+        const int r = u >> 2, part = u & 3;
+        if (r < UF) {
+            const bool ok = kb0 + r < KB;
+            csrc[j] = U8 + (ok ? (size_t)(kb0 + r) : 0) * kFp8CodeBytes + part * 1024 + lane * 16;
+            cstride[j] = (size_t)KB * kFp8CodeBytes;
         } else {
-            const int rv = rec - UF;
-            const bool ok = pc < S::kPieces && gb0 + rv < G32;
-            src[i] = V8 + (ok ? (size_t)(gb0 + rv) * kFp8RecBytes : 0) + (size_t)within * 16;
-            stride[i] = (size_t)G32 * kFp8RecBytes;
+            const bool ok = gb0 + r - UF < G32;
+            csrc[j] = V8 + (ok ? (size_t)(gb0 + r - UF) : 0) * kFp8CodeBytes + part * 1024 + lane * 16;
+            cstride[j] = (size_t)G32 * kFp8CodeBytes;
         }
     }
-
-    u32x4 ring[D][NL];
-    auto gload = [&](u32x4 (&slot)[NL], int step) {
+    // scale word d = wave * 64 + lane of the step's list (record d / 32, word d % 32); words past
+    // the step's records re-read record 0 of U (their LDS slots are never read)
+    const int sd = wave * 64 + lane, sr = sd >> 5, sw = sd & 31;
+    const uint8_t* ssrc;
+    size_t sstride;
+    if (sr < UF || sr >= UF + VF) {
+        const bool ok = sr < UF && kb0 + sr < KB;
+        ssrc = U8s + (ok ? (size_t)(kb0 + sr) : 0) * kFp8ScaleBytes + sw * 4;
+        sstride = (size_t)KB * kFp8ScaleBytes;
+    } else {
+        const bool ok = gb0 + sr - UF < G32;
+        ssrc = V8s + (ok ? (size_t)(gb0 + sr - UF) : 0) * kFp8ScaleBytes + sw * 4;
+        sstride = (size_t)G32 * kFp8ScaleBytes;
+    }
+    const uint32_t lds0 = lds_address((const half_t*)smem);
+    auto dma = [&](int step, int stage) {
         const size_t s = (size_t)min(step, nsteps - 1);
+        const uint32_t sb = lds0 + (uint32_t)(stage * Sh::kStageBytes);
 #pragma unroll
-        for (int i = 0; i < NL; ++i) slot[i] = *(const u32x4*)(src[i] + s * stride[i]);
-    };
-    auto lstore = [&](const u32x4 (&slot)[NL], int buf) {
-#pragma unroll
-        for (int i = 0; i < NL; ++i) *(u32x4*)(smem + buf * S::kStageBytes + (tid + i * S::kThreads) * 16) = slot[i];
+        for (int j = 0; j < CPW; ++j)
+            lds_dma16((const half_t*)(csrc[j] + s * cstride[j]),
+                      __builtin_amdgcn_readfirstlane(sb + (uint32_t)((wave * CPW + j) * 1024)));
+        lds_dma4(ssrc + s * sstride, __builtin_amdgcn_readfirstlane(sb + (uint32_t)(Sh::kScaleOfs + wave * 256)));
     };
 
     floatx16 acc[MI][NI];
@@ -294,34 +336,45 @@ __device__ inline void gemm_fp8_body(const mykernelParamType& prm, int kblk, int
             for (int q = 0; q < 4; ++q) Y[q][mi][ni] = acc[mi][ni];
         }
 
-    Fp8Frag ra[2][MI], rb[2][NI];
-    auto lread = [&](Fp8Frag (&A)[MI], Fp8Frag (&B)[NI], int buf) {
-        const uint8_t* sb = smem + buf * S::kStageBytes;
+    Fp8Frag ra[DB ? 2 : 1][MI], rb[DB ? 2 : 1][NI];
+    auto lread = [&](Fp8Frag (&A)[MI], Fp8Frag (&B)[NI], int stage) {
+        const uint8_t* sb = smem + stage * Sh::kStageBytes;
+        const uint8_t* ss = sb + Sh::kScaleOfs;
 #pragma unroll
-        for (int mi = 0; mi < MI; ++mi) A[mi] = read_frag(sb + (wk * MI + mi) * kFp8RecBytes, lane);
+        for (int mi = 0; mi < MI; ++mi) {
+            const int r = wk * MI + mi;
+            A[mi] = read_frag(sb + r * kFp8CodeBytes, ss + r * kFp8ScaleBytes, lane);
+        }
 #pragma unroll
-        for (int ni = 0; ni < NI; ++ni) B[ni] = read_frag(sb + (UF + wt * NI + ni) * kFp8RecBytes, lane);
+        for (int ni = 0; ni < NI; ++ni) {
+            const int r = UF + wt * NI + ni;
+            B[ni] = read_frag(sb + r * kFp8CodeBytes, ss + r * kFp8ScaleBytes, lane);
+        }
     };
 
-    // pipeline as in the FP16 GEMM (regs(s) feed the MFMAs while regs(s+1) are read from LDS,
-    // LDS[s & 1] is refilled with step s+2, and the ring reloads step s+2+D)
-    gload(ring[0], 0);
-    lstore(ring[0], 0);
-    gload(ring[0], D);
+    // Pipeline (stage = step % S):
+    //   prologue: DMA steps 0..S-2; wait for step 0; barrier; (DB) regs <- stage 0
+    //   iteration s: wait for steps <= s+1 (S-3 steps stay in flight) | barrier (stage s+1
+    //   complete for every wave; every wave's reads of stage s-1 done) | DMA step s+S-1 into
+    //   stage s-1 | DB: regs(s+1) <- stage s+1, MFMAs on regs(s); else regs <- stage s, MFMAs |
+    //   fold at a position's end
 #pragma unroll
-    for (int d = 1; d < D; ++d) gload(ring[d], d);
+    for (int p = 0; p < S - 1; ++p) dma(p, p);
+    wait_vm<(S - 2) * VPW>();
     __syncthreads();
-    lread(ra[0], rb[0], 0);
-    lstore(ring[1 % D], 1);
-    gload(ring[1 % D], 1 + D);
-    __syncthreads();
+    if constexpr (DB) lread(ra[0], rb[0], 0);
 
     int pos = 0, cq = 0;
-    for (int s0 = 0; s0 < nsteps; s0 += D) {
+    for (int s0 = 0; s0 < nsteps; s0 += S) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
+        for (int d = 0; d < S; ++d) {
             const int s = s0 + d;
-            const int cur = d & 1;
+            const int cur = DB ? (d & 1) : 0;  // S is even, so s & 1 == d & 1
+            wait_vm<(S - 3) * VPW>();
+            __syncthreads();
+            dma(s + S - 1, (d + S - 1) % S);
+            if constexpr (DB) lread(ra[cur ^ 1], rb[cur ^ 1], (d + 1) % S);
+            else lread(ra[0], rb[0], d);
 #pragma unroll
             for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -334,10 +387,6 @@ __device__ inline void gemm_fp8_body(const mykernelParamType& prm, int kblk, int
                     c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a.lo, b.hi, c, 0, 0, 1, a.sc, 0, b.sc);
                     acc[mi][ni] = c;
                 }
-            lread(ra[cur ^ 1], rb[cur ^ 1], cur ^ 1);
-            lstore(ring[(d + 2) % D], cur);
-            gload(ring[(d + 2) % D], s + 2 + D);
-            __syncthreads();
             if (++cq == CQ) {
                 fold_position(pos, acc, Y);
 #pragma unroll
@@ -351,35 +400,35 @@ __device__ inline void gemm_fp8_body(const mykernelParamType& prm, int kblk, int
             }
         }
     }
+    wait_vm<0>();  // the trailing (re-read) DMAs land before the workgroup's LDS is released
 
     store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
 
 template <int BK, int BT, int WK, int WT>
-constexpr int kFp8WavesPerEU = (BK / WK / 32) * (BT / WT / 32) >= 4 || WK * WT <= 4 ? 1 : 2;
+constexpr int kFp8WavesPerEU = WK * WT >= 8 ? 2 : 1;
 
-template <int BK, int BT, int WK, int WT, int D>
+template <int BK, int BT, int WK, int WT, int S>
 __global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kFp8WavesPerEU<BK, BT, WK, WT>)))
 wino_gemm_fp8_kernel(mykernelParamType prm) {
-    using S = Fp8Shape<BK, BT, WK, WT>;
-    __shared__ __attribute__((aligned(16))) uint8_t smem[2 * S::kStageBytes];
-    const int nk = (prm.KB32 + S::UF - 1) / S::UF;
-    const int ng = (prm.G32 + S::VF - 1) / S::VF;
+    using Sh = Fp8Shape<BK, BT, WK, WT, S>;
+    __shared__ __attribute__((aligned(16))) uint8_t smem[Sh::kSmemBytes];
+    const int nk = (prm.KB32 + Sh::UF - 1) / Sh::UF;
+    const int ng = (prm.G32 + Sh::VF - 1) / Sh::VF;
     int kblk, gblk;
     xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
-    gemm_fp8_body<BK, BT, WK, WT, D>(prm, kblk, gblk, smem);
+    gemm_fp8_body<BK, BT, WK, WT, S>(prm, kblk, gblk, smem);
 }
 
-#define FP8_CFG(BK, BT, WK, WT, D) \
-    { BK, BT, WK * WT * 64, D, (const void*)&wino_gemm_fp8_kernel<BK, BT, WK, WT, D>, nullptr }
+#define FP8_CFG(BK, BT, WK, WT, S) \
+    { BK, BT, WK * WT * 64, S, (const void*)&wino_gemm_fp8_kernel<BK, BT, WK, WT, S>, nullptr }
 
 const GemmConfig kFp8Configs[] = {
-    FP8_CFG(128, 64, 2, 2, 2),   // 0: wave 64x32, 4 waves
-    FP8_CFG(64, 128, 1, 4, 2),   // 1: wave 64x32, 4 waves
-    FP8_CFG(64, 128, 2, 2, 2),   // 2: wave 32x64, 4 waves
-    FP8_CFG(32, 128, 1, 4, 2),   // 3: wave 32x32, 4 waves (small K)
-    FP8_CFG(64, 64, 1, 2, 2),    // 4: wave 64x32, 2 waves (small problems)
-    FP8_CFG(128, 64, 2, 2, 4),   // 5
-    FP8_CFG(32, 64, 1, 2, 2),    // 6: wave 32x32, 2 waves (small problems)
+    FP8_CFG(128, 128, 2, 4, 4),  // 0: wave 64x32, 8 waves
+    FP8_CFG(128, 64, 2, 2, 4),   // 1: wave 64x32, 4 waves
+    FP8_CFG(64, 128, 1, 4, 4),   // 2: wave 64x32, 4 waves
+    FP8_CFG(32, 128, 1, 4, 4),   // 3: wave 32x32, 4 waves (small K)
+    FP8_CFG(64, 64, 1, 2, 4),    // 4: wave 64x32, 2 waves (small problems)
+    FP8_CFG(32, 64, 1, 2, 4),    // 5: wave 32x32, 2 waves (small problems)
 };
 const int kNumFp8Configs = (int)(sizeof(kFp8Configs) / sizeof(kFp8Configs[0]));

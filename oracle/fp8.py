@@ -155,26 +155,30 @@ def input_transform_f32(x16: np.ndarray, geo: wg.Geometry) -> np.ndarray:
 # 32-channel block, its bytes 16-31 the same channels of the second block; the E8M0 scale
 # of (row r, 32-channel block kb) is supplied by lane r + 32 * kb (op_sel picks the byte of
 # that lane's scale register). One record per (pos, cq, rb) (cq = 64-channel block, rb =
-# 32-row block), 4224 bytes:
+# 32-row block): 4096 code bytes
 #   [0, 1024)     codes of block kb = 0: lane * 16 + (c % 16), lane = (c % 32 // 16) * 32 + r % 32
 #   [1024, 2048)  codes of block kb = 1, same arrangement            (hi part)
 #   [2048, 4096)  lo codes, same arrangement
-#   [4096, 4224)  scales: lane * 2 -> E_hi + 127, lane * 2 + 1 -> E_lo + 127, lane = kb * 32 + r % 32
-# Records are ordered [pos][cq][rb]. Each 1 KiB half is one contiguous run across the wave,
-# so a lane's operand is two conflict-free 16-byte LDS reads.
-REC_BYTES = 4224
+# and 128 scale bytes: lane * 2 -> E_hi + 127, lane * 2 + 1 -> E_lo + 127, lane = kb * 32 + r % 32.
+# The codes of all records come first, ordered [pos][cq][rb] (each 1 KiB unit one contiguous run
+# across the wave, so a lane's operand is conflict-free 16-byte LDS reads and a record moves as
+# four whole 1 KiB LDS-DMA units), then the scales of all records in the same order.
+CODE_BYTES = 4096
+SCALE_BYTES = 128
+REC_BYTES = CODE_BYTES + SCALE_BYTES  # bytes per record over both regions
 
 
 def pack_operand(q: dict, rows_blocks: int) -> np.ndarray:
-    """Workspace image of a quantised operand (dict from quantize_U / quantize_V)."""
+    """Workspace image of a quantised operand (dict from quantize_U / quantize_V): the code
+    region, then the scale region."""
     P, R, NB, _ = q["hi"].shape
     CQ = NB // 2
     RB = rows_blocks
-    out = np.zeros((P, CQ, RB, REC_BYTES), dtype=np.uint8)
+    codes = np.zeros((P, CQ, RB, CODE_BYTES), dtype=np.uint8)
 
-    def halves(codes):  # (P, R, NB, 32) -> (P, CQ, RB, kb 2, lane 64, 16 bytes)
+    def halves(c8):  # (P, R, NB, 32) -> (P, CQ, RB, kb 2, lane 64, 16 bytes)
         full = np.zeros((P, RB * 32, NB, 32), dtype=np.uint8)
-        full[:, :R] = codes
+        full[:, :R] = c8
         # (P, rb, r32, cq, kb, hl, b16) -> (P, cq, rb, kb, hl, r32, b16)
         x = full.reshape(P, RB, 32, CQ, 2, 2, 16).transpose(0, 3, 1, 4, 5, 2, 6)
         return x.reshape(P, CQ, RB, 2048)
@@ -185,11 +189,18 @@ def pack_operand(q: dict, rows_blocks: int) -> np.ndarray:
         x = full.reshape(P, RB, 32, CQ, 2).transpose(0, 3, 1, 4, 2)  # P cq rb kb r32
         return x.reshape(P, CQ, RB, 64).astype(np.uint8)
 
-    out[..., 0:2048] = halves(q["hi"])
-    out[..., 2048:4096] = halves(q["lo"])
+    codes[..., 0:2048] = halves(q["hi"])
+    codes[..., 2048:4096] = halves(q["lo"])
     sc = np.stack([lane_exps(q["eh"]), lane_exps(q["el"])], axis=-1)  # (P, CQ, RB, 64, 2)
-    out[..., 4096:] = sc.reshape(P, CQ, RB, 128)
-    return out.reshape(-1)
+    return np.concatenate([codes.reshape(-1), sc.reshape(-1)])
+
+
+def split_regions(flat: np.ndarray, c: int, rows_blocks: int):
+    """(codes (P, CQ, RB, 4096), scales (P, CQ, RB, 128)) views of an operand image."""
+    CQ = (c + 63) // 64
+    n = 16 * CQ * rows_blocks
+    return (flat[:n * CODE_BYTES].reshape(16, CQ, rows_blocks, CODE_BYTES),
+            flat[n * CODE_BYTES:n * REC_BYTES].reshape(16, CQ, rows_blocks, SCALE_BYTES))
 
 
 def operand_bytes(c: int, rows_blocks: int) -> int:

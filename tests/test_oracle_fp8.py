@@ -76,23 +76,28 @@ def test_record_layout_offsets():
     q = fp8.quantize_U(vals.astype(np.float16))
     flat = fp8.pack_operand(q, rows_blocks=2)
     CQ = 3
-    assert flat.size == fp8.operand_bytes(C, 2) == P * CQ * 2 * fp8.REC_BYTES
+    nrec = P * CQ * 2
+    assert flat.size == fp8.operand_bytes(C, 2) == nrec * (4096 + 128) == nrec * fp8.REC_BYTES
+    scale0 = nrec * 4096  # the scale region follows every record's codes
     for (pos, r, c) in [(5, 17, 130), (0, 0, 0), (15, 39, 149), (3, 33, 63), (9, 2, 64)]:
         cq, kb, b = c // 64, (c % 64) // 32, c % 32
         lane = (b // 16) * 32 + r % 32
-        rec = ((pos * CQ + cq) * 2 + r // 32) * fp8.REC_BYTES
-        off = rec + kb * 1024 + lane * 16 + b % 16
+        ri = (pos * CQ + cq) * 2 + r // 32
+        off = ri * 4096 + kb * 1024 + lane * 16 + b % 16
         assert flat[off] == q["hi"][pos, r, c // 32, b]
         assert flat[off + 2048] == q["lo"][pos, r, c // 32, b]
         slane = kb * 32 + r % 32
-        assert flat[rec + 4096 + 2 * slane] == q["eh"][pos, r, c // 32] + 127
-        assert flat[rec + 4096 + 2 * slane + 1] == q["el"][pos, r, c // 32] + 127
+        assert flat[scale0 + ri * 128 + 2 * slane] == q["eh"][pos, r, c // 32] + 127
+        assert flat[scale0 + ri * 128 + 2 * slane + 1] == q["el"][pos, r, c // 32] + 127
     # padding rows (40..63) and channels (160..191: a whole padding 32-block) are zero codes
     # with zero scale bytes
-    rec = ((2 * CQ + 2) * 2 + 1) * fp8.REC_BYTES  # pos 2, cq 2, rb 1
+    ri = (2 * CQ + 2) * 2 + 1  # pos 2, cq 2, rb 1
     for lane in range(64):
-        assert not flat[rec + 1024 + lane * 16:rec + 1024 + lane * 16 + 16].any()
-    assert flat[rec + 4096 + 2 * (32 + 45 % 32)] == 0
+        assert not flat[ri * 4096 + 1024 + lane * 16:ri * 4096 + 1024 + lane * 16 + 16].any()
+    assert flat[scale0 + ri * 128 + 2 * (32 + 45 % 32)] == 0
+    codes, scales = fp8.split_regions(flat, C, 2)
+    assert codes.shape == (16, 3, 2, 4096) and scales.shape == (16, 3, 2, 128)
+    assert scales[5, 2, 0, 2 * 17] == q["eh"][5, 17, 130 // 32] + 127  # c = 130: block kb = 0 of cq = 2
 
 
 def test_two_passes_miss_and_three_meet_the_bound():
