@@ -31,26 +31,39 @@ __device__ inline int block_exponent(float m) {
     return E < -127 ? -127 : (E > 127 ? 127 : E);
 }
 
-// Four values (already scaled into [-448, 448]) -> four e4m3 codes, nearest-even.
-__device__ inline uint32_t pack4_e4m3(float a, float b, float c, float d) {
-    const uint32_t w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0u, false);
-    return __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+typedef short short2v __attribute__((ext_vector_type(2)));
+
+// The FP32 scale 2^E of an exponent pair member. Only all-zero blocks get E = -127 (every
+// nonzero transformed value is at least 2^-24, so E >= -37); their codes are zero for any
+// scale, so the scale is kept at the smallest normal power of two.
+__device__ inline float exp_scale(int E) { return ldexpf(1.f, E < -126 ? -126 : E); }
+
+// Four values -> four e4m3 codes of v / s, nearest-even (s a power of two, so the scaling is
+// exact; gfx950's scaled conversion gives the same codes as scaling then converting, checked
+// on 4M values with ties, subnormals and zeros of both signs).
+__device__ inline uint32_t pack4_e4m3(float a, float b, float c, float d, float s) {
+    short2v w = {0, 0};
+    w = __builtin_amdgcn_cvt_scalef32_pk_fp8_f32(w, a, b, s, false);
+    w = __builtin_amdgcn_cvt_scalef32_pk_fp8_f32(w, c, d, s, true);
+    return __builtin_bit_cast(uint32_t, w);
 }
 
-// Splits n values (multiple of 4) into hi / lo code words given the block exponents.
+// Splits n values (multiple of 4) into hi / lo code words given the block exponents: hi codes
+// of v / 2^E_hi, then lo codes of the residual (v - hi * 2^E_hi) / 2^E_lo; the residual is exact
+// in FP32, and the scaled decode equals decoding then scaling (same check).
 template <int N>
 __device__ inline void split_hi_lo(const float (&v)[N], int Ehi, int Elo, uint32_t (&hi)[N / 4],
                                    uint32_t (&lo)[N / 4]) {
+    const float shi = exp_scale(Ehi), slo = exp_scale(Elo);
 #pragma unroll
     for (int q = 0; q < N / 4; ++q) {
-        hi[q] = pack4_e4m3(ldexpf(v[4 * q], -Ehi), ldexpf(v[4 * q + 1], -Ehi), ldexpf(v[4 * q + 2], -Ehi),
-                           ldexpf(v[4 * q + 3], -Ehi));
-        const float h[4] = {__builtin_amdgcn_cvt_f32_fp8((int)hi[q], 0), __builtin_amdgcn_cvt_f32_fp8((int)hi[q], 1),
-                            __builtin_amdgcn_cvt_f32_fp8((int)hi[q], 2), __builtin_amdgcn_cvt_f32_fp8((int)hi[q], 3)};
-        float r[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) r[e] = ldexpf(v[4 * q + e] - ldexpf(h[e], Ehi), -Elo);
-        lo[q] = pack4_e4m3(r[0], r[1], r[2], r[3]);
+        hi[q] = pack4_e4m3(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3], shi);
+        const int h = (int)hi[q];
+        const float r0 = v[4 * q] - __builtin_amdgcn_cvt_scalef32_f32_fp8(h, shi, 0);
+        const float r1 = v[4 * q + 1] - __builtin_amdgcn_cvt_scalef32_f32_fp8(h, shi, 1);
+        const float r2 = v[4 * q + 2] - __builtin_amdgcn_cvt_scalef32_f32_fp8(h, shi, 2);
+        const float r3 = v[4 * q + 3] - __builtin_amdgcn_cvt_scalef32_f32_fp8(h, shi, 3);
+        lo[q] = pack4_e4m3(r0, r1, r2, r3, slo);
     }
 }
 

@@ -27,8 +27,13 @@ SHAPES = [
     (2, 96, 28, 28, 256, 1, 1),    # four k-blocks, ragged blocks of a 14x14 tile grid
     (1, 8, 4, 130, 32, 1, 1),      # 2x65 tiles -> 2x32 blocks (the widest patch: 816 units)
     (1, 512, 12, 12, 64, 1, 1),    # 16 channel chunks (the double-buffered patch alternates 16x)
-    # odd widths: 2-byte staging loads, pixel pairs straddling the right edge
+    # odd widths: pixel-pair loads at 2-byte alignment, the pair at the right edge loading the
+    # dword that ends with its pixel (widths 3 and 5: that dword starts one pixel earlier in the
+    # same row; odd H*W puts every other channel plane on an odd offset)
     (1, 16, 9, 11, 8, 1, 1),
+    (2, 40, 5, 3, 24, 1, 1),
+    (1, 16, 7, 5, 8, 0, 0),
+    (1, 16, 3, 3, 8, 2, 2),
     (2, 96, 27, 27, 96, 1, 1),
     (1, 40, 13, 13, 20, 0, 0),
     (1, 8, 4, 131, 32, 2, 2),      # odd width with 2x32 blocks and pad 2
@@ -56,7 +61,7 @@ SHAPES = [
 
 
 def _load_width(w, prm):
-    """Staging load width in bytes: 2 for odd widths, else 8 when the patch takes one quad
+    """Staging load width label: 2 for odd widths (pixel-pair loads at 2-byte alignment), else 8 when the patch takes one quad
     unit per thread (fnu = 1) and 4 for pixel pairs (always two units per thread)."""
     return 2 if w % 2 else (8 if prm.fnu == 1 else 4)
 
