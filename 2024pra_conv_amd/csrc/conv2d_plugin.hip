@@ -116,7 +116,7 @@ int derive_geometry_io(const problem_t* pb, int io, Geometry* g) {
         // dword before the image's first pixel)
         g->fused_ok = (!nhwc_in || nhwc_ok) && g->fnu <= (nhwc_in ? kFusedNhwcUnits : 2) &&
                       (g->flw != kFusedLoad2 || pb->w != 1) &&
-                      fused_kernel_ptr(g->flw, fused_shape_index(g->fbh, g->fbw), g->fsh) &&
+                      fused_kernel_ptr(g->flw, fused_shape_index(g->fbh, g->fbw), g->fsh, fused_tail(g->KB32)) &&
                       g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
                       out_img_bytes < (1LL << 31) && (long long)g->u_bytes < (1LL << 31);
     }
@@ -431,7 +431,7 @@ Launch slot_filter(const Geometry& g) {
 // input patches during the channel loop and the 128 KiB Z exchange after it.
 Launch slot_fused(const Geometry& g) {
     Launch L;
-    L.kernel = fused_kernel_ptr(g.flw, fused_shape_index(g.fbh, g.fbw), g.fsh);
+    L.kernel = fused_kernel_ptr(g.flw, fused_shape_index(g.fbh, g.fbw), g.fsh, fused_tail(g.KB32));
     L.grid = dim3((unsigned)g.fitems, 1, 1);
     L.block = dim3((unsigned)fused_threads(), 1, 1);
     const unsigned patches = 2u * 4u * (unsigned)g.fpr * (unsigned)g.fpc * 16u;

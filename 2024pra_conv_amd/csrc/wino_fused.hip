@@ -288,9 +288,11 @@ struct PatchStager {
 // workgroup, in the launch's dynamic LDS, with staging load width LW. The patch is
 // PR = 2*FBH + 2 rows by PC columns starting FSH columns left of 2*tw0 - q
 // (fused_patch_cols, fused_col_shift), so every LDS offset except the lane's two bases
-// (window rows rp, rm) and the chunk's buffer is a compile-time constant.
-template <int LW, int BH, int FBH, int FBW, int FSH>
-__device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gblk) {
+// (window rows rp, rm) and the chunk's buffer is a compile-time constant. HALF: the block's
+// second 32 output channels lie beyond K (the last block when K % 64 is 1..32); their U loads,
+// MFMAs and epilogue work are left out.
+template <int LW, int BH, int FBH, int FBW, int FSH, bool HALF>
+__device__ __attribute__((always_inline)) inline void fused_item(const mykernelParamType& prm, int kblk, int gblk) {
     constexpr int PR = fused_patch_rows(FBH), PC = fused_patch_cols(LW, FBW, FSH);
     constexpr int PB = 4 * PR * PC * 8;  // halves per patch buffer
     static_assert(FBH * FBW == 64, "64 tiles per block");
@@ -325,7 +327,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
-                for (int m = 0; m < 2; ++m) {
+                for (int m = 0; m < (HALF ? 1 : 2); ++m) {
                     const int kb = kblk * 2 + m < KB ? kblk * 2 + m : 0;
                     const uint32_t frag = ((uint32_t)(4 * a + 2 * BH + bb) * C16 + ch * 2 + ks) * KB + kb;
                     const u32x4 raw =
@@ -355,7 +357,8 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     asm volatile("" : "+s"(m1));
 
     // written first by chunk 0's MFMAs, whose accumulator input is the constant zero (no
-    // zeroing instructions per item)
+    // zeroing instructions per item); with HALF the second channel block is never written or
+    // read
     floatx16 acc[2][2][2];
 
     // the next chunk's patch loads are in flight during this chunk's MFMAs; U has one
@@ -411,7 +414,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
 #pragma unroll
                 for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
-                    for (int m = 0; m < 2; ++m)
+                    for (int m = 0; m < (HALF ? 1 : 2); ++m)
                         acc[bb][m][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
                             ucur[(ks * 2 + bb) * 2 + m], v[bb], (FIRST && ks == 0) ? floatx16{} : acc[bb][m][tb], 0, 0, 0);
                 __builtin_amdgcn_s_setprio(0);
@@ -437,8 +440,9 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     auto zslot = [&](int j, int m, int tb, int e4) {
         return Z + ((((a * 2 + j) * 4 + m * 2 + tb) * 4 + e4) * 64 + lane) * 4;
     };
+    constexpr bool kGive = !(HALF && give == 1), kFold = !(HALF && fold == 1);
 #pragma unroll
-    for (int tb = 0; tb < 2; ++tb)
+    for (int tb = 0; tb < (kGive ? 2 : 0); ++tb)
 #pragma unroll
         for (int e4 = 0; e4 < 4; ++e4) {
             floatx4 v0, v1;
@@ -454,7 +458,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
         }
     __syncthreads();
 #pragma unroll
-    for (int tb = 0; tb < 2; ++tb)
+    for (int tb = 0; tb < (kFold ? 2 : 0); ++tb)
 #pragma unroll
         for (int e4 = 0; e4 < 4; ++e4) {
             float* z0p = zslot(0, fold, tb, e4);
@@ -480,6 +484,7 @@ __device__ inline void fused_item(const mykernelParamType& prm, int kblk, int gb
     __syncthreads();
     const int blk = wave >> 1;  // output block of this wave, e/4 groups 2*(wave&1) .. +1
     const int m = blk >> 1, tb = blk & 1;
+    if (HALF && m == 1) return;  // output channels beyond K
     const int t = tb * 32 + (lane & 31);
     const int th = th0 + t / FBW, tw = tw0 + t % FBW;
     // Output of this image through a buffer resource: 32-bit byte offsets (the host checks
@@ -645,22 +650,27 @@ template <> struct BlockShape<1> { static constexpr int h = 4, w = 16; };
 template <> struct BlockShape<2> { static constexpr int h = 16, w = 4; };
 template <> struct BlockShape<3> { static constexpr int h = 2, w = 32; };
 
-// S = 4 * (block shape index) + column shift FSH
-template <int LW, int S>
-__device__ inline void fused_item_dispatch(const mykernelParamType& prm, int kblk, int gblk) {
+// S = 4 * (block shape index) + column shift FSH. With TAIL the item takes the half-block body
+// when its second 32 channels lie beyond K (wave-uniform choice per item).
+template <int LW, int S, bool TAIL>
+__device__ __attribute__((always_inline)) inline void fused_item_dispatch(const mykernelParamType& prm, int kblk, int gblk) {
     constexpr int FBH = BlockShape<S / 4>::h, FBW = BlockShape<S / 4>::w, FSH = S & 3;
     // waves 0-3 take BH = 0, waves 4-7 BH = 1 (wave-uniform; both paths pass the same barriers)
-    if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 8) == 0)
-        fused_item<LW, 0, FBH, FBW, FSH>(prm, kblk, gblk);
-    else
-        fused_item<LW, 1, FBH, FBW, FSH>(prm, kblk, gblk);
+    const bool bh1 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8) != 0;
+    if (TAIL && kblk * 64 + 32 >= prm.k) {
+        if (!bh1) fused_item<LW, 0, FBH, FBW, FSH, true>(prm, kblk, gblk);
+        else fused_item<LW, 1, FBH, FBW, FSH, true>(prm, kblk, gblk);
+    } else {
+        if (!bh1) fused_item<LW, 0, FBH, FBW, FSH, false>(prm, kblk, gblk);
+        else fused_item<LW, 1, FBH, FBW, FSH, false>(prm, kblk, gblk);
+    }
 }
 
-template <int LW, int S>
+template <int LW, int S, bool TAIL>
 __global__ void __launch_bounds__(512, 1) wino_fused_kernel(mykernelParamType prm) {
     int kblk, gblk;
     xcd_remap(blockIdx.x, gridDim.x, (prm.k + 63) / 64, &kblk, &gblk);
-    fused_item_dispatch<LW, S>(prm, kblk, gblk);
+    fused_item_dispatch<LW, S, TAIL>(prm, kblk, gblk);
 }
 
 // Multi-layer form: one launch over the fused items of several layers (descriptor i owns items
@@ -678,14 +688,17 @@ wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, in
     }
     const BatchLayer& bl = layers[lo];
     const int local = item - bl.s3_first;
-    fused_item_dispatch<LW, S>(bl.prm, local % bl.nk, local / bl.nk);
+    fused_item_dispatch<LW, S, true>(bl.prm, local % bl.nk, local / bl.nk);
 }
 
 int fused_threads() { return kFusedThreads; }
 
 // Instances: 2- and 4-byte staging for every block shape with column shift q & 1; 8-byte
 // staging with the shifts (4 - q % 4) % 4 of q = 0, 1, 2 wherever the quad patch fits one unit
-// per thread (all but the 16 x 4 block with shift 3). The tables map (lw, shape, fsh) to them.
+// per thread (all but the 16 x 4 block with shift 3); each with and without the half-block
+// tail body (layers whose K % 64 is 1..32 take the tail instance). The grouped kernels always
+// carry the tail body (their layers have different K). The tables map (lw, shape, fsh, tail)
+// to them.
 namespace {
 template <int LW, int S>
 constexpr bool fused_instance() {
@@ -693,26 +706,30 @@ constexpr bool fused_instance() {
                              : (LW == kFusedLoadNHWC ? (S & 3) == 0 : (S & 3) <= 1);
 }
 
-template <int LW, int S, bool Grouped>
+// index I = 64 * tail + 16 * lw + S
+template <int I, bool Grouped>
 const void* fused_ptr_one() {
-    if constexpr (!fused_instance<LW, S>()) return nullptr;
+    constexpr int LW = (I % 64) / 16, S = I % 16;
+    constexpr bool TAIL = I >= 64;
+    if constexpr (!fused_instance<LW, S>() || (Grouped && !TAIL)) return nullptr;
     else if constexpr (Grouped) return (const void*)&wino_fused_grouped_kernel<LW, S>;
-    else return (const void*)&wino_fused_kernel<LW, S>;
+    else return (const void*)&wino_fused_kernel<LW, S, TAIL>;
 }
 
 template <bool Grouped, int... I>
 const void* fused_ptr_lookup(int idx, std::integer_sequence<int, I...>) {
-    static const void* const table[] = {fused_ptr_one<I / 16, I % 16, Grouped>()...};
+    static const void* const table[] = {fused_ptr_one<I, Grouped>()...};
     return table[idx];
 }
 }  // namespace
 
-const void* fused_kernel_ptr(int lw, int shape, int fsh) {
+const void* fused_kernel_ptr(int lw, int shape, int fsh, bool tail) {
     if (lw < 0 || lw >= kFusedLoadWidths || shape < 0 || shape > 3 || fsh < 0 || fsh > 3) return nullptr;
-    return fused_ptr_lookup<false>(lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 16 * kFusedLoadWidths>{});
+    return fused_ptr_lookup<false>((tail ? 64 : 0) + lw * 16 + shape * 4 + fsh,
+                                   std::make_integer_sequence<int, 32 * kFusedLoadWidths>{});
 }
 
 const void* fused_grouped_kernel_ptr(int lw, int shape, int fsh) {
     if (lw < 0 || lw >= kFusedLoadWidths || shape < 0 || shape > 3 || fsh < 0 || fsh > 3) return nullptr;
-    return fused_ptr_lookup<true>(lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 16 * kFusedLoadWidths>{});
+    return fused_ptr_lookup<true>(64 + lw * 16 + shape * 4 + fsh, std::make_integer_sequence<int, 32 * kFusedLoadWidths>{});
 }
