@@ -71,6 +71,32 @@ __global__ void __launch_bounds__(256) direct_pack_input(mykernelParamType prm) 
     const int tap0 = j0 / C;
     int c = j0 - tap0 * C, ri = tap0 / S, si = tap0 - ri * S;
     half8 v;
+    if ((C & 7) == 0) {
+        // C % 8 == 0 (wave-uniform): j0 is a multiple of 8, so the lane's 8 values are channels
+        // c .. c+7 of one tap: one bounds check, and for channel-contiguous input one 16-byte load
+        // when the pixel's channels are 16-byte aligned
+        const int ih = ih0 + ri, iw = iw0 + si;
+        const bool ok = g < prm.NT && j0 < jn && ih >= 0 && ih < H && iw >= 0 && iw < W;
+        v = half8{};
+        if (ok) {
+            if (prm.io & CONV2D_IO_IN_NHWC) {
+                const half_t* px = x + ((size_t)ih * W + iw) * C + c;
+                if ((((uintptr_t)px) & 15) == 0) {
+                    v = *(const half8*)px;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) v[e] = px[e];
+                }
+            } else {
+                const half_t* px = x + ((size_t)c * H + ih) * W + iw;
+                const size_t plane = (size_t)H * W;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = px[e * plane];
+            }
+        }
+        *(half8*)((half_t*)prm.transform_pin + ((size_t)L * P32 + pb) * kFragHalves + lane * 8) = v;
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int ih = ih0 + ri, iw = iw0 + si;
