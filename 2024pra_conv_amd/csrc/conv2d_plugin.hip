@@ -89,7 +89,8 @@ int derive_geometry_io(const problem_t* pb, int io, Geometry* g) {
         g->fnbh = (g->tiles_h + g->fbh - 1) / g->fbh;
         g->fnbw = (g->tiles_w + g->fbw - 1) / g->fbw;
         // staging load width: 8-byte pixel quads when the width and base allow them and the
-        // quad patch fits one unit per thread, else 4-byte pairs, else 2-byte pixels
+        // quad patch fits one unit per thread, else 4-byte aligned pairs, else 4-byte pairs at
+        // 2-byte aligned offsets (odd width or a base that is not 4-byte aligned)
         const uintptr_t in_addr = (uintptr_t)pb->in;
         g->fpr = fused_patch_rows(g->fbh);
         g->flw = (pb->w % 2) != 0 || (in_addr & 3) != 0 ? kFusedLoad2 : kFusedLoad4;

@@ -12,10 +12,11 @@ inline int fused_shape_index(int fbh, int fbw) {
         if (kFusedShapes[i][0] == fbh && kFusedShapes[i][1] == fbw) return i;
     return -1;
 }
-// Staging load width of the fused kernels, per channel of a staging unit: one 2-byte pixel
-// (odd width, or an input base that is not 4-byte aligned), one 4-byte pixel pair, or one
-// 8-byte pixel quad (width a multiple of 4, base 8-byte aligned, patch within one unit per
-// thread).
+// Staging load width of the fused kernels, per channel of a staging unit: kFusedLoad2, one
+// 4-byte pixel pair at a 2-byte aligned offset (odd width, or an input base that is not 4-byte
+// aligned; the pair at the right edge of an odd width loads the dword ending with its one
+// pixel); kFusedLoad4, one 4-byte aligned pixel pair; kFusedLoad8, one 8-byte pixel quad (width
+// a multiple of 4, base 8-byte aligned, patch within one unit per thread).
 // kFusedLoadNHWC: channel-contiguous (N,H,W,C) input, one 16-byte load of 8 channels per pixel
 // (C % 8 == 0, 16-byte aligned base).
 enum FusedLoad { kFusedLoad2 = 0, kFusedLoad4 = 1, kFusedLoad8 = 2, kFusedLoadNHWC = 3 };

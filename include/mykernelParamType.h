@@ -42,7 +42,7 @@ typedef struct mykernelParamType {
        of fpr rows x fpc columns starting fsh columns left of 2*tw0 - q, fnu staging units
        per thread of the 512-thread workgroup: 1 with 8-byte loads (8 channels x 4 pixels;
        w % 4 == 0, 8-byte aligned input, fsh = (4 - q % 4) % 4), else 2 (8 channels x 2
-       pixels, 4- or 2-byte loads, fsh = q & 1) */
+       pixels, 4-byte loads, 2-byte aligned for odd widths, fsh = q & 1) */
     int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
     /* output epilogue, applied in FP32 before the one FP16 rounding of every output:
        out = act(y + bias[k]); bias: K FP16 values or NULL, act: 0 none, 1 ReLU */
