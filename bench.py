@@ -154,6 +154,11 @@ CPU_SHARE_THREADS = 16
 PEAK_FP8_TFLOPS = 5033.2  # dense FP8 MFMA
 
 
+def _progress(msg: str) -> None:
+    """One progress line per finished section, on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def _err(y, ref) -> dict:
     d = (y.double() - ref).abs()
     return {"max_rel": (d.max() / ref.abs().max()).item(), "rms_rel": (d.norm() / ref.norm()).item(),
@@ -576,13 +581,14 @@ def main() -> int:
                           "definition": "max|y-y*|/max|y*| and ||y-y*||/||y*|| vs FP64 direct conv"},
         "wall_s_timed": round(wall, 4),
     }
+    _progress(f"headline workload done: {ms * 1e3:.2f} us per step")
     if not args.no_forms:
         del conv
         torch.cuda.empty_cache()
-        line["f43_form"] = f43_section(runner, dev, max(5, args.steps // 5))
-        line["fp8_form"] = fp8_section(runner, dev, max(5, args.steps // 5))
-        line["batched_form"] = batched_section(runner, dev, max(5, args.steps // 5))
-        line["direct_form"] = direct_section(runner, dev, max(5, args.steps // 5))
+        for key, section in (("f43_form", f43_section), ("fp8_form", fp8_section),
+                             ("batched_form", batched_section), ("direct_form", direct_section)):
+            line[key] = section(runner, dev, max(5, args.steps // 5))
+            _progress(f"{key} done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(1, c, h, w, k)
         line["cpu_baseline"].pop("seconds", None)
