@@ -674,21 +674,17 @@ __global__ void __launch_bounds__(512, 1) wino_fused_kernel(mykernelParamType pr
 }
 
 // Multi-layer form: one launch over the fused items of several layers (descriptor i owns items
-// [s3_first, s3_first + s3_items), nk = 64-channel blocks), every layer with the same staging
-// load width, block shape and column shift.
+// [s3_first, s3_first + s3_items), nk = 64-channel blocks, remapped per XCD within that range),
+// every layer with the same staging load width, block shape and column shift.
 template <int LW, int S>
 __global__ void __launch_bounds__(512, 1)
 wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
-    int item, unused;
-    xcd_remap(blockIdx.x, total, total, &item, &unused);
-    int lo = 0, hi = nlayers - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (layers[mid].s3_first <= item) lo = mid; else hi = mid - 1;
-    }
-    const BatchLayer& bl = layers[lo];
-    const int local = item - bl.s3_first;
-    fused_item_dispatch<LW, S, true>(bl.prm, local % bl.nk, local / bl.nk);
+    int local;
+    const BatchLayer& bl = grouped_layer_item(layers, nlayers, blockIdx.x, &local);
+    // the layer's parameters by value, loaded once, as the single-layer kernel receives them
+    const mykernelParamType prm = bl.prm;
+    const int nk = bl.nk;
+    fused_item_dispatch<LW, S, true>(prm, local % nk, local / nk);
 }
 
 int fused_threads() { return kFusedThreads; }

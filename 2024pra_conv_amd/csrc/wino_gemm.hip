@@ -423,16 +423,14 @@ __global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_
 wino_gemm_dma_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
     using Sh = DmaShape<BK, BT, WK, WT, S>;
     __shared__ __attribute__((aligned(16))) half_t smem[Sh::kSmemHalves];
-    int item, unused;
-    xcd_remap(blockIdx.x, total, total, &item, &unused);
-    int lo = 0, hi = nlayers - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (layers[mid].s3_first <= item) lo = mid; else hi = mid - 1;
-    }
-    const BatchLayer& bl = layers[lo];
-    const int local = item - bl.s3_first;
-    gemm_dma_body<BK, BT, WK, WT, S>(bl.prm, local % bl.nk, local / bl.nk, smem);
+    int local;
+    const BatchLayer& bl = grouped_layer_item(layers, nlayers, blockIdx.x, &local);
+    // the layer's parameters by value, loaded once (read through a reference into global
+    // memory, the epilogue's loops grew past the unroller's size limit in the larger tile
+    // configurations and their accumulators went to 576 bytes per lane of scratch)
+    const mykernelParamType prm = bl.prm;
+    const int nk = bl.nk;
+    gemm_dma_body<BK, BT, WK, WT, S>(prm, local % nk, local / nk, smem);
 }
 
 template <int BK, int BT, int WK, int WT, int D>
@@ -449,9 +447,10 @@ wino_gemm_kernel(mykernelParamType prm) {
 
 // ---------------------------------------------------------------------------------------
 // Multi-layer (grouped) GEMM: one launch covers every layer that uses this tile
-// configuration. Workgroup ids are remapped per XCD (as in the single-layer kernel) onto the
-// group's (layer, k-block, column block) items, which are numbered layer by layer in
-// decreasing cost; a binary search finds the item's layer. One item per workgroup keeps the
+// configuration. Each layer owns a contiguous range of workgroup ids (layers in decreasing
+// cost); a binary search finds the layer, and the id is remapped per XCD within that range (as
+// in the single-layer kernel) onto the layer's (k-block, column block) items, so every XCD runs
+// an equal share of every layer. One item per workgroup keeps the
 // register allocation of the single-layer kernel (a persistent loop around the body spills).
 // ---------------------------------------------------------------------------------------
 template <int BK, int BT, int WK, int WT, int D>
@@ -459,16 +458,14 @@ __global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_
 wino_gemm_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
     using S = GemmShape<BK, BT, WK, WT>;
     __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
-    int item, unused;
-    xcd_remap(blockIdx.x, total, total, &item, &unused);
-    int lo = 0, hi = nlayers - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (layers[mid].s3_first <= item) lo = mid; else hi = mid - 1;
-    }
-    const BatchLayer& bl = layers[lo];
-    const int local = item - bl.s3_first;
-    gemm_body<BK, BT, WK, WT, D>(bl.prm, local % bl.nk, local / bl.nk, smem);
+    int local;
+    const BatchLayer& bl = grouped_layer_item(layers, nlayers, blockIdx.x, &local);
+    // the layer's parameters by value, loaded once (read through a reference into global
+    // memory, the epilogue's loops grew past the unroller's size limit in the larger tile
+    // configurations and their accumulators went to 576 bytes per lane of scratch)
+    const mykernelParamType prm = bl.prm;
+    const int nk = bl.nk;
+    gemm_body<BK, BT, WK, WT, D>(prm, local % nk, local / nk, smem);
 }
 
 // ---- configuration table --------------------------------------------------------------

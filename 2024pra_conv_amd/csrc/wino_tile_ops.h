@@ -379,3 +379,23 @@ __device__ inline void xcd_remap(int L, int total, int nk, int* kblk, int* gblk)
     *kblk = m % nk;
     *gblk = m / nk;
 }
+
+// Multi-layer (grouped) launches: the workgroup ids of each layer form one contiguous range
+// (first id s3_first, s3_items ids), layers in decreasing per-item cost. Id L is remapped within
+// its layer's range as above, so every XCD runs an equal share of every layer (a contiguous run
+// of its k-block-fastest items) and the layers follow one another in launch order. (Remapped
+// over the whole group instead, the first XCDs would get the costliest layers' items and the
+// last ones the cheapest, while each XCD still runs the same number of workgroups.) Returns the
+// layer; *local is the item within it.
+__device__ inline const BatchLayer& grouped_layer_item(const BatchLayer* __restrict__ layers, int nlayers, int L,
+                                                       int* local) {
+    int lo = 0, hi = nlayers - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (layers[mid].s3_first <= L) lo = mid; else hi = mid - 1;
+    }
+    const BatchLayer& bl = layers[lo];
+    int unused;
+    xcd_remap(L - bl.s3_first, bl.s3_items, bl.s3_items, local, &unused);
+    return bl;
+}
