@@ -149,8 +149,8 @@ int launch_forward(conv2d_batch* b, hipStream_t st) {
     for (int i = 0; i < nseg; ++i) {
         const conv2d_batch::Segment& seg = b->segs[i];
         const BatchLayer* layers = b->d_layers + seg.first;
-        int count = seg.count, items = seg.items;
-        void* args[3] = {(void*)&layers, (void*)&count, (void*)&items};
+        int count = seg.count;
+        void* args[2] = {(void*)&layers, (void*)&count};
         const hipError_t e =
             seg.fused ? hipLaunchKernel(fused_grouped_kernel_ptr(seg.lw, seg.shape >> 2, seg.shape & 3), dim3(seg.items), dim3(fused_threads()), args, seg.lds, st)
                    : hipLaunchKernel(kGemmConfigs[seg.cfg].grouped_kernel, dim3(seg.items),

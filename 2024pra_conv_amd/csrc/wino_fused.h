@@ -42,9 +42,9 @@ constexpr int fused_patch_rows(int fbh) { return 2 * fbh + 2; }
 
 // Kernel for staging load width lw, block shape index `shape` and column shift fsh (tail: with
 // the half-block body for the last 64-channel block when K % 64 is 1..32), and its multi-layer
-// (grouped) form with arguments (const BatchLayer* layers, int nlayers, int total items), which
-// always has the tail body; null for a combination without an instance (8-byte loads with a
-// patch that needs more than one unit per thread).
+// (grouped) form with arguments (const BatchLayer* layers, int nlayers), one workgroup per
+// item, which always has the tail body; null for a combination without an instance (8-byte
+// loads with a patch that needs more than one unit per thread).
 const void* fused_kernel_ptr(int lw, int shape, int fsh, bool tail);
 const void* fused_grouped_kernel_ptr(int lw, int shape, int fsh);
 // Whether a layer's last 64-channel block has its second 32 channels beyond K (K % 64 is 1..32,

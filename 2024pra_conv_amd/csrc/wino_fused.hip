@@ -678,7 +678,7 @@ __global__ void __launch_bounds__(512, 1) wino_fused_kernel(mykernelParamType pr
 // every layer with the same staging load width, block shape and column shift.
 template <int LW, int S>
 __global__ void __launch_bounds__(512, 1)
-wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
+wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers) {
     int local;
     const BatchLayer& bl = grouped_layer_item(layers, nlayers, blockIdx.x, &local);
     // the layer's parameters by value, loaded once, as the single-layer kernel receives them

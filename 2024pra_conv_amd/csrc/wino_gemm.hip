@@ -420,7 +420,7 @@ wino_gemm_dma_kernel(mykernelParamType prm) {
 
 template <int BK, int BT, int WK, int WT, int S>
 __global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
-wino_gemm_dma_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
+wino_gemm_dma_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers) {
     using Sh = DmaShape<BK, BT, WK, WT, S>;
     __shared__ __attribute__((aligned(16))) half_t smem[Sh::kSmemHalves];
     int local;
@@ -455,7 +455,7 @@ wino_gemm_kernel(mykernelParamType prm) {
 // ---------------------------------------------------------------------------------------
 template <int BK, int BT, int WK, int WT, int D>
 __global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
-wino_gemm_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers, int total) {
+wino_gemm_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers) {
     using S = GemmShape<BK, BT, WK, WT>;
     __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
     int local;
