@@ -171,8 +171,14 @@ int derive_geometry_io(const problem_t* pb, int io, Geometry* g) {
         g->nsteps = (int)steps;
         g->band = 0;
         g->fused_ok = false;
-        g->v_bytes = (size_t)steps * 2 * (size_t)g->G32 * kFragHalves * 2;
         g->u_bytes = (size_t)steps * 2 * (size_t)g->KB32 * kFragHalves * 2;
+        // a 1x1 filter makes the reduction index the input channel: the GEMM gathers its B
+        // operand from the input (no packed V, no packing pass)
+        g->pointwise = pb->r == 1 && pb->s == 1;
+        g->point_quads = g->pointwise && pb->u == 1 && pb->v == 1 && pb->p == 0 && pb->q == 0 &&
+                         (io & CONV2D_IO_IN_NHWC) == 0 && ((long long)pb->h * pb->w) % 4 == 0 &&
+                         ((uintptr_t)pb->in & 7) == 0;
+        g->v_bytes = g->pointwise ? 0 : (size_t)steps * 2 * (size_t)g->G32 * kFragHalves * 2;
     }
     return CONV2D_OK;
 }
@@ -447,6 +453,7 @@ Launch slot_fused(const Geometry& g) {
 
 Launch slot_input(const problem_t* pb, const Geometry& g) {
     Launch L;
+    if (g.direct && g.pointwise) return slot_noop();  // the GEMM reads the input itself
     if (g.direct) {
         L.kernel = (const void*)&direct_pack_input;
         const long long threads = (long long)g.G32 * 2 * g.nsteps * 64;
@@ -463,7 +470,9 @@ Launch slot_input(const problem_t* pb, const Geometry& g) {
 }
 
 Launch slot_gemm(const Geometry& g, int cfg) {
-    const GemmConfig& c = g.direct ? kDirectConfigs[cfg] : kGemmConfigs[cfg];
+    const GemmConfig* table = !g.direct ? kGemmConfigs
+                              : (g.point_quads ? kPointQuadConfigs : (g.pointwise ? kPointConfigs : kDirectConfigs));
+    const GemmConfig& c = table[cfg];
     Launch L;
     L.kernel = c.kernel;
     const int vf = c.bt / 32, uf = c.bk / 32;

@@ -13,6 +13,9 @@ struct Geometry {
     int Oh, Ow;
     bool winograd_ok;      // 3x3, stride 1, pad <= 2
     bool direct;           // runs on the general-geometry (implicit-GEMM direct) path
+    bool pointwise;        // direct path with a 1x1 filter: the GEMM reads the input itself
+    bool point_quads;      // ... in 8-byte pieces along its rows (stride 1, no padding, N,C,H,W,
+                           // H*W % 4 == 0, 8-byte aligned input)
     int tiles_h, tiles_w;  // Winograd: 2x2 output tiles; direct: Oh, Ow
     long long T, NT, G32;  // GEMM columns per image / in total, 32-column blocks of V
     int CB, KB32;          // CB: 32-channel blocks; KB32: 32-row blocks of U

@@ -21,3 +21,9 @@ extern const int kNumPosConfigs;
 // Implicit-GEMM direct convolution (general-geometry path): same fields, no grouped kernels.
 extern const GemmConfig kDirectConfigs[];
 extern const int kNumDirectConfigs;
+// 1x1 filters on the general-geometry path: the GEMM reads the input itself; configuration i
+// has the tile shape of kDirectConfigs[i].
+extern const GemmConfig kPointConfigs[];
+// The same for stride 1, no padding, N,C,H,W input with H*W % 4 == 0 and an 8-byte aligned base:
+// 8-byte loads along the input rows, transposed in registers.
+extern const GemmConfig kPointQuadConfigs[];

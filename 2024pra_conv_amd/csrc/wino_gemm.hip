@@ -36,12 +36,16 @@ namespace {
 // KS 16-channel substeps per pipeline step (one v_mfma_f32_32x32x16_f16 k-step each).
 constexpr int KS = 2;
 
+typedef half_t half4v __attribute__((ext_vector_type(4)));
+
 // What a GEMM body computes:
 //   kGemmWino   the F(2x2) GEMM: 16 positions, fold per position, 2x2-tile epilogue;
 //   kGemmDirect the implicit-GEMM direct convolution of the general-geometry path;
 //   kGemmPos    positions fpos .. fpos + f43_npos - 1 of the F(4x4) form (36 positions): their
-//               packed U and V in sequence, each M stored in FP16 to mid_result (store_m16).
-enum { kGemmWino = 0, kGemmDirect = 1, kGemmPos = 2 };
+//               packed U and V in sequence, each M stored in FP16 to mid_result (store_m16);
+//   kGemmPoint  the direct convolution of a 1x1 filter: as kGemmDirect, but the B operand is
+//               read from the input itself (register-staged body only), with no packing pass.
+enum { kGemmWino = 0, kGemmDirect = 1, kGemmPos = 2, kGemmPoint = 3 };
 
 template <int BK, int BT, int WK, int WT>
 struct GemmShape {
@@ -58,6 +62,33 @@ struct GemmShape {
     static_assert(BK % (WK * 32) == 0 && BT % (WT * 32) == 0, "tile must divide evenly");
 };
 
+// kGemmPoint: the 8 input channels c .. c+7 at input element xo of channel 0 (-1: the pixel
+// is outside the image), zero for channels >= C. NCHW input: 8 loads one channel plane apart
+// (across a wave's lanes they read consecutive pixels); channel-contiguous input: one 16-byte
+// load when C % 8 == 0 and it is aligned. Loads are not predicated per lane (a lane without a
+// value reads element 0 of the input and discards it), so the compiler's memory-counter waits
+// stay per load (with per-lane branches it waits for every load in flight).
+__device__ inline half8 point_piece(const mykernelParamType& prm, long long xo, int c, bool nhwc) {
+    const half_t* __restrict__ X = (const half_t*)prm.pin;
+    const int C = prm.c;
+    half8 v;
+    if (nhwc && (C & 7) == 0 && (((uintptr_t)X) & 15) == 0) {
+        const bool ok = xo >= 0 && c < C;
+        const half8 raw = *(const half8*)(X + (ok ? xo + c : 0));
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = ok ? raw[e] : (half_t)0.0f;
+        return v;
+    }
+    const long long es = nhwc ? 1 : (long long)prm.h * prm.w;  // elements between channels
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const bool ok = xo >= 0 && c + e < C;
+        const half_t raw = X[ok ? xo + (long long)(c + e) * es : 0];
+        v[e] = ok ? raw : (half_t)0.0f;
+    }
+    return v;
+}
+
 }  // namespace
 
 // One GEMM work item (k-block kblk x column block gblk). smem holds 2 stages. Ends after
@@ -69,6 +100,10 @@ struct GemmShape {
 // reduction index into one accumulator, whose elements are the outputs themselves.
 // MODE kGemmPos: positions fpos .. fpos + prm.f43_npos - 1 of the F(4x4) form in one pipeline
 // (npos * CB steps), each position's M stored by store_m16 at its last step.
+// MODE kGemmPoint: a 1x1 filter (any stride and padding), so the reduction index is the input
+// channel: each V piece of a step is gathered from the input (8 channels of one pixel) and
+// stored to LDS in the fragment layout the packing pass would have written, so the MFMAs see
+// the same operands in the same order as in kGemmDirect (the results are bit-identical).
 template <int BK, int BT, int WK, int WT, int D, int MODE = kGemmWino>
 __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem, int fpos = 0) {
     using S = GemmShape<BK, BT, WK, WT>;
@@ -85,7 +120,8 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     const int kb0 = kblk * UF;  // first 32-row block of U
     // 32-value reduction chunks (a multiple of 8 except for F(4x4) position groups, whose steps
     // past npos * CB issue no MFMAs)
-    const int nsteps = MODE == kGemmDirect ? prm.nsteps : (MODE == kGemmPos ? prm.f43_npos * CB : 16 * CB);
+    constexpr bool kDirectSteps = MODE == kGemmDirect || MODE == kGemmPoint;
+    const int nsteps = kDirectSteps ? prm.nsteps : (MODE == kGemmPos ? prm.f43_npos * CB : 16 * CB);
     const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
     const half_t* __restrict__ V = (const half_t*)prm.transform_pin;
     if constexpr (MODE == kGemmPos) {
@@ -98,6 +134,13 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     // position), so fragment addresses advance by 2 blocks per step.
     const half_t* src[NL];
     size_t stride[NL];
+    // kGemmPoint: whether piece i is a V piece (the same for the 64 lanes of a wave), the first
+    // of its 8 channels within a step (xc) and the input element of channel 0 at its pixel
+    // (xo; -1 outside the image, past the last column or for a padding slot)
+    [[maybe_unused]] bool isv[NL];
+    [[maybe_unused]] int xc[NL];
+    [[maybe_unused]] long long xo[NL];
+    [[maybe_unused]] const bool nhwc = (prm.io & CONV2D_IO_IN_NHWC) != 0;
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
         const int pc = tid + i * S::kThreads;
@@ -108,11 +151,29 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
             const bool ok = pc < S::kPieces && kb0 + q / 64 < KB;
             src[i] = U + (ok ? ((size_t)kk * KB + kb0) * kFragHalves + (size_t)q * 8 : (size_t)(q % 64) * 8);
             stride[i] = (size_t)2 * KB * kFragHalves;
+            if constexpr (MODE == kGemmPoint) isv[i] = false, xc[i] = 0, xo[i] = -1;
         } else {
             const int qv = q - UF * 64;
             const bool ok = pc < S::kPieces && gb0 + qv / 64 < G32;
             src[i] = V + (ok ? ((size_t)kk * G32 + gb0) * kFragHalves + (size_t)qv * 8 : (size_t)(qv % 64) * 8);
             stride[i] = (size_t)2 * G32 * kFragHalves;
+            if constexpr (MODE == kGemmPoint) {
+                // fragment lane slot qv % 64: pixel (column) g, channels 8 * (slot / 32) of the
+                // 16-channel block kk of the step
+                const int slot = qv % 64;
+                const int g = (gb0 + qv / 64) * 32 + (slot & 31);
+                isv[i] = true;
+                xc[i] = kk * 16 + (slot >> 5) * 8;
+                xo[i] = -1;
+                if (pc < S::kPieces && g < prm.NT) {
+                    const int img = g / prm.T, pix = g - (g / prm.T) * prm.T;
+                    const int oh = pix / prm.Ow, ow = pix - (pix / prm.Ow) * prm.Ow;
+                    const int ih = oh * prm.u - prm.p, iw = ow * prm.v - prm.q;
+                    const long long H = prm.h, W = prm.w, C = prm.c;
+                    if (ih >= 0 && ih < H && iw >= 0 && iw < W)
+                        xo[i] = nhwc ? ((long long)img * H * W + ih * W + iw) * C : (long long)img * C * H * W + ih * W + iw;
+                }
+            }
         }
     }
 
@@ -120,7 +181,15 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
     auto gload = [&](half8 (&slot)[NL], int step) {
         const size_t s = (size_t)min(step, nsteps - 1);
 #pragma unroll
-        for (int i = 0; i < NL; ++i) slot[i] = *(const half8*)(src[i] + s * stride[i]);
+        for (int i = 0; i < NL; ++i) {
+            if constexpr (MODE == kGemmPoint) {
+                if (isv[i]) {
+                    slot[i] = point_piece(prm, xo[i], (int)s * 32 + xc[i], nhwc);
+                    continue;
+                }
+            }
+            slot[i] = *(const half8*)(src[i] + s * stride[i]);
+        }
     };
     auto lstore = [&](const half8 (&slot)[NL], int buf) {
 #pragma unroll
@@ -219,11 +288,153 @@ __device__ inline void gemm_body(const mykernelParamType& prm, int kblk, int gbl
         }
     }
 
-    if constexpr (MODE == kGemmDirect)
+    if constexpr (kDirectSteps)
         store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
     else if constexpr (MODE == kGemmWino)
         store_tiles32<MI, NI>(prm, Y, gb0 + wt * NI, kb0 + wk * MI, lane);
     // kGemmPos: every position of the group was stored at its last step
+}
+
+// 1x1 filter, stride 1, no padding, N,C,H,W input with H*W % 4 == 0 and an 8-byte aligned
+// base: the register-staged pipeline of gemm_body with the input read in 8-byte pieces along
+// its rows. Unit (16-channel block kk, channel quad cq, fragment f, pixel quad pq) of a step =
+// 4 channels x 4 consecutive pixels (GEMM columns, all in one image since H*W % 4 == 0): four
+// 8-byte loads, one per channel, across a wave's lanes 32 consecutive pixel quads of one
+// channel row; transposed in registers, it becomes half of the fragment lane slots of its 4
+// pixels (8-byte LDS stores). The MFMAs see the operands of the packed direct path in the same
+// order, so the results are the same bits.
+template <int BK, int BT, int WK, int WT, int D>
+__device__ inline void point_quad_body(const mykernelParamType& prm, int kblk, int gblk, half_t* smem) {
+    using S = GemmShape<BK, BT, WK, WT>;
+    constexpr int MI = S::MI, NI = S::NI, UF = S::UF, VF = S::VF;
+    constexpr int kStage = KS * (UF + VF) * kFragHalves;  // halves per LDS stage (<= S::kStageHalves)
+    constexpr int kUPieces = KS * UF * 64;                 // 16-byte U pieces per step
+    constexpr int NU = (kUPieces + S::kThreads - 1) / S::kThreads;
+    constexpr int kVUnits = KS * 4 * VF * 8;               // 4 x 4 input units per step
+    static_assert(kVUnits % 64 == 0 && kVUnits <= S::kThreads, "at most one unit per thread, whole waves");
+    static_assert(8 % D == 0 && D % 2 == 0, "ring depth: even, dividing the step count (a multiple of 8)");
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wk = wave % WK;
+    const int wt = wave / WK;
+    const int KB = prm.KB32, C = prm.c;
+    const int gb0 = gblk * VF, kb0 = kblk * UF;
+    const int nsteps = prm.nsteps;
+    const half_t* __restrict__ U = (const half_t*)prm.transform_filter;
+    const half_t* __restrict__ X = (const half_t*)prm.pin;
+
+    // U: 16-byte fragment pieces as in gemm_body (slots past kUPieces, whole waves, store nothing)
+    const half_t* usrc[NU];
+    int udst[NU];
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        const int pc = tid + i * S::kThreads;
+        const int p = pc < kUPieces ? pc : 0;
+        const int kk = p / (UF * 64), q = p % (UF * 64);
+        const bool ok = pc < kUPieces && kb0 + q / 64 < KB;
+        usrc[i] = U + (ok ? ((size_t)kk * KB + kb0) * kFragHalves + (size_t)q * 8 : (size_t)(q % 64) * 8);
+        udst[i] = pc < kUPieces ? kk * (UF + VF) * kFragHalves + q * 8 : -1;
+    }
+    const size_t ustride = (size_t)2 * KB * kFragHalves;
+
+    // V: this thread's unit (pixel quad fastest, then fragment, channel quad, 16-channel block)
+    const bool hasv = tid < kVUnits;
+    const int pq = tid & 7, f = (tid >> 3) % VF, cq = (tid >> 3) / VF % 4, kkv = (tid >> 3) / VF / 4;
+    const int g0 = (gb0 + f) * 32 + pq * 4;
+    const bool gok = hasv && g0 < prm.NT;
+    const size_t hw = (size_t)prm.T;  // H*W (stride 1, no padding)
+    const int img = g0 / prm.T;
+    const size_t xbase = (size_t)img * C * hw + (size_t)(g0 - img * prm.T);
+    const int c0 = kkv * 16 + cq * 4;
+    const int vdst = kkv * (UF + VF) * kFragHalves + (UF + f) * kFragHalves + ((cq >> 1) * 32 + pq * 4) * 8 + (cq & 1) * 4;
+
+    half8 uring[D][NU];
+    half4v vring[D][4];
+    auto gload = [&](half8 (&us)[NU], half4v (&vs)[4], int step) {
+        const int s = min(step, nsteps - 1);
+#pragma unroll
+        for (int i = 0; i < NU; ++i) us[i] = *(const half8*)(usrc[i] + (size_t)s * ustride);
+        if (hasv) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                // not predicated per lane: a lane without a value reads element 0 and drops it
+                const int c = s * 32 + c0 + e;
+                const bool ok = gok && c < C;
+                const half4v raw = *(const half4v*)(X + (ok ? xbase + (size_t)c * hw : 0));
+                vs[e] = ok ? raw : half4v{};
+            }
+        }
+    };
+    auto lstore = [&](const half8 (&us)[NU], const half4v (&vs)[4], int buf) {
+        half_t* b = smem + buf * kStage;
+#pragma unroll
+        for (int i = 0; i < NU; ++i)
+            if (udst[i] >= 0) *(half8*)(b + udst[i]) = us[i];
+        if (hasv) {
+#pragma unroll
+            for (int px = 0; px < 4; ++px) {
+                const half4v o = {vs[0][px], vs[1][px], vs[2][px], vs[3][px]};
+                *(half4v*)(b + vdst + px * 8) = o;
+            }
+        }
+    };
+
+    floatx16 acc[MI][NI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[mi][ni][e] = 0.f;
+
+    half8 ra[2][KS][MI], rb[2][KS][NI];
+    auto lread = [&](half8 (&A)[KS][MI], half8 (&B)[KS][NI], int buf) {
+        const half_t* sb = smem + buf * kStage;
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const half_t* sk = sb + kk * (UF + VF) * kFragHalves;
+#pragma unroll
+            for (int mi = 0; mi < MI; ++mi) A[kk][mi] = *(const half8*)(sk + ((wk * MI + mi) * 64 + lane) * 8);
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni)
+                B[kk][ni] = *(const half8*)(sk + UF * kFragHalves + ((wt * NI + ni) * 64 + lane) * 8);
+        }
+    };
+
+    // the pipeline of gemm_body: ring slot = step % D, LDS buffer = step & 1
+    gload(uring[0], vring[0], 0);
+    lstore(uring[0], vring[0], 0);
+    gload(uring[0], vring[0], D);
+#pragma unroll
+    for (int d = 1; d < D; ++d) gload(uring[d], vring[d], d);
+    __syncthreads();
+    lread(ra[0], rb[0], 0);
+    lstore(uring[1 % D], vring[1 % D], 1);
+    gload(uring[1 % D], vring[1 % D], 1 + D);
+    __syncthreads();
+
+    for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int s = s0 + d;
+            const int cur = d & 1;
+#pragma unroll
+            for (int kk = 0; kk < KS; ++kk)
+#pragma unroll
+                for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < NI; ++ni)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[cur][kk][mi], rb[cur][kk][ni],
+                                                                             acc[mi][ni], 0, 0, 0);
+            lread(ra[cur ^ 1], rb[cur ^ 1], cur ^ 1);
+            lstore(uring[(d + 2) % D], vring[(d + 2) % D], cur);
+            gload(uring[(d + 2) % D], vring[(d + 2) % D], s + 2 + D);
+            __syncthreads();
+        }
+    }
+    store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -393,6 +604,7 @@ __device__ inline void gemm_dma_body(const mykernelParamType& prm, int kblk, int
     }
     wait_vm<0>();  // the trailing (re-read) DMAs land before the workgroup's LDS is released
 
+    static_assert(MODE != kGemmPoint, "1x1 gathers need the register-staged body");
     if constexpr (MODE == kGemmDirect)
         store_plain32<MI, NI>(prm, acc, gb0 + wt * NI, kb0 + wk * MI, lane);
     else if constexpr (MODE == kGemmWino)
@@ -544,6 +756,57 @@ const GemmConfig kDirectConfigs[] = {
     DIRECT_DMA_CFG(128, 64, 2, 2, 4),   // 3: LDS-DMA, 4-stage ring, wave 64x32, 4 waves
 };
 const int kNumDirectConfigs = (int)(sizeof(kDirectConfigs) / sizeof(kDirectConfigs[0]));
+
+// 1x1 filters: the register-staged body gathers the input itself (kGemmPoint), so the input
+// needs no packing pass. Configuration i has the tile shape of kDirectConfigs[i] (the LDS-DMA
+// ones become register-staged: DMA copies whole fragments and cannot gather).
+template <int BK, int BT, int WK, int WT, int D>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
+wino_point_kernel(mykernelParamType prm) {
+    using S = GemmShape<BK, BT, WK, WT>;
+    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
+    const int nk = (prm.KB32 + S::UF - 1) / S::UF;
+    const int ng = (prm.G32 + S::VF - 1) / S::VF;
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
+    gemm_body<BK, BT, WK, WT, D, kGemmPoint>(prm, kblk, gblk, smem);
+}
+
+#define POINT_CFG(BK, BT, WK, WT, D) \
+    { BK, BT, WK * WT * 64, D, (const void*)&wino_point_kernel<BK, BT, WK, WT, D>, nullptr }
+
+const GemmConfig kPointConfigs[] = {
+    POINT_CFG(128, 128, 2, 4, 2),  // 0: wave 64x32, 8 waves
+    POINT_CFG(64, 128, 1, 4, 2),   // 1: wave 64x32, 4 waves
+    POINT_CFG(32, 64, 1, 2, 2),    // 2: wave 32x32, 2 waves (small problems)
+    POINT_CFG(128, 64, 2, 2, 2),   // 3: wave 64x32, 4 waves
+};
+static_assert(sizeof(kPointConfigs) == sizeof(kDirectConfigs), "one 1x1 configuration per direct configuration");
+
+// 1x1, stride 1, no padding, N,C,H,W input with H*W % 4 == 0 (8-byte aligned): the input is
+// read in 8-byte pieces along its rows and transposed in registers (point_quad_body).
+template <int BK, int BT, int WK, int WT, int D>
+__global__ void __launch_bounds__(WK * WT * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<BK, BT, WK, WT>)))
+wino_point_quad_kernel(mykernelParamType prm) {
+    using S = GemmShape<BK, BT, WK, WT>;
+    __shared__ __attribute__((aligned(16))) half_t smem[2 * S::kStageHalves];
+    const int nk = (prm.KB32 + S::UF - 1) / S::UF;
+    const int ng = (prm.G32 + S::VF - 1) / S::VF;
+    int kblk, gblk;
+    xcd_remap(blockIdx.x, nk * ng, nk, &kblk, &gblk);
+    point_quad_body<BK, BT, WK, WT, D>(prm, kblk, gblk, smem);
+}
+
+#define POINT_QUAD_CFG(BK, BT, WK, WT, D) \
+    { BK, BT, WK * WT * 64, D, (const void*)&wino_point_quad_kernel<BK, BT, WK, WT, D>, nullptr }
+
+const GemmConfig kPointQuadConfigs[] = {
+    POINT_QUAD_CFG(128, 128, 2, 4, 2),  // 0: wave 64x32, 8 waves
+    POINT_QUAD_CFG(64, 128, 1, 4, 2),   // 1: wave 64x32, 4 waves
+    POINT_QUAD_CFG(32, 64, 1, 2, 2),    // 2: wave 32x32, 2 waves (small problems)
+    POINT_QUAD_CFG(128, 64, 2, 2, 2),   // 3: wave 64x32, 4 waves
+};
+static_assert(sizeof(kPointQuadConfigs) == sizeof(kDirectConfigs), "one configuration per direct configuration");
 
 // ---- per-position GEMM of the F(4x4,3x3) form -----------------------------------------------
 // One workgroup per (k-block, column block, group of f43_npos positions): the group's steps in

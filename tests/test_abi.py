@@ -220,7 +220,7 @@ def test_getkernelinfo_rejects(wc, change, code):
 
 @pytest.mark.parametrize("geo", [
     dict(r=5, s=5, p=2, q=2), dict(u=2, v=2), dict(p=3, q=3), dict(r=1, s=1, p=0, q=0),
-    dict(r=7, s=7, u=2, v=2, p=3, q=3), dict(r=3, s=1, p=1, q=0), dict(r=2, s=2, u=2, v=2, p=0, q=0),
+    dict(r=1, s=1, u=2, v=2, p=0, q=0), dict(r=1, s=1, p=1, q=1), dict(r=7, s=7, u=2, v=2, p=3, q=3), dict(r=3, s=1, p=1, q=0), dict(r=2, s=2, u=2, v=2, p=0, q=0),
 ])
 def test_general_geometry_runs_on_the_direct_path(wc, geo):
     """Geometry the Winograd path does not take (the reference ignored r, s, u, v, p, q and
@@ -241,10 +241,18 @@ def test_general_geometry_runs_on_the_direct_path(wc, geo):
     assert (prm.Oh, prm.Ow, prm.T, prm.NT, prm.G32, prm.CB, prm.nsteps) == (oh, ow, oh * ow, P, -(-P // 32), cb, nsteps)
     assert prm.KB32 == -(-args["k"] // 32)
     vb, ub, mb = wc.workspace_size(pb)
-    assert vb == nsteps * 2 * prm.G32 * 1024 and ub == nsteps * 2 * prm.KB32 * 1024 and mb == 0
+    pointwise = args["r"] == 1 and args["s"] == 1
+    # a 1x1 filter: the GEMM gathers the input itself, so there is no packed input and slot 2
+    # (the packing pass of other filters) is the empty kernel of slot 4
+    assert vb == (0 if pointwise else nsteps * 2 * prm.G32 * 1024)
+    assert ub == nsteps * 2 * prm.KB32 * 1024 and mb == 0
     assert all(ki.slot(i)["kernel"] for i in range(1, 5))
     assert ki.slot(1)["grid"][0] * 256 >= nsteps * 2 * prm.KB32 * 64
-    assert ki.slot(2)["grid"][0] * 256 >= nsteps * 2 * prm.G32 * 64
+    if pointwise:
+        assert ki.slot(2)["kernel"] == ki.slot(4)["kernel"] and ki.slot(2)["grid"] == (1, 1, 1)
+    else:
+        assert ki.slot(2)["grid"][0] * 256 >= nsteps * 2 * prm.G32 * 64
+        assert ki.slot(2)["kernel"] != ki.slot(4)["kernel"]
     ncfg = L.conv2d_direct_num_tile_configs()
     assert 0 <= prm.cfg < ncfg
     bk, bt, th = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -9,6 +9,9 @@ Bars
   end to end        : normwise max relative error <= 1e-2 against the FP64 direct conv.
   configurations    : bit-identical output across the direct tile configurations; the
                       four-slot protocol equals the stage API.
+A 1x1 filter has no packing pass (the GEMM gathers its input operand from the input itself):
+its tests check that nothing is written to the input workspace and hold the GEMM output to the
+same bound, which is computed from the same operands.
 """
 import ctypes
 
@@ -55,7 +58,10 @@ def test_direct_packing_gemm_and_accuracy(geo, runner, wc, cuda):
     conv.input_transform()
     torch.cuda.synchronize()
     assert np.array_equal(conv.u_halves().cpu().numpy().view(np.uint16), dg.pack_weights(w, geo).view(np.uint16))
-    assert np.array_equal(conv.v_halves().cpu().numpy().view(np.uint16), dg.pack_input(x, geo).view(np.uint16))
+    if geo.r == 1 and geo.s == 1:
+        assert conv.v_bytes == 0 and bool((conv.V == 0x55).all())  # no packed input
+    else:
+        assert np.array_equal(conv.v_halves().cpu().numpy().view(np.uint16), dg.pack_input(x, geo).view(np.uint16))
 
     conv.out.fill_(float("nan"))
     conv.gemm()
@@ -120,3 +126,77 @@ def test_direct_full_size_against_fp64_gpu(shape, runner, cuda):
     rms_rel = (d.norm() / ref.norm()).item()
     print(f"  max_rel={max_rel:.3e} rms_rel={rms_rel:.3e}", flush=True)
     assert bool(torch.isfinite(conv.out).all()) and max_rel <= 1e-2
+
+
+POINT_GEOS = [
+    # n, c, h, w, k, r, s, u, v, p, q
+    dg.DirectGeometry(2, 16, 6, 7, 16, 1, 1, 1, 1, 0, 0),     # columns straddle images
+    dg.DirectGeometry(1, 13, 9, 9, 40, 1, 1, 2, 2, 0, 0),     # stride 2, C not a multiple of 8
+    dg.DirectGeometry(1, 8, 5, 5, 8, 1, 1, 1, 1, 1, 1),       # pad 1: a border of zero inputs
+    dg.DirectGeometry(3, 72, 7, 7, 130, 1, 1, 1, 1, 0, 0),    # three channel chunks, K > 128
+    # H*W % 4 == 0, stride 1, no padding: N,C,H,W input read in 8-byte pieces along its rows
+    dg.DirectGeometry(2, 16, 6, 8, 24, 1, 1, 1, 1, 0, 0),
+    dg.DirectGeometry(1, 13, 4, 9, 40, 1, 1, 1, 1, 0, 0),     # channel quads cut by C = 13
+    dg.DirectGeometry(3, 72, 14, 14, 130, 1, 1, 1, 1, 0, 0),  # 32-column blocks straddle images
+]
+QUAD_GEOS = [g for g in POINT_GEOS if g.u == 1 and g.p == 0 and (g.h * g.w) % 4 == 0]
+
+
+@pytest.mark.parametrize("geo", POINT_GEOS, ids=_ids)
+@pytest.mark.parametrize("nhwc", [False, True], ids=["nchw", "nhwc"])
+def test_pointwise_gemm_reads_the_input(geo, nhwc, runner, wc, cuda):
+    """1x1 filters: every tile configuration gives the same bits, within the FP32 bound of the
+    FP64 GEMM on the same operands and 1e-2 of the FP64 convolution, for N,C,H,W and
+    channel-contiguous input (16-byte loads when C % 8 == 0, element loads otherwise)."""
+    print(f"case {_ids(geo)} nhwc={nhwc}", flush=True)
+    x = wg.uniform_pm1(900 + geo.c, (geo.n, geo.c, geo.h, geo.w))
+    w = wg.uniform_pm1(901 + geo.k, (geo.k, geo.c, 1, 1))
+    xd = _dev(x.transpose(0, 2, 3, 1) if nhwc else x, cuda)
+    conv = runner.WinogradConv(xd, _dev(w, cuda), pad=(geo.p, geo.q), stride=(geo.u, geo.v), io=1 if nhwc else 0)
+    assert conv.method() == wc.METHOD_DIRECT and conv.v_bytes == 0
+    conv.prepare_weights()
+    outs = []
+    for cfg in range(wc.load_library().conv2d_direct_num_tile_configs()):
+        conv.out.fill_(float("nan"))
+        conv.forward(cfg)  # the layout-aware entry point (the stage API reads N,C,H,W input)
+        torch.cuda.synchronize()
+        outs.append(conv.out.clone())
+    for i, o in enumerate(outs[1:], 1):
+        assert torch.equal(o.view(torch.int16), outs[0].view(torch.int16)), f"config {i}"
+    y = outs[0].cpu().numpy().astype(np.float64)
+    y_model = dg.gemm_output_f64(w, x, geo)
+    e = (32 * geo.nsteps + 16) * 2.0 ** -24 * dg.gemm_output_f64(w, x, geo, absolute=True)
+    tol = 2.0 ** -11 * (np.abs(y_model) + e) + e + 2.0 ** -25
+    assert np.all(np.abs(y - y_model) <= tol), f"max excess {np.max(np.abs(y - y_model) - tol)}"
+    ref = cdirect.conv2d_f64(x, w, stride=(geo.u, geo.v), pad=(geo.p, geo.q))
+    m = wg.error_metrics(y, ref)
+    print(f"  max_rel={m['max_rel']:.3e}", flush=True)
+    assert m["finite"] and m["max_rel"] <= 1e-2
+    # the reference protocol on the same problem: four slots, the second one empty
+    conv.out.fill_(float("nan"))
+    conv.launch_slots()
+    torch.cuda.synchronize()
+    assert torch.equal(conv.out.view(torch.int16), outs[conv.selected_config()].view(torch.int16))
+
+
+@pytest.mark.parametrize("geo", QUAD_GEOS, ids=_ids)
+def test_pointwise_row_pieces_match_gathered_input(geo, runner, wc, cuda):
+    """The 8-byte row pieces (N,C,H,W, H*W % 4 == 0, 8-byte aligned input) and the per-channel
+    gather (the same input 2 bytes past an 8-byte boundary) give the same bits."""
+    print(f"case {_ids(geo)}", flush=True)
+    x = _dev(wg.uniform_pm1(950 + geo.c, (geo.n, geo.c, geo.h, geo.w)), cuda)
+    w = _dev(wg.uniform_pm1(951 + geo.k, (geo.k, geo.c, 1, 1)), cuda)
+    shifted = torch.empty(x.numel() + 4, dtype=torch.float16, device=cuda)[1:1 + x.numel()].view(x.shape)
+    shifted.copy_(x)
+    assert shifted.data_ptr() % 8 == 2 and x.data_ptr() % 8 == 0
+    outs = []
+    for xi in (x, shifted):
+        conv = runner.WinogradConv(xi, w, pad=(0, 0))
+        conv.prepare_weights()
+        for cfg in range(wc.load_library().conv2d_direct_num_tile_configs()):
+            conv.out.fill_(float("nan"))
+            conv.forward(cfg)
+            torch.cuda.synchronize()
+            outs.append(conv.out.clone())
+    for i, o in enumerate(outs[1:], 1):
+        assert torch.equal(o.view(torch.int16), outs[0].view(torch.int16)), f"run {i}"
