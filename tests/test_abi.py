@@ -218,15 +218,16 @@ def test_getkernelinfo_rejects(wc, change, code):
     assert rc == getattr(wc, code)
 
 
+@pytest.mark.parametrize("c", [40, 13])
 @pytest.mark.parametrize("geo", [
     dict(r=5, s=5, p=2, q=2), dict(u=2, v=2), dict(p=3, q=3), dict(r=1, s=1, p=0, q=0),
     dict(r=1, s=1, u=2, v=2, p=0, q=0), dict(r=1, s=1, p=1, q=1), dict(r=7, s=7, u=2, v=2, p=3, q=3), dict(r=3, s=1, p=1, q=0), dict(r=2, s=2, u=2, v=2, p=0, q=0),
 ])
-def test_general_geometry_runs_on_the_direct_path(wc, geo):
+def test_general_geometry_runs_on_the_direct_path(wc, geo, c):
     """Geometry the Winograd path does not take (the reference ignored r, s, u, v, p, q and
     computed garbage) is accepted and set up as an implicit-GEMM direct convolution."""
     L = wc.load_library()
-    args = dict(n=2, c=40, h=17, w=13, k=24, r=3, s=3, u=1, v=1, p=1, q=1)
+    args = dict(n=2, c=c, h=17, w=13, k=24, r=3, s=3, u=1, v=1, p=1, q=1)
     args.update(geo)
     pb = wc.make_problem(**args, ptrs=FAKE)
     m = ctypes.c_int()
@@ -241,14 +242,15 @@ def test_general_geometry_runs_on_the_direct_path(wc, geo):
     assert (prm.Oh, prm.Ow, prm.T, prm.NT, prm.G32, prm.CB, prm.nsteps) == (oh, ow, oh * ow, P, -(-P // 32), cb, nsteps)
     assert prm.KB32 == -(-args["k"] // 32)
     vb, ub, mb = wc.workspace_size(pb)
-    pointwise = args["r"] == 1 and args["s"] == 1
+    gather = args["r"] == 1 and args["s"] == 1
     # a 1x1 filter: the GEMM gathers the input itself, so there is no packed input and slot 2
-    # (the packing pass of other filters) is the empty kernel of slot 4
-    assert vb == (0 if pointwise else nsteps * 2 * prm.G32 * 1024)
+    # (the packing pass otherwise) is the empty kernel of slot 4 (larger filters gather only
+    # from channel-contiguous input)
+    assert vb == (0 if gather else nsteps * 2 * prm.G32 * 1024)
     assert ub == nsteps * 2 * prm.KB32 * 1024 and mb == 0
     assert all(ki.slot(i)["kernel"] for i in range(1, 5))
     assert ki.slot(1)["grid"][0] * 256 >= nsteps * 2 * prm.KB32 * 64
-    if pointwise:
+    if gather:
         assert ki.slot(2)["kernel"] == ki.slot(4)["kernel"] and ki.slot(2)["grid"] == (1, 1, 1)
     else:
         assert ki.slot(2)["grid"][0] * 256 >= nsteps * 2 * prm.G32 * 64
@@ -552,3 +554,20 @@ def test_f43_entry_points_refuse(wc):
     assert wc.form_workspace_size(direct, wc.FORM_F43) is None
     assert L.conv2d_tuned_set_form(ctypes.byref(pb), 0, 3) == wc.ERR_INVALID
     assert L.conv2d_f43_tile_config(-1, None, None, None) == wc.ERR_INVALID
+
+
+def test_null_input_workspace_only_where_it_is_zero_bytes(wc):
+    """hipMalloc(0) gives NULL: a problem whose reported input workspace is 0 bytes (the direct
+    path gathering its input: a 1x1 filter) may pass transform_pin = NULL; others get
+    CONV2D_ERR_NULL."""
+    no_v = dict(FAKE, transform_pin=None)
+    cases = [(dict(n=2, c=16, h=9, w=9, k=8, r=1, s=1, p=0, q=0), wc.OK),          # 1x1
+             (dict(n=2, c=13, h=9, w=9, k=8, r=1, s=1, u=2, v=2, p=0, q=0), wc.OK),  # 1x1 stride 2
+             (dict(n=2, c=16, h=9, w=9, k=8, r=3, s=3, u=2, v=2), wc.ERR_NULL),    # packed im2col
+             (dict(n=2, c=16, h=9, w=9, k=8), wc.ERR_NULL)]                        # Winograd
+    for args, want in cases:
+        pb = wc.make_problem(**args, ptrs=no_v)
+        vb, _, _ = wc.workspace_size(pb)
+        assert (vb == 0) == (want == wc.OK)
+        rc, _, _ = _ki(wc, pb)
+        assert rc == want, args

@@ -49,7 +49,8 @@ WINO = [
 DIRECT = [
     # n, c, h, w, k, r, s, u, v, p, q
     (2, 12, 15, 17, 20, 5, 5, 2, 2, 2, 2),
-    (2, 24, 9, 9, 40, 1, 1, 1, 1, 0, 0),
+    (2, 24, 9, 9, 40, 1, 1, 1, 1, 0, 0),    # 1x1: the GEMM gathers the input in every layout
+    (2, 16, 13, 11, 24, 3, 3, 2, 2, 1, 1),  # C % 8 == 0: one 16-byte load per packed value group
 ]
 
 
