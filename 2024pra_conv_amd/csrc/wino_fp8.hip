@@ -125,9 +125,12 @@ __global__ void __launch_bounds__(256) wino_filter_fp8(mykernelParamType prm) {
 // Input transform straight to FP8 records. Same workgroup decomposition and LDS staging as
 // the FP16 transform (band of tile rows x 32-channel block x image). Work unit = 16
 // consecutive tiles x the 32 staged channels; lane = (tile lane % 16, channel quarter
-// lane / 16) computes 8 channels x 16 positions in FP32; the four quarters of a tile agree
-// on the block exponents by a cross-lane maximum, then each stores its 8 code bytes per
-// part. A wave's stores cover two contiguous 256-byte runs per part and position.
+// lane / 16) computes 8 channels x 16 positions in FP32, one row of 4 positions at a time
+// (B^T d B row a combines window rows rp(a) and rm(a): the same operations as the full
+// transform, so the same bits, in 68 instead of 180 registers); the
+// four quarters of a tile agree on the block exponents by a cross-lane maximum, then each
+// stores its 8 code bytes per part. A wave's stores cover two contiguous 256-byte runs per
+// part and position.
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) wino_input_transform_fp8(mykernelParamType prm) {
     extern __shared__ __attribute__((aligned(16))) half_t lds[];
@@ -156,50 +159,62 @@ __global__ void __launch_bounds__(256) wino_input_transform_fp8(mykernelParamTyp
         const int g = (grp0 + u) * 16 + (lane & 15);
         const bool valid = g >= gs && g < ge;
         const bool store = g >= gs && g < gend;
-        float v[16][8];
-        if (valid) {
-            const int t = g - n * T;
-            const int th = t / tiles_w;
-            const int tw = t - th * tiles_w;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                float d[16], w[16];
-                load_window(lds, LR, LWp, 2 * (th - th0), 2 * tw + 8, q * 8 + e, d);
-                input_tile_fp32(d, w);
-#pragma unroll
-                for (int pos = 0; pos < 16; ++pos) v[pos][e] = w[pos];
-            }
-        } else {
-#pragma unroll
-            for (int pos = 0; pos < 16; ++pos)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) v[pos][e] = 0.f;
-        }
+        const int t = valid ? g - n * T : 0;
+        const int th = t / tiles_w;
+        const int tw = t - th * tiles_w;
         const int lcode = (q >> 1) * 32 + (g & 31);  // operand lane holding these channels
         const int lscale = part * 32 + (g & 31);     // lane holding this block's scales
+#pragma unroll 1
+        for (int a = 0; a < 4; ++a) {
+            // row a of B^T d B: s_j = d[rp][j] + sg * d[rm][j] (rows 0-2, 1+2, 2-1, 1-3; the
+            // product with sg = +-1 is exact, so the sum rounds once like the subtraction)
+            const int rp = a == 0 ? 0 : (a == 2 ? 2 : 1);
+            const int rm = a == 0 ? 2 : (a == 2 ? 1 : (a == 1 ? 2 : 3));
+            const float sg = a == 1 ? 1.f : -1.f;
+            float v[4][8];
 #pragma unroll
-        for (int pos = 0; pos < 16; ++pos) {
-            float m = 0.f;
+            for (int e = 0; e < 8; ++e) {
+                float sj[4] = {0.f, 0.f, 0.f, 0.f};
+                if (valid) {
+                    const half_t* base = lds + ((q * 8 + e) * LR + 2 * (th - th0)) * LWp + 2 * tw + 8;
+                    const half2_t p0 = *(const half2_t*)(base + rp * LWp), p1 = *(const half2_t*)(base + rp * LWp + 2);
+                    const half2_t m0 = *(const half2_t*)(base + rm * LWp), m1 = *(const half2_t*)(base + rm * LWp + 2);
+                    const float dp[4] = {(float)p0[0], (float)p0[1], (float)p1[0], (float)p1[1]};
+                    const float dm[4] = {(float)m0[0], (float)m0[1], (float)m1[0], (float)m1[1]};
 #pragma unroll
-            for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(v[pos][e]));
-            m = fmaxf(m, __shfl_xor(m, 16));
-            m = fmaxf(m, __shfl_xor(m, 32));
-            const int Ehi = block_exponent(m), Elo = lo_exponent(Ehi);
-            uint32_t hi[2], lo[2];
-            split_hi_lo<8>(v[pos], Ehi, Elo, hi, lo);
-            if (store) {
-                const size_t ri = (size_t)(pos * CQ + (cb >> 1)) * G32 + (g >> 5);  // record index
-                uint8_t* rec = V8 + ri * kFp8CodeBytes;
-                uint8_t* sc = V8 + fp8_scale_offset(prm.c, G32) + ri * kFp8ScaleBytes;
-                uint8_t* dst = rec + part * 1024 + lcode * 16 + (q & 1) * 8;
-                *(uint2*)dst = make_uint2(hi[0], hi[1]);
-                *(uint2*)(dst + 2048) = make_uint2(lo[0], lo[1]);
-                if (q == 0) *(uint16_t*)(sc + lscale * 2) = (uint16_t)((Ehi + 127) | ((Elo + 127) << 8));
-                if (pad_half) {
-                    uint8_t* z = rec + 1024 + lcode * 16 + (q & 1) * 8;
-                    *(uint2*)z = make_uint2(0u, 0u);
-                    *(uint2*)(z + 2048) = make_uint2(0u, 0u);
-                    if (q == 0) *(uint16_t*)(sc + (32 + (g & 31)) * 2) = (uint16_t)0;
+                    for (int j = 0; j < 4; ++j) sj[j] = dp[j] + sg * dm[j];
+                }
+                // columns, as the full transform: s0 - s2, s1 + s2, s2 - s1, s1 - s3
+                v[0][e] = sj[0] - sj[2];
+                v[1][e] = sj[1] + sj[2];
+                v[2][e] = sj[2] - sj[1];
+                v[3][e] = sj[1] - sj[3];
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int pos = 4 * a + b;
+                float m = 0.f;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(v[b][e]));
+                m = fmaxf(m, __shfl_xor(m, 16));
+                m = fmaxf(m, __shfl_xor(m, 32));
+                const int Ehi = block_exponent(m), Elo = lo_exponent(Ehi);
+                uint32_t hi[2], lo[2];
+                split_hi_lo<8>(v[b], Ehi, Elo, hi, lo);
+                if (store) {
+                    const size_t ri = (size_t)(pos * CQ + (cb >> 1)) * G32 + (g >> 5);  // record index
+                    uint8_t* rec = V8 + ri * kFp8CodeBytes;
+                    uint8_t* sc = V8 + fp8_scale_offset(prm.c, G32) + ri * kFp8ScaleBytes;
+                    uint8_t* dst = rec + part * 1024 + lcode * 16 + (q & 1) * 8;
+                    *(uint2*)dst = make_uint2(hi[0], hi[1]);
+                    *(uint2*)(dst + 2048) = make_uint2(lo[0], lo[1]);
+                    if (q == 0) *(uint16_t*)(sc + lscale * 2) = (uint16_t)((Ehi + 127) | ((Elo + 127) << 8));
+                    if (pad_half) {
+                        uint8_t* z = rec + 1024 + lcode * 16 + (q & 1) * 8;
+                        *(uint2*)z = make_uint2(0u, 0u);
+                        *(uint2*)(z + 2048) = make_uint2(0u, 0u);
+                        if (q == 0) *(uint16_t*)(sc + (32 + (g & 31)) * 2) = (uint16_t)0;
+                    }
                 }
             }
         }
