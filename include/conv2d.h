@@ -298,8 +298,8 @@ int conv2d_select_tile_config(const problem_t* problem, int* cfg);
  * conv2d_input_transform, conv2d_gemm and conv2d_forward serve both paths. A 1x1 filter (any
  * stride and padding) has no im2col: its GEMM reads the input itself, slot 2 is the empty
  * kernel too, conv2d_input_transform launches nothing but that kernel, the input workspace
- * (transform_pin) needs 0 bytes, and conv2d_gemm reads the input as N,C,H,W (other layouts
- * through conv2d_forward_io or getkernelInfo_io). */
+ * (transform_pin) needs 0 bytes and may be NULL, and conv2d_gemm reads the input as N,C,H,W
+ * (other layouts through conv2d_forward_io or getkernelInfo_io). */
 int conv2d_method(const problem_t* problem, int* method);
 int conv2d_direct_num_tile_configs(void);
 int conv2d_direct_tile_config(int cfg, int* bk, int* bt, int* threads);
