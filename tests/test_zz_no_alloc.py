@@ -37,7 +37,10 @@ def _stream():
     (dict(n=4, c=64, h=28, w=28, k=128), dict(n=1, c=40, h=13, w=11, k=20), 9),
     # general-geometry path (5x5 stride 2, then 3x3 stride 2 over a smaller input)
     (dict(n=2, c=16, h=20, w=20, k=24, r=5, s=5, stride=2, pad=2), dict(n=1, c=8, h=9, w=9, k=12, stride=2), 0),
-], ids=["staged", "direct"])
+    # 1x1 filters, whose GEMM reads the input itself (8-byte row pieces, then the per-channel
+    # gather of a strided layer)
+    (dict(n=2, c=24, h=8, w=12, k=40, r=1, s=1, pad=0), dict(n=1, c=13, h=9, w=9, k=8, r=1, s=1, stride=2, pad=0), 1),
+], ids=["staged", "direct", "pointwise"])
 def test_static_forward_equals_forward(pair, cuda):
     L = wc.load_library()
     fwd_static, cap = _static_fn(L)
