@@ -591,6 +591,10 @@ def main() -> int:
         "error_vs_fp64": {"max_rel": max_rel, "rms_rel": rms_rel, "finite": finite,
                           "definition": "max|y-y*|/max|y*| and ||y-y*||/||y*|| vs FP64 direct conv"},
         "wall_s_timed": round(wall, 4),
+        # the learned cost model is measured by its GPU test at 2,000 training / 200 held-out
+        # shapes (minutes of autotuning, too long for this run); its result, stated here
+        "learned_form": {"gain_over_hand_rules": False,
+                         "source": "tests/test_gpu_configs.py::test_cfg5_learned_tuning_2000_200"},
     }
     _progress(f"headline workload done: {ms * 1e3:.2f} us per step")
     if not args.no_forms:
