@@ -458,6 +458,14 @@ Launch slot_fused(const Geometry& g) {
 Launch slot_input(const problem_t* pb, const Geometry& g) {
     Launch L;
     if (g.direct && g.pointwise) return slot_noop();  // the GEMM reads the input itself
+    if (g.direct && (g.io & CONV2D_IO_IN_NHWC) == 0 && pb->c % 8 != 0) {
+        // few channels: a wave per 32-pixel block walks every 16-value block
+        L.kernel = (const void*)&direct_pack_input_walk;
+        L.grid = dim3((unsigned)((g.G32 + 3) / 4), 1, 1);
+        L.block = dim3(256, 1, 1);
+        L.lds = 0;
+        return L;
+    }
     if (g.direct) {
         L.kernel = (const void*)&direct_pack_input;
         const long long threads = (long long)g.G32 * 2 * g.nsteps * 64;

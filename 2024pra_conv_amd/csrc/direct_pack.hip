@@ -110,3 +110,49 @@ __global__ void __launch_bounds__(256) direct_pack_input(mykernelParamType prm) 
     }
     *(half8*)((half_t*)prm.transform_pin + ((size_t)L * P32 + pb) * kFragHalves + lane * 8) = v;
 }
+
+// The general gather (C % 8 != 0) with the per-pixel work hoisted: one wave per 32-pixel block,
+// each lane keeping its output pixel and its (tap, channel) position as it walks the 16-value
+// blocks L (no divisions per fragment slot). Same values, same order as direct_pack_input.
+__global__ void __launch_bounds__(256) direct_pack_input_walk(mykernelParamType prm) {
+    const int P32 = prm.G32;
+    const int pb = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (pb >= P32) return;  // whole waves
+    const int lane = threadIdx.x & 63;
+    const int S = prm.s, C = prm.c, H = prm.h, W = prm.w, NT = prm.NT;
+    const int jn = prm.r * S * C;
+    const int g = pb * 32 + (lane & 31);
+    const int img = g < NT ? g / prm.T : 0;
+    const int pix = g - img * prm.T;
+    const int oh = pix / prm.Ow, ow = pix - (pix / prm.Ow) * prm.Ow;
+    const int ih0 = oh * prm.u - prm.p, iw0 = ow * prm.v - prm.q;
+    const half_t* x = (const half_t*)prm.pin + (size_t)img * C * H * W;
+    // (tap, channel) of j = 8 * (lane / 32), stepped 8 values per block and 8 skipped (the other
+    // half-wave's)
+    int j = (lane >> 5) * 8;
+    int c = j % C, ri = (j / C) / S, si = (j / C) % S;
+    half_t* dst = (half_t*)prm.transform_pin + (size_t)pb * kFragHalves + lane * 8;
+    const int nL = 2 * prm.nsteps;
+    for (int L = 0; L < nL; ++L) {
+        half8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int ih = ih0 + ri, iw = iw0 + si;
+            const bool ok = g < NT && j + e < jn && ih >= 0 && ih < H && iw >= 0 && iw < W;
+            v[e] = ok ? x[((size_t)c * H + ih) * W + iw] : (half_t)0.0f;
+            if (++c == C) {
+                c = 0;
+                if (++si == S) si = 0, ++ri;
+            }
+        }
+        *(half8*)(dst + (size_t)L * P32 * kFragHalves) = v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (++c == C) {
+                c = 0;
+                if (++si == S) si = 0, ++ri;
+            }
+        }
+        j += 16;
+    }
+}

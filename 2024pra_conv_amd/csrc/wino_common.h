@@ -76,6 +76,8 @@ __global__ void conv_direct_f64_kernel(mykernelParamType prm, double* out);
 // general-geometry path: packing kernels of the implicit-GEMM direct convolution
 __global__ void direct_pack_weights(mykernelParamType prm);
 __global__ void direct_pack_input(mykernelParamType prm);
+// The same packing for C % 8 != 0 with one wave per 32-pixel block walking every 16-value block.
+__global__ void direct_pack_input_walk(mykernelParamType prm);
 
 // Per-item bodies shared by the per-layer kernels and the multi-layer persistent kernels.
 __device__ void input_transform_block(const mykernelParamType& prm, int n, int cb, int bi, half_t* lds);

@@ -252,6 +252,9 @@ def test_general_geometry_runs_on_the_direct_path(wc, geo, c):
     assert ki.slot(1)["grid"][0] * 256 >= nsteps * 2 * prm.KB32 * 64
     if gather:
         assert ki.slot(2)["kernel"] == ki.slot(4)["kernel"] and ki.slot(2)["grid"] == (1, 1, 1)
+    elif args["c"] % 8:
+        # few channels: one wave per 32-pixel block, walking every 16-value block
+        assert ki.slot(2)["grid"] == (-(-prm.G32 // 4), 1, 1) and ki.slot(2)["block"] == (256, 1, 1)
     else:
         assert ki.slot(2)["grid"][0] * 256 >= nsteps * 2 * prm.G32 * 64
         assert ki.slot(2)["kernel"] != ki.slot(4)["kernel"]
