@@ -10,6 +10,12 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
+#include <functional>
+#include <queue>
+#include <utility>
+#include <vector>
+
 #include "conv2d.h"
 #include "mykernelParamType.h"
 #include "wino_common.h"
@@ -40,6 +46,37 @@ static int input_band(const problem_t* pb, const Geometry& g, bool prefer8) {
         }
     }
     return input_lds_bytes(1, g.tiles_w, (int)pb->w, (int)pb->q) <= 163840 ? 1 : 0;
+}
+
+static int compute_units();
+
+// Finish time of a list schedule on P identical compute units: n1 workgroups of duration d1,
+// then n2 of duration d2, each started on the earliest free unit (units free at the same time
+// are kept as one group, so the loop runs about twice per round).
+static double list_schedule_time(long long n1, double d1, long long n2, double d2, long long P) {
+    typedef std::pair<double, long long> Group;  // (free at, units)
+    std::priority_queue<Group, std::vector<Group>, std::greater<Group>> free_at;
+    free_at.push(Group(0.0, P));
+    double end = 0.0;
+    const long long n[2] = {n1, n2};
+    const double d[2] = {d1, d2};
+    for (int i = 0; i < 2; ++i) {
+        long long left = n[i];
+        while (left > 0) {
+            Group g = free_at.top();
+            free_at.pop();
+            while (!free_at.empty() && free_at.top().first == g.first) {
+                g.second += free_at.top().second;
+                free_at.pop();
+            }
+            const long long take = std::min(g.second, left);
+            left -= take;
+            free_at.push(Group(g.first + d[i], take));
+            end = std::max(end, g.first + d[i]);
+            if (g.second > take) free_at.push(Group(g.first, g.second - take));
+        }
+    }
+    return end;
 }
 
 int derive_geometry(const problem_t* pb, Geometry* g) { return derive_geometry_io(pb, CONV2D_IO_NCHW, g); }
@@ -112,7 +149,21 @@ int derive_geometry_io(const problem_t* pb, int io, Geometry* g) {
         g->fpc = fused_patch_cols(g->flw, g->fbw, g->fsh);
         const int units = 4 * g->fpr * (g->fpc / fused_unit_pixels(g->flw));
         g->fnu = (units + kFusedThreads - 1) / kFusedThreads;
-        g->fitems = (long long)pb->n * g->fnbh * g->fnbw * ((pb->k + 63) / 64);
+        g->fgroup_items = (long long)pb->n * g->fnbh * g->fnbw * ((pb->k + 63) / 64);
+        // a last block of 1..32 channels as wide items (32 channels x 128 tiles, where the
+        // single-layer kernel has them) when a list schedule over the compute units finishes
+        // sooner, counting a full or half-block item as 1 and a wide item as 1.5 (measured
+        // 1.3-1.55 half-block items): N=32 C=K=96 55x55 896 -> 672 items, 42 -> 35 us;
+        // N=32 C=K=32 55x55 13.4 -> 9.9 us; 64x256x27x160 (3 vs 3.5 rounds), 16x128x30x96 and
+        // 64x16x33x32 (3 rounds of half-block items vs 2 of wide ones) keep half-block items,
+        // with which they measured 13%, 33% and 4% faster
+        const long long full = (long long)pb->n * g->fnbh * g->fnbw * (pb->k / 64);
+        const long long wide_items = full + (long long)pb->n * fused_wide_items(g->fnbh, g->fnbw);
+        const long long cus = compute_units();
+        const bool wide = fused_tail(g->KB32) && fused_wide_tail(g->flw, fused_shape_index(g->fbh, g->fbw)) &&
+                          list_schedule_time(full, 1.0, wide_items - full, 1.5, cus) <
+                              list_schedule_time(full, 1.0, g->fgroup_items - full, 1.0, cus);
+        g->fitems = wide ? wide_items : g->fgroup_items;
         // buffer offsets are 32-bit: one input image with 32 channels of slack under 2^31
         // bytes (out-of-image offsets are placed at 2^31 and above), one output image and U
         // under 2^31 bytes
@@ -123,7 +174,7 @@ int derive_geometry_io(const problem_t* pb, int io, Geometry* g) {
         g->fused_ok = (!nhwc_in || nhwc_ok) && g->fnu <= (nhwc_in ? kFusedNhwcUnits : 2) &&
                       (g->flw != kFusedLoad2 || pb->w != 1) &&
                       fused_kernel_ptr(g->flw, fused_shape_index(g->fbh, g->fbw), g->fsh, fused_tail(g->KB32)) &&
-                      g->fitems < (1LL << 31) && img_bytes < (1LL << 31) &&
+                      g->fgroup_items < (1LL << 31) && img_bytes < (1LL << 31) &&
                       out_img_bytes < (1LL << 31) && (long long)g->u_bytes < (1LL << 31);
     }
     g->f43_ok = false;

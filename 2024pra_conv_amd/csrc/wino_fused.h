@@ -40,8 +40,9 @@ constexpr int kFusedNhwcUnits = 4;
 // Patch rows of a block fbh tiles high.
 constexpr int fused_patch_rows(int fbh) { return 2 * fbh + 2; }
 
-// Kernel for staging load width lw, block shape index `shape` and column shift fsh (tail: with
-// the half-block body for the last 64-channel block when K % 64 is 1..32), and its multi-layer
+// Kernel for staging load width lw, block shape index `shape` and column shift fsh (tail: for a
+// last 64-channel block of 1..32 channels, run as wide items, fused_wide_tail, or with the
+// half-block body), and its multi-layer
 // (grouped) form with arguments (const BatchLayer* layers, int nlayers), one workgroup per
 // item, which always has the tail body; null for a combination without an instance (8-byte
 // loads with a patch that needs more than one unit per thread).
@@ -50,3 +51,12 @@ const void* fused_grouped_kernel_ptr(int lw, int shape, int fsh);
 // Whether a layer's last 64-channel block has its second 32 channels beyond K (K % 64 is 1..32,
 // i.e. an odd number of 32-row blocks).
 inline bool fused_tail(int kb32) { return (kb32 & 1) != 0; }
+// Whether the single-layer tail instance for staging load width lw and block shape index
+// `shape` runs the last 64-channel block as wide items (32 channels x a block of fbh x 2*fbw
+// tiles) instead of half-block items: 2- and 4-byte pair staging, whose 128-tile patches fit
+// three staging units per thread, except for the 2 x 32 block (a 2 x 64 patch needs four).
+constexpr bool fused_wide_tail(int lw, int shape) {
+    return (lw == kFusedLoad2 || lw == kFusedLoad4) && shape != 3;
+}
+// Wide tail items per image: fnbh block rows of ceil(fnbw / 2) wide blocks.
+inline long long fused_wide_items(int fnbh, int fnbw) { return (long long)fnbh * ((fnbw + 1) / 2); }

@@ -240,14 +240,16 @@ __device__ inline void nhwc_store(const NhwcStaging& st, half_t* buf) {
     for (int i = 0; i < kFusedNhwcUnits; ++i) *(u32x4*)(buf + st.lds[i]) = st.w[i];
 }
 
-// The staging of one thread for load width LW: 8-byte quads, or up to two 2- or 4-byte pair
-// units (512 threads x 2 units cover the largest pair patch, 816 units).
+// The staging of one thread for load width LW: 8-byte quads, or up to three 2- or 4-byte pair
+// units (512 threads x 2 units cover the largest 64-tile pair patch, 816 units; the 128-tile
+// patches of the wide tail items need three, up to 1,360 units).
 template <int LW, int PR, int PC>
 struct PatchStager {
     static constexpr bool quad = LW == kFusedLoad8;
     static constexpr bool d16 = LW == kFusedLoad2;
     static constexpr bool nhwc = LW == kFusedLoadNHWC;
-    static constexpr int NU = 2;
+    static constexpr int NU = (4 * PR * (PC / 2) + 511) / 512;
+    static_assert(nhwc || quad || (NU >= 2 && NU <= 3), "two or three pair units per thread");
     QuadStaging q;
     Staging<NU> p;
     NhwcStaging n;
@@ -266,12 +268,15 @@ struct PatchStager {
             constexpr int nunits = 4 * PR * (PC / 4);
             if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 < nunits) quad_load(in, chunk_bytes, chan_bytes, q);
         } else {
-            // 648-816 pair units: the second unit set (threads + 512) ends in wave 2-4; the
-            // later waves skip its loads (wave-uniform branch; their stores go to the dummy pair)
+            // 648-1,360 pair units: the last unit set (threads + 512 * (NU - 1)) ends part way
+            // through the workgroup; the later waves skip its loads (wave-uniform branch; their
+            // stores go to the dummy pair)
             constexpr int nunits = 4 * PR * (PC / 2);
             stage_load<1>(in, chunk_bytes, chan_bytes, p, 0);
-            if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 + 512 < nunits)
-                stage_load<1>(in, chunk_bytes, chan_bytes, p, 1);
+#pragma unroll
+            for (int i = 1; i < NU; ++i)
+                if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 + 512 * i < nunits)
+                    stage_load<1>(in, chunk_bytes, chan_bytes, p, i);
         }
     }
     __device__ void store(half_t* buf) const {
@@ -290,10 +295,16 @@ struct PatchStager {
 // (fused_patch_cols, fused_col_shift), so every LDS offset except the lane's two bases
 // (window rows rp, rm) and the chunk's buffer is a compile-time constant. HALF: the block's
 // second 32 output channels lie beyond K (the last block when K % 64 is 1..32); their U loads,
-// MFMAs and epilogue work are left out.
-template <int LW, int BH, int FBH, int FBW, int FSH, bool HALF>
+// MFMAs and epilogue work are left out. WIDE (the same last block, the other way): 32 output
+// channels x a block of FBH x 2*FBW = 128 tiles (gblk then numbers these wide blocks, half as
+// many per block row, rounded up), with the accumulators' channel-block index m standing for
+// the block's two 64-tile halves: one item does the MFMA work of a full one, with half its U
+// loads, where two HALF items would each pay the per-item prologue and epilogue.
+template <int LW, int BH, int FBH, int FBW, int FSH, bool HALF, bool WIDE = false>
 __device__ __attribute__((always_inline)) inline void fused_item(const mykernelParamType& prm, int kblk, int gblk) {
-    constexpr int PR = fused_patch_rows(FBH), PC = fused_patch_cols(LW, FBW, FSH);
+    static_assert(!(HALF && WIDE), "one tail form");
+    constexpr int BW = WIDE ? 2 * FBW : FBW;  // block width in tiles
+    constexpr int PR = fused_patch_rows(FBH), PC = fused_patch_cols(LW, BW, FSH);
     constexpr int PB = 4 * PR * PC * 8;  // halves per patch buffer
     static_assert(FBH * FBW == 64, "64 tiles per block");
     static_assert(2 * PB * 2 + PC * 16 <= 131072, "two patch buffers and the dummy area within the Z area");
@@ -305,11 +316,11 @@ __device__ __attribute__((always_inline)) inline void fused_item(const mykernelP
 
     const int K = prm.k, C = prm.c;
     const int KB = prm.KB32, CB = prm.CB, C16 = 2 * CB;
-    const int nbw = prm.fnbw, nbh = prm.fnbh;
+    const int nbw = WIDE ? (prm.fnbw + 1) / 2 : prm.fnbw, nbh = prm.fnbh;
     const int img = gblk / (nbh * nbw);
     const int brem = gblk - img * (nbh * nbw);
     const int th0 = (brem / nbw) * FBH;
-    const int tw0 = (brem - (brem / nbw) * nbw) * FBW;
+    const int tw0 = (brem - (brem / nbw) * nbw) * BW;
     const int ih0 = 2 * th0 - prm.p;
     const int iw0 = 2 * tw0 - prm.q - FSH;
     const half_t* in_img = (const half_t*)prm.pin + (size_t)img * C * prm.h * prm.w;
@@ -327,7 +338,7 @@ __device__ __attribute__((always_inline)) inline void fused_item(const mykernelP
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
-                for (int m = 0; m < (HALF ? 1 : 2); ++m) {
+                for (int m = 0; m < (HALF || WIDE ? 1 : 2); ++m) {
                     const int kb = kblk * 2 + m < KB ? kblk * 2 + m : 0;
                     const uint32_t frag = ((uint32_t)(4 * a + 2 * BH + bb) * C16 + ch * 2 + ks) * KB + kb;
                     const u32x4 raw =
@@ -341,16 +352,20 @@ __device__ __attribute__((always_inline)) inline void fused_item(const mykernelP
     const int rp = a == 0 ? 0 : (a == 2 ? 2 : 1);
     const int rm = a == 0 ? 2 : (a == 2 ? 1 : (a == 1 ? 2 : 3));
     // LDS offsets (halves) of this lane's window rows rp and rm in channel group lane/32 of
-    // buffer 0: tile t = tb*32 + lane%32 at patch row 2*(t / FBW), column pair t % FBW; window
-    // column j then sits at parity (FSH + j) & 1, position (FSH + j) / 2 past the pair
-    int rbase[2][2];
+    // buffer 0: tile t = 64*mt + tb*32 + lane%32 (mt the 64-tile half, WIDE only) at patch row
+    // 2*(t / BW), column pair t % BW; window column j then sits at parity (FSH + j) & 1, position
+    // (FSH + j) / 2 past the pair
+    constexpr int NMT = WIDE ? 2 : 1;
+    int rbase[NMT][2][2];
 #pragma unroll
-    for (int tb = 0; tb < 2; ++tb) {
-        const int t = tb * 32 + (lane & 31);
-        const int lb = ((lane >> 5) * PR * PC + 2 * (t / FBW) * PC + t % FBW) * 8;
-        rbase[tb][0] = lb + rp * PC * 8;
-        rbase[tb][1] = lb + rm * PC * 8;
-    }
+    for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) {
+            const int t = mt * 64 + tb * 32 + (lane & 31);
+            const int lb = ((lane >> 5) * PR * PC + 2 * (t / BW) * PC + t % BW) * 8;
+            rbase[mt][tb][0] = lb + rp * PC * 8;
+            rbase[mt][tb][1] = lb + rm * PC * 8;
+        }
     const half_t sgh = a == 1 ? (half_t)1.0f : (half_t)-1.0f;
     const half8 sg = {sgh, sgh, sgh, sgh, sgh, sgh, sgh, sgh};
     half8 m1 = {-1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16, -1.f16};
@@ -388,9 +403,11 @@ __device__ __attribute__((always_inline)) inline void fused_item(const mykernelP
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
 #pragma unroll
+            for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
             for (int tb = 0; tb < 2; ++tb) {
-                const half_t* wpp = lds + buf_ofs + rbase[tb][0] + ks * 2 * PR * PC * 8;
-                const half_t* wpm = lds + buf_ofs + rbase[tb][1] + ks * 2 * PR * PC * 8;
+                const half_t* wpp = lds + buf_ofs + rbase[mt][tb][0] + ks * 2 * PR * PC * 8;
+                const half_t* wpm = lds + buf_ofs + rbase[mt][tb][1] + ks * 2 * PR * PC * 8;
                 half8 s3[3];
 #pragma unroll
                 for (int jj = 0; jj < 3; ++jj) {
@@ -414,9 +431,12 @@ __device__ __attribute__((always_inline)) inline void fused_item(const mykernelP
 #pragma unroll
                 for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
-                    for (int m = 0; m < (HALF ? 1 : 2); ++m)
+                    for (int mi = 0; mi < (HALF || WIDE ? 1 : 2); ++mi) {
+                        // WIDE: accumulator block m = the tile half, U of the one channel block
+                        const int m = WIDE ? mt : mi;
                         acc[bb][m][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
-                            ucur[(ks * 2 + bb) * 2 + m], v[bb], (FIRST && ks == 0) ? floatx16{} : acc[bb][m][tb], 0, 0, 0);
+                            ucur[(ks * 2 + bb) * 2 + mi], v[bb], (FIRST && ks == 0) ? floatx16{} : acc[bb][m][tb], 0, 0, 0);
+                    }
                 __builtin_amdgcn_s_setprio(0);
             }
         }
@@ -483,10 +503,11 @@ __device__ __attribute__((always_inline)) inline void fused_item(const mykernelP
         }
     __syncthreads();
     const int blk = wave >> 1;  // output block of this wave, e/4 groups 2*(wave&1) .. +1
-    const int m = blk >> 1, tb = blk & 1;
-    if (HALF && m == 1) return;  // output channels beyond K
-    const int t = tb * 32 + (lane & 31);
-    const int th = th0 + t / FBW, tw = tw0 + t % FBW;
+    const int mb = blk >> 1, tb = blk & 1;
+    if (HALF && mb == 1) return;  // output channels beyond K
+    const int m = WIDE ? 0 : mb;  // 32-channel block of the outputs (WIDE: mb is the tile half)
+    const int t = (WIDE ? mb * 64 : 0) + tb * 32 + (lane & 31);
+    const int th = th0 + t / BW, tw = tw0 + t % BW;
     // Output of this image through a buffer resource: 32-bit byte offsets (the host checks
     // K*Oh*Ow*2 < 2^31), two FP16 values of a row in one 4-byte store when every row starts
     // 4-byte aligned (even Ow, aligned base; then a tile's second column is always inside).
@@ -501,7 +522,7 @@ __device__ __attribute__((always_inline)) inline void fused_item(const mykernelP
     // per-element checks
     const bool out_bf16 = (prm.io & CONV2D_IO_OUT_BF16) != 0;
     const bool fast = (prm.io & CONV2D_IO_OUT_NHWC) == 0 && pairs && (Oh & 1) == 0 && !prm.bias && !prm.act &&
-                      kblk * 64 + 64 <= K;
+                      kblk * 64 + (WIDE ? 32 : 64) <= K;
     // one copy of the stores per output type, so that the type test is not repeated per pair
     auto fast_stores = [&](auto bf_tag) {
         constexpr bool BF = decltype(bf_tag)::value;
@@ -666,9 +687,40 @@ __device__ __attribute__((always_inline)) inline void fused_item_dispatch(const 
     }
 }
 
+// The last 64-channel block as wide items (32 channels x 128 tiles), gblk numbering the wide
+// blocks; K % 64 is 1..32.
+template <int LW, int S>
+__device__ __attribute__((always_inline)) inline void fused_wide_dispatch(const mykernelParamType& prm, int gblk) {
+    constexpr int FBH = BlockShape<S / 4>::h, FBW = BlockShape<S / 4>::w, FSH = S & 3;
+    const bool bh1 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8) != 0;
+    if (!bh1) fused_item<LW, 0, FBH, FBW, FSH, false, true>(prm, prm.k >> 6, gblk);
+    else fused_item<LW, 1, FBH, FBW, FSH, false, true>(prm, prm.k >> 6, gblk);
+}
+
+// Single-layer launch. With TAIL (K % 64 is 1..32) and a wide tail form for this staging and
+// block shape, a grid smaller than one item per (64-channel block, 64-tile block) means the host
+// chose wide tail items: items [0, nmain) are the K / 64 full 64-channel blocks of every
+// 64-tile block and the rest the wide items (fused_wide_items per image), each range remapped
+// per XCD on its own. Otherwise the last block's items take the half-block body.
 template <int LW, int S, bool TAIL>
 __global__ void __launch_bounds__(512, 1) wino_fused_kernel(mykernelParamType prm) {
     int kblk, gblk;
+    if constexpr (TAIL && fused_wide_tail(LW, S / 4)) {
+        const int nblk = prm.n * prm.fnbh * prm.fnbw;
+        const int nfull = prm.k >> 6;
+        const int nmain = nfull * nblk;
+        const int L = blockIdx.x;
+        if ((int)gridDim.x < nmain + nblk) {
+            if (L < nmain) {
+                xcd_remap(L, nmain, nfull, &kblk, &gblk);
+                fused_item_dispatch<LW, S, false>(prm, kblk, gblk);
+            } else {
+                xcd_remap(L - nmain, (int)gridDim.x - nmain, 1, &kblk, &gblk);
+                fused_wide_dispatch<LW, S>(prm, gblk);
+            }
+            return;
+        }
+    }
     xcd_remap(blockIdx.x, gridDim.x, (prm.k + 63) / 64, &kblk, &gblk);
     fused_item_dispatch<LW, S, TAIL>(prm, kblk, gblk);
 }
@@ -691,9 +743,10 @@ int fused_threads() { return kFusedThreads; }
 
 // Instances: 2- and 4-byte staging for every block shape with column shift q & 1; 8-byte
 // staging with the shifts (4 - q % 4) % 4 of q = 0, 1, 2 wherever the quad patch fits one unit
-// per thread (all but the 16 x 4 block with shift 3); each with and without the half-block
-// tail body (layers whose K % 64 is 1..32 take the tail instance). The grouped kernels always
-// carry the tail body (their layers have different K). The tables map (lw, shape, fsh, tail)
+// per thread (all but the 16 x 4 block with shift 3); each with and without a tail body
+// (layers whose K % 64 is 1..32 take the tail instance: wide tail items where fused_wide_tail
+// allows, else the half-block body). The grouped kernels always carry the half-block body
+// (their layers have different K). The tables map (lw, shape, fsh, tail)
 // to them.
 namespace {
 template <int LW, int S>

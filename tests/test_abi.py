@@ -1,5 +1,6 @@
 """C-ABI boundary tests (CPU only: no call here launches a kernel)."""
 import ctypes
+import heapq
 import os
 import re
 import subprocess
@@ -79,6 +80,39 @@ def _ki(wc, pb):
 
 
 FAKE = {"in_": 256, "weight": 256, "out": 256, "transform_pin": 256, "transform_filter": 256}
+
+
+def _list_schedule(n1, d1, n2, d2, units=256):
+    """Finish time of n1 workgroups of duration d1 then n2 of duration d2, each started on the
+    earliest free of `units` compute units (one heap entry per workgroup: small cases only)."""
+    free = [0.0] * units
+    end = 0.0
+    for n, d in ((n1, d1), (n2, d2)):
+        for _ in range(n):
+            t = heapq.heappop(free) + d
+            end = max(end, t)
+            heapq.heappush(free, t)
+    return end
+
+
+def test_list_schedule_restatement():
+    # closed forms: whole rounds of equal workgroups, and a half round of longer ones after them
+    assert _list_schedule(512, 1.0, 0, 1.0) == 2.0 and _list_schedule(513, 1.0, 0, 1.0) == 3.0
+    assert _list_schedule(448, 1.0, 224, 1.5) == 3.5 and _list_schedule(0, 1.0, 384, 1.5) == 3.0
+
+
+def test_wide_tail_items_where_the_schedule_gains(wc):
+    """Fused layers whose last 64-channel block holds 1..32 channels: wide tail items (grid
+    smaller than one item per 64-channel block and 64-tile block) for the measured gains, half-block
+    items for the measured losses and where the kernel has no wide form (8-byte staging)."""
+    cases = [((32, 96, 55, 55, 96), True), ((32, 96, 54, 54, 96), True), ((32, 32, 55, 55, 32), True),
+             ((64, 256, 27, 27, 160), False), ((16, 128, 30, 30, 96), False), ((64, 16, 33, 33, 32), False),
+             ((32, 96, 56, 56, 96), False)]
+    for (n, c, h, w, k), want in cases:
+        rc, ki, prm = _ki(wc, wc.make_problem(n, c, h, w, k, ptrs=FAKE))
+        assert rc == wc.OK and prm.form == wc.FORM_FUSED
+        half = n * prm.fnbh * prm.fnbw * -(-k // 64)
+        assert (ki.slot(2)["grid"][0] < half) == want, (n, c, h, w, k)
 
 
 def test_getkernelinfo_fills_slots_and_param(wc):
@@ -181,7 +215,15 @@ def test_form_choice_and_fused_slots(wc):
         rc, ki, prm = _ki(wc, pb)
         assert rc == wc.OK and prm.form == want
         if want == wc.FORM_FUSED:
-            assert ki.slot(2)["block"] == (512, 1, 1) and ki.slot(2)["grid"][0] == n * prm.fnbh * prm.fnbw * -(-k // 64)
+            # a last block of 1..32 channels runs as wide items (32 channels x two 64-tile blocks
+            # side by side) with pair staging (two units per thread) and a block other than 2 x 32,
+            # when the list schedule on the 256 compute units assumed without a GPU ends sooner
+            half = n * prm.fnbh * prm.fnbw * -(-k // 64)
+            wide = n * prm.fnbh * (prm.fnbw * (k // 64) + (prm.fnbw + 1) // 2)
+            full = n * prm.fnbh * prm.fnbw * (k // 64)
+            use_wide = (0 < k % 64 <= 32 and prm.fnu == 2 and (prm.fbh, prm.fbw) != (2, 32)
+                        and _list_schedule(full, 1.0, wide - full, 1.5) < _list_schedule(full, 1.0, half - full, 1.0))
+            assert ki.slot(2)["block"] == (512, 1, 1) and ki.slot(2)["grid"][0] == (wide if use_wide else half)
             assert ki.slot(3)["grid"] == (1, 1, 1) and ki.slot(4)["grid"] == (1, 1, 1)
             assert ki.slot(2)["lds"] >= 131072
     assert L.conv2d_form(None, ctypes.byref(f)) == wc.ERR_NULL

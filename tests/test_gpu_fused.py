@@ -200,3 +200,80 @@ def test_fused_full_size_against_fp64_gpu(runner, cuda):
     conv.fused_gemm()
     torch.cuda.synchronize()
     assert torch.equal(conv.out.view(torch.int16), fused.view(torch.int16))
+
+
+# Layers whose last 64-channel block (1..32 channels) runs as wide items, 32 channels x two
+# 64-tile blocks side by side: pair staging (2-byte for odd widths, 4-byte otherwise), block
+# shapes 8x8, 4x16 and 16x4, column shifts 0 and 1, K below 64 and above it, enough blocks that
+# the wide items save a round of workgroups on 256 compute units.
+WIDE_SHAPES = [
+    # n, c, h, w, k, p, q
+    (40, 16, 33, 33, 32, 1, 1),    # 8x8, 2-byte, shift 1, wide items only
+    (48, 40, 15, 61, 160, 1, 1),   # 8x8, 2-byte, two full blocks + wide
+    (32, 96, 55, 55, 96, 1, 1),    # 4x16, 2-byte
+    (64, 16, 31, 33, 24, 1, 1),    # 16x4, 2-byte, 24 channels in the last block
+    (40, 40, 30, 62, 96, 1, 1),    # 8x8, 4-byte, shift 1
+    (96, 24, 34, 34, 32, 0, 0),    # 4-byte, shift 0
+]
+
+
+@pytest.mark.parametrize("shape", WIDE_SHAPES)
+def test_wide_tail_items_match_half_block_items(shape, runner, cuda):
+    """The single-layer launch takes wide tail items (its grid is smaller than one item per
+    64-channel block and 64-tile block); a one-layer batch runs the same layer with half-block
+    items in the grouped kernel. Same arithmetic per output: bit-identical outputs."""
+    n, c, h, w, k, p, q = shape
+    print(f"case wide {shape}", flush=True)
+    x = wg.uniform_pm1(1300 + c, (n, c, h, w))
+    wt = wg.uniform_pm1(1307 + c, (k, c, 3, 3))
+    conv = runner.WinogradConv(_to_dev(x, cuda), _to_dev(wt, cuda), pad=(p, q))
+    prm = conv.param()
+    ki, _ = conv.kernel_info()
+    half = n * prm.fnbh * prm.fnbw * -(-k // 64)
+    wide = n * prm.fnbh * (prm.fnbw * (k // 64) + (prm.fnbw + 1) // 2)
+    assert torch.cuda.get_device_properties(0).multi_processor_count == 256
+    assert prm.form == 1 and prm.fnu == 2 and ki.slot(2)["grid"][0] == wide < half, (prm.fbh, prm.fbw)
+    conv.prepare_weights()
+    conv.out.fill_(float("nan"))
+    conv.fused_gemm()
+    torch.cuda.synchronize()
+    y_wide = conv.out.clone()
+    b = runner.Batch([conv])
+    b.prepare_weights()
+    conv.out.fill_(float("nan"))
+    b.forward()
+    torch.cuda.synchronize()
+    assert torch.equal(conv.out.view(torch.int16), y_wide.view(torch.int16))
+    b.close()
+    y64 = conv.direct_f64()
+    d = (y_wide.double() - y64).abs()
+    assert torch.isfinite(y_wide).all() and (d.max() / y64.abs().max()).item() <= 1e-2
+
+
+def test_wide_tail_items_epilogue_and_nhwc_output(runner, cuda):
+    """Wide items through the epilogue paths: NHWC FP16 output equal to the NCHW output
+    transposed, bit for bit; bias + ReLU within the end-to-end bound of the FP64 reference."""
+    n, c, h, w, k, p, q = 64, 16, 31, 33, 24, 1, 1
+    print("case wide epilogue", flush=True)
+    x = _to_dev(wg.uniform_pm1(1400, (n, c, h, w)), cuda)
+    wt = _to_dev(wg.uniform_pm1(1401, (k, c, 3, 3)), cuda)
+    conv = runner.WinogradConv(x, wt, pad=(p, q))
+    conv.prepare_weights()
+    conv.forward()
+    torch.cuda.synchronize()
+    y = conv.out.clone()
+    nhwc = runner.WinogradConv(x, wt, pad=(p, q), io=2)
+    prm = nhwc.param()
+    assert prm.form == 1 and prm.fnu == 2
+    nhwc.prepare_weights()
+    nhwc.out.fill_(float("nan"))
+    nhwc.forward()
+    torch.cuda.synchronize()
+    assert torch.equal(nhwc.out.view(torch.int16), y.permute(0, 2, 3, 1).contiguous().view(torch.int16))
+    bias = (torch.arange(k, device=cuda, dtype=torch.float32) / k - 0.5).half()
+    conv.out.fill_(float("nan"))
+    conv.forward_epilogue(bias, 1)
+    torch.cuda.synchronize()
+    y64 = torch.relu(conv.direct_f64() + bias.double().view(1, k, 1, 1))
+    d = (conv.out.double() - y64).abs()
+    assert torch.isfinite(conv.out).all() and (d.max() / y64.abs().max()).item() <= 1e-2
