@@ -52,11 +52,12 @@ const void* fused_grouped_kernel_ptr(int lw, int shape, int fsh);
 // i.e. an odd number of 32-row blocks).
 inline bool fused_tail(int kb32) { return (kb32 & 1) != 0; }
 // Whether the single-layer tail instance for staging load width lw and block shape index
-// `shape` runs the last 64-channel block as wide items (32 channels x a block of fbh x 2*fbw
-// tiles) instead of half-block items: 2- and 4-byte pair staging, whose 128-tile patches fit
-// three staging units per thread, except for the 2 x 32 block (a 2 x 64 patch needs four).
+// `shape` can run the last 64-channel block as wide items (32 channels x a block of fbh x 2*fbw
+// tiles) instead of half-block items: 2- and 4-byte pair staging (three units per thread for a
+// 128-tile patch) and 8-byte quad staging (two), except for the 2 x 32 block (a 2 x 64 patch
+// needs four pair units).
 constexpr bool fused_wide_tail(int lw, int shape) {
-    return (lw == kFusedLoad2 || lw == kFusedLoad4) && shape != 3;
+    return (lw == kFusedLoad2 || lw == kFusedLoad4 || lw == kFusedLoad8) && shape != 3;
 }
 // Wide tail items per image: fnbh block rows of ceil(fnbw / 2) wide blocks.
 inline long long fused_wide_items(int fnbh, int fnbw) { return (long long)fnbh * ((fnbw + 1) / 2); }

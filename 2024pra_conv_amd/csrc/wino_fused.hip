@@ -147,12 +147,12 @@ struct QuadStaging {
 };
 
 template <int PR, int PC>
-__device__ inline void quad_setup(const mykernelParamType& prm, int ih0, int iw0, QuadStaging& st) {
+__device__ inline void quad_setup(const mykernelParamType& prm, int ih0, int iw0, QuadStaging& st, int set) {
     constexpr int PQ = PC >> 2;
     constexpr int nunits = 4 * PR * PQ;
-    static_assert(PC % 4 == 0 && nunits <= 512, "one quad unit per thread");
+    static_assert(PC % 4 == 0 && nunits <= 1024, "one or two quad units per thread");
     const int H = prm.h, W = prm.w;
-    const int u = threadIdx.x;
+    const int u = threadIdx.x + 512 * set;
     const int g = u / (PR * PQ);
     const int rem = u - g * (PR * PQ);
     const int pr = rem / PQ;
@@ -240,22 +240,27 @@ __device__ inline void nhwc_store(const NhwcStaging& st, half_t* buf) {
     for (int i = 0; i < kFusedNhwcUnits; ++i) *(u32x4*)(buf + st.lds[i]) = st.w[i];
 }
 
-// The staging of one thread for load width LW: 8-byte quads, or up to three 2- or 4-byte pair
-// units (512 threads x 2 units cover the largest 64-tile pair patch, 816 units; the 128-tile
-// patches of the wide tail items need three, up to 1,360 units).
+// The staging of one thread for load width LW: one or two 8-byte quad units, or two or three
+// 2- or 4-byte pair units (512 threads x 2 pair units cover the largest 64-tile pair patch, 816
+// units; the 128-tile patches of the wide tail items need three pair units, up to 1,360, or two
+// quad units, up to 816).
 template <int LW, int PR, int PC>
 struct PatchStager {
     static constexpr bool quad = LW == kFusedLoad8;
     static constexpr bool d16 = LW == kFusedLoad2;
     static constexpr bool nhwc = LW == kFusedLoadNHWC;
     static constexpr int NU = (4 * PR * (PC / 2) + 511) / 512;
+    static constexpr int NQ = quad ? (4 * PR * (PC / 4) + 511) / 512 : 1;
     static_assert(nhwc || quad || (NU >= 2 && NU <= 3), "two or three pair units per thread");
-    QuadStaging q;
+    QuadStaging q[NQ];
     Staging<NU> p;
     NhwcStaging n;
     __device__ void setup(const mykernelParamType& prm, int ih0, int iw0) {
         if constexpr (nhwc) nhwc_setup<PR, PC>(prm, ih0, iw0, n);
-        else if constexpr (quad) quad_setup<PR, PC>(prm, ih0, iw0, q);
+        else if constexpr (quad) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) quad_setup<PR, PC>(prm, ih0, iw0, q[i], i);
+        }
         else stage_setup<NU, d16, 512, PR, PC>(prm, ih0, iw0, p);
     }
     // NHWC: chunk index ch and the channel count; otherwise the chunk's and one channel's
@@ -263,10 +268,14 @@ struct PatchStager {
     __device__ void load_nhwc(__amdgpu_buffer_rsrc_t in, int ch, int C) { nhwc_load(in, ch, C, n); }
     __device__ void load(__amdgpu_buffer_rsrc_t in, uint32_t chunk_bytes, uint32_t chan_bytes) {
         if constexpr (quad) {
-            // 360-432 quad units: the waves past the last unit issue no loads (one wave-uniform
-            // branch around all eight; their stores go to the dummy area)
+            // 360-432 quad units (720-816 for wide items): the waves past the last unit of a set
+            // issue no loads for it (one wave-uniform branch around all eight; their stores go to
+            // the dummy area)
             constexpr int nunits = 4 * PR * (PC / 4);
-            if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 < nunits) quad_load(in, chunk_bytes, chan_bytes, q);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)
+                if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64 + 512 * i < nunits)
+                    quad_load(in, chunk_bytes, chan_bytes, q[i]);
         } else {
             // 648-1,360 pair units: the last unit set (threads + 512 * (NU - 1)) ends part way
             // through the workgroup; the later waves skip its loads (wave-uniform branch; their
@@ -281,7 +290,10 @@ struct PatchStager {
     }
     __device__ void store(half_t* buf) const {
         if constexpr (nhwc) nhwc_store(n, buf);
-        else if constexpr (quad) quad_store<PC>(q, buf);
+        else if constexpr (quad) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) quad_store<PC>(q[i], buf);
+        }
         else stage_store<NU, d16>(p, buf, PC / 2);
     }
 };

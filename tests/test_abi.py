@@ -104,10 +104,10 @@ def test_list_schedule_restatement():
 def test_wide_tail_items_where_the_schedule_gains(wc):
     """Fused layers whose last 64-channel block holds 1..32 channels: wide tail items (grid
     smaller than one item per 64-channel block and 64-tile block) for the measured gains, half-block
-    items for the measured losses and where the kernel has no wide form (8-byte staging)."""
+    items for the measured losses."""
     cases = [((32, 96, 55, 55, 96), True), ((32, 96, 54, 54, 96), True), ((32, 32, 55, 55, 32), True),
              ((64, 256, 27, 27, 160), False), ((16, 128, 30, 30, 96), False), ((64, 16, 33, 33, 32), False),
-             ((32, 96, 56, 56, 96), False)]
+             ((32, 96, 56, 56, 96), True), ((16, 512, 112, 112, 96), True)]
     for (n, c, h, w, k), want in cases:
         rc, ki, prm = _ki(wc, wc.make_problem(n, c, h, w, k, ptrs=FAKE))
         assert rc == wc.OK and prm.form == wc.FORM_FUSED
@@ -216,12 +216,12 @@ def test_form_choice_and_fused_slots(wc):
         assert rc == wc.OK and prm.form == want
         if want == wc.FORM_FUSED:
             # a last block of 1..32 channels runs as wide items (32 channels x two 64-tile blocks
-            # side by side) with pair staging (two units per thread) and a block other than 2 x 32,
+            # side by side) with pair or quad staging and a block other than 2 x 32,
             # when the list schedule on the 256 compute units assumed without a GPU ends sooner
             half = n * prm.fnbh * prm.fnbw * -(-k // 64)
             wide = n * prm.fnbh * (prm.fnbw * (k // 64) + (prm.fnbw + 1) // 2)
             full = n * prm.fnbh * prm.fnbw * (k // 64)
-            use_wide = (0 < k % 64 <= 32 and prm.fnu == 2 and (prm.fbh, prm.fbw) != (2, 32)
+            use_wide = (0 < k % 64 <= 32 and prm.fnu in (1, 2) and (prm.fbh, prm.fbw) != (2, 32)
                         and _list_schedule(full, 1.0, wide - full, 1.5) < _list_schedule(full, 1.0, half - full, 1.0))
             assert ki.slot(2)["block"] == (512, 1, 1) and ki.slot(2)["grid"][0] == (wide if use_wide else half)
             assert ki.slot(3)["grid"] == (1, 1, 1) and ki.slot(4)["grid"] == (1, 1, 1)

@@ -203,9 +203,10 @@ def test_fused_full_size_against_fp64_gpu(runner, cuda):
 
 
 # Layers whose last 64-channel block (1..32 channels) runs as wide items, 32 channels x two
-# 64-tile blocks side by side: pair staging (2-byte for odd widths, 4-byte otherwise), block
-# shapes 8x8, 4x16 and 16x4, column shifts 0 and 1, K below 64 and above it, enough blocks that
-# the wide items save a round of workgroups on 256 compute units.
+# 64-tile blocks side by side: pair staging (2-byte for odd widths, 4-byte for even ones) and
+# quad staging (8-byte, widths a multiple of 4), block shapes 8x8, 4x16 and 16x4, column shifts
+# 0 to 3, K below 64 and above it, enough blocks that the wide items finish sooner on 256
+# compute units.
 WIDE_SHAPES = [
     # n, c, h, w, k, p, q
     (40, 16, 33, 33, 32, 1, 1),    # 8x8, 2-byte, shift 1, wide items only
@@ -214,6 +215,9 @@ WIDE_SHAPES = [
     (64, 16, 31, 33, 24, 1, 1),    # 16x4, 2-byte, 24 channels in the last block
     (40, 40, 30, 62, 96, 1, 1),    # 8x8, 4-byte, shift 1
     (96, 24, 34, 34, 32, 0, 0),    # 4-byte, shift 0
+    (32, 96, 56, 56, 96, 1, 1),    # 4x16, 8-byte, shift 3
+    (40, 40, 32, 60, 96, 1, 2),    # 8x8, 8-byte, shift 2
+    (64, 16, 64, 24, 32, 1, 0),    # 16x4, 8-byte, shift 0
 ]
 
 
@@ -232,7 +236,7 @@ def test_wide_tail_items_match_half_block_items(shape, runner, cuda):
     half = n * prm.fnbh * prm.fnbw * -(-k // 64)
     wide = n * prm.fnbh * (prm.fnbw * (k // 64) + (prm.fnbw + 1) // 2)
     assert torch.cuda.get_device_properties(0).multi_processor_count == 256
-    assert prm.form == 1 and prm.fnu == 2 and ki.slot(2)["grid"][0] == wide < half, (prm.fbh, prm.fbw)
+    assert prm.form == 1 and prm.fnu in (1, 2) and ki.slot(2)["grid"][0] == wide < half, (prm.fbh, prm.fbw)
     conv.prepare_weights()
     conv.out.fill_(float("nan"))
     conv.fused_gemm()
