@@ -99,7 +99,7 @@ int upload(conv2d_batch* b) {
             bl.s2_first = s2;
             bl.s2_items = 0;
             bl.nk = ((int)pb.k + 63) / 64;
-            bl.s3_items = (int)g.fgroup_items;
+            bl.s3_items = (int)g.fitems;  // wide tail items where the single-layer launch has them
             const wino::Launch L = wino::slot_fused(g);
             seg.lds = std::max(seg.lds, L.lds);
         } else {

@@ -27,8 +27,9 @@ struct Geometry {
     bool staged_ok;  // Winograd path with an input-transform band (the staged form applies)
     int flw;    // staging load width (FusedLoad: 2-, 4- or 8-byte loads per channel)
     int fbh, fbw, fnbh, fnbw, fpr, fpc, fsh, fnu;
-    // workgroups of the single-layer launch (wide tail items, when fused_wide_tail applies,
-    // counted once per 128-tile block) and of the layer's share of a grouped launch
+    // workgroups of the layer's launch, single-layer or its share of a grouped one (wide tail
+    // items, when chosen, counted once per 128-tile block), and the count with half-block tail
+    // items (one per 64-channel block and 64-tile block)
     long long fitems, fgroup_items;
     int io;  // CONV2D_IO_* tensor layouts and output type
     // F(4x4,3x3) form: 4x4 output tiles (t4h x t4w per image), input-transform band of band4

@@ -739,7 +739,8 @@ __global__ void __launch_bounds__(512, 1) wino_fused_kernel(mykernelParamType pr
 
 // Multi-layer form: one launch over the fused items of several layers (descriptor i owns items
 // [s3_first, s3_first + s3_items), nk = 64-channel blocks, remapped per XCD within that range),
-// every layer with the same staging load width, block shape and column shift.
+// every layer with the same staging load width, block shape and column shift; a layer's last
+// block runs as wide or half-block items as in its single-layer launch.
 template <int LW, int S>
 __global__ void __launch_bounds__(512, 1)
 wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers) {
@@ -748,6 +749,17 @@ wino_fused_grouped_kernel(const BatchLayer* __restrict__ layers, int nlayers) {
     // the layer's parameters by value, loaded once, as the single-layer kernel receives them
     const mykernelParamType prm = bl.prm;
     const int nk = bl.nk;
+    if constexpr (fused_wide_tail(LW, S / 4)) {
+        // fewer items than (64-channel block, 64-tile block) pairs: the host chose wide tail
+        // items for this layer, after its K / 64 full blocks of every 64-tile block
+        const int nblk = prm.n * prm.fnbh * prm.fnbw;
+        const int nmain = (prm.k >> 6) * nblk;
+        if (bl.s3_items < nk * nblk) {
+            if (local < nmain) fused_item_dispatch<LW, S, false>(prm, local % (prm.k >> 6), local / (prm.k >> 6));
+            else fused_wide_dispatch<LW, S>(prm, local - nmain);
+            return;
+        }
+    }
     fused_item_dispatch<LW, S, true>(prm, local % nk, local / nk);
 }
 

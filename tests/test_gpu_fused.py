@@ -222,10 +222,11 @@ WIDE_SHAPES = [
 
 
 @pytest.mark.parametrize("shape", WIDE_SHAPES)
-def test_wide_tail_items_match_half_block_items(shape, runner, cuda):
+def test_wide_tail_items_match_the_oracle(shape, runner, cuda):
     """The single-layer launch takes wide tail items (its grid is smaller than one item per
-    64-channel block and 64-tile block); a one-layer batch runs the same layer with half-block
-    items in the grouped kernel. Same arithmetic per output: bit-identical outputs."""
+    64-channel block and 64-tile block): within the FP32 bound of the oracle's fused model, like
+    the half-block items of the smaller layers, and bit-identical to the grouped kernel's wide
+    items (a one-layer batch)."""
     n, c, h, w, k, p, q = shape
     print(f"case wide {shape}", flush=True)
     x = wg.uniform_pm1(1300 + c, (n, c, h, w))
@@ -249,6 +250,11 @@ def test_wide_tail_items_match_half_block_items(shape, runner, cuda):
     torch.cuda.synchronize()
     assert torch.equal(conv.out.view(torch.int16), y_wide.view(torch.int16))
     b.close()
+    y = y_wide.cpu().numpy().astype(np.float64)
+    y_model, scale = wg.fused_gpu_model(x, wt, wg.Geometry(n, c, h, w, k, p, q))
+    e = (c + 16) * 2.0 ** -24 * scale
+    tol = 2.0 ** -11 * (np.abs(y_model) + e) + e + 2.0 ** -25
+    assert np.all(np.abs(y - y_model) <= tol), f"max excess {np.max(np.abs(y - y_model) - tol)}"
     y64 = conv.direct_f64()
     d = (y_wide.double() - y64).abs()
     assert torch.isfinite(y_wide).all() and (d.max() / y64.abs().max()).item() <= 1e-2
