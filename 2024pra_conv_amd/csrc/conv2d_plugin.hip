@@ -494,8 +494,9 @@ Launch slot_filter(const Geometry& g) {
     return L;
 }
 
-// Fused form: one workgroup of 8 waves per (64 output channels, 64-tile block); LDS holds two
-// input patches during the channel loop and the 128 KiB Z exchange after it.
+// Fused form: one workgroup of 8 waves per (64 output channels, 64-tile block), or per
+// (32 channels, 128-tile block) for wide tail items (g.fitems counts both); LDS holds two input
+// patches during the channel loop and the 128 KiB Z exchange after it.
 Launch slot_fused(const Geometry& g) {
     Launch L;
     L.kernel = fused_kernel_ptr(g.flw, fused_shape_index(g.fbh, g.fbw), g.fsh, fused_tail(g.KB32));
